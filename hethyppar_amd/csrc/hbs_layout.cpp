@@ -1,0 +1,400 @@
+// hbs_layout.cpp -- host-side layout: Matrix Market reader, vector files, and
+// one rank's shard (row split, loc/shr split, halo send/recv lists).
+//
+// Behaviour follows the reference's L2 layer (src/util/hhp_wrap_bora.c,
+// src/util/hhp_matrix.c, src/util/hhp_cpu.c); the algorithms are this
+// library's own: a single in-memory parse for the reader, and for the shard a
+// direct construction of one rank's lists without the MPI round trip.
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hbs.h"
+#include "hbs_err.h"
+
+namespace {
+
+int lfail(int code)
+{
+    const char *what = code == HBS_ERR_IO       ? "cannot open file"
+                       : code == HBS_ERR_FORMAT ? "malformed or unsupported file"
+                       : code == HBS_ERR_ALLOC  ? "host allocation failed"
+                       : code == HBS_ERR_PARTITION
+                           ? "partition vector out of range, or a rank without rows or entries"
+                           : "bad argument";
+    return hbs::set_error(code, "%s", what);
+}
+
+bool read_file(const char *fname, std::string &buf)
+{
+    FILE *f = std::fopen(fname, "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (sz < 0) {
+        std::fclose(f);
+        return false;
+    }
+    buf.resize((size_t)sz);
+    size_t got = sz ? std::fread(&buf[0], 1, (size_t)sz, f) : 0;
+    std::fclose(f);
+    return got == (size_t)sz;
+}
+
+template <typename T>
+T *hmalloc(size_t n)
+{
+    return static_cast<T *>(std::malloc(sizeof(T) * (n ? n : 1)));
+}
+
+}  // namespace
+
+extern "C" void hbs_host_free(void *p) { std::free(p); }
+
+// Matrix Market coordinate reader.  Semantics of the reference
+// readMatrixMarketFromFile (hhp_wrap_bora.c:13-102): banner checked as
+// mm_read_banner (mmio.c:96-177); complex, array and symmetric are refused
+// (:36-41), and pattern too since its entries carry no value; comment lines
+// skipped before the size line (mmio.c:189-218); 1-based triplets; within a
+// row the entries keep FILE order and duplicates are kept.  Here the file is
+// read in one piece and the triplets bucketed by a stable counting sort.
+extern "C" int hbs_mm_read(const char *fname, int *m, int *n, int *nnz, int **rowptr, int **col,
+                           double **val)
+{
+    if (!fname || !m || !n || !nnz || !rowptr || !col || !val) return lfail(HBS_ERR_ARG);
+    std::string buf;
+    if (!read_file(fname, buf)) return lfail(HBS_ERR_IO);
+    const char *p = buf.c_str();
+    const char *end = p + buf.size();
+    // banner line
+    const char *eol = std::strchr(p, '\n');
+    std::string line(p, eol ? eol : end);
+    char banner[64] = {0}, mtx[64] = {0}, crd[64] = {0}, dtype[64] = {0}, storage[64] = {0};
+    if (std::sscanf(line.c_str(), "%63s %63s %63s %63s %63s", banner, mtx, crd, dtype, storage) != 5)
+        return lfail(HBS_ERR_FORMAT);
+    auto lower = [](char *s) {
+        for (; *s; ++s) *s = (char)std::tolower((unsigned char)*s);
+    };
+    lower(mtx);
+    lower(crd);
+    lower(dtype);
+    lower(storage);
+    if (std::strncmp(banner, "%%MatrixMarket", 14) != 0) return lfail(HBS_ERR_FORMAT);
+    if (std::strcmp(mtx, "matrix") != 0 || std::strcmp(crd, "coordinate") != 0) return lfail(HBS_ERR_FORMAT);
+    if (std::strcmp(dtype, "real") != 0 && std::strcmp(dtype, "integer") != 0) return lfail(HBS_ERR_FORMAT);
+    if (std::strcmp(storage, "general") != 0 && std::strcmp(storage, "hermitian") != 0 &&
+        std::strcmp(storage, "skew-symmetric") != 0)
+        return lfail(HBS_ERR_FORMAT);
+    p = eol ? eol + 1 : end;
+    // skip comment lines, then the size line
+    while (p < end && *p == '%') {
+        const char *e = std::strchr(p, '\n');
+        p = e ? e + 1 : end;
+    }
+    char *q = nullptr;
+    long M = std::strtol(p, &q, 10);
+    if (q == p) return lfail(HBS_ERR_FORMAT);
+    p = q;
+    long N = std::strtol(p, &q, 10);
+    if (q == p) return lfail(HBS_ERR_FORMAT);
+    p = q;
+    long NZ = std::strtol(p, &q, 10);
+    if (q == p) return lfail(HBS_ERR_FORMAT);
+    p = q;
+    if (M < 0 || N < 0 || NZ < 0 || M > 0x7fffffffL || N > 0x7fffffffL || NZ > 0x7fffffffL)
+        return lfail(HBS_ERR_FORMAT);
+    std::vector<int> ri((size_t)NZ), ci((size_t)NZ);
+    std::vector<double> vv((size_t)NZ);
+    for (long k = 0; k < NZ; ++k) {
+        long r = std::strtol(p, &q, 10);
+        if (q == p) return lfail(HBS_ERR_FORMAT);
+        p = q;
+        long c = std::strtol(p, &q, 10);
+        if (q == p) return lfail(HBS_ERR_FORMAT);
+        p = q;
+        double v = std::strtod(p, &q);
+        if (q == p) return lfail(HBS_ERR_FORMAT);
+        p = q;
+        if (r < 1 || r > M || c < 1 || c > N) return lfail(HBS_ERR_FORMAT);
+        ri[(size_t)k] = (int)(r - 1);
+        ci[(size_t)k] = (int)(c - 1);
+        vv[(size_t)k] = v;
+    }
+    int *rp = hmalloc<int>((size_t)M + 1);
+    int *cc = hmalloc<int>((size_t)NZ);
+    double *va = hmalloc<double>((size_t)NZ);
+    if (!rp || !cc || !va) {
+        std::free(rp);
+        std::free(cc);
+        std::free(va);
+        return lfail(HBS_ERR_ALLOC);
+    }
+    std::fill(rp, rp + M + 1, 0);
+    for (long k = 0; k < NZ; ++k) rp[ri[(size_t)k] + 1]++;
+    for (long i = 0; i < M; ++i) rp[i + 1] += rp[i];
+    std::vector<int> fill(rp, rp + M);
+    for (long k = 0; k < NZ; ++k) {   // stable: file order within a row
+        const int d = fill[(size_t)ri[(size_t)k]]++;
+        cc[d] = ci[(size_t)k];
+        va[d] = vv[(size_t)k];
+    }
+    *m = (int)M;
+    *n = (int)N;
+    *nnz = (int)NZ;
+    *rowptr = rp;
+    *col = cc;
+    *val = va;
+    return HBS_OK;
+}
+
+// Whitespace separated doubles, no header (reference vector_read,
+// hhp_cpu.c:411-428).  A short file is an error here instead of a printed
+// warning with uninitialised tail.
+extern "C" int hbs_vector_read(const char *fname, int size, double *out)
+{
+    if (!fname || size < 0 || (size > 0 && !out)) return lfail(HBS_ERR_ARG);
+    std::string buf;
+    if (!read_file(fname, buf)) return lfail(HBS_ERR_IO);
+    const char *p = buf.c_str();
+    char *q = nullptr;
+    for (int i = 0; i < size; ++i) {
+        double v = std::strtod(p, &q);
+        if (q == p) return lfail(HBS_ERR_FORMAT);
+        out[i] = v;
+        p = q;
+    }
+    return HBS_OK;
+}
+
+// "%lf\n" per value (reference vector_write, hhp_cpu.c:430-442).
+extern "C" int hbs_vector_write(const char *fname, int size, const double *v)
+{
+    if (!fname || size < 0 || (size > 0 && !v)) return lfail(HBS_ERR_ARG);
+    FILE *f = std::fopen(fname, "w");
+    if (!f) return lfail(HBS_ERR_IO);
+    for (int i = 0; i < size; ++i)
+        if (std::fprintf(f, "%lf\n", v[i]) < 0) {
+            std::fclose(f);
+            return lfail(HBS_ERR_IO);
+        }
+    std::fclose(f);
+    return HBS_OK;
+}
+
+// y = A x with each row summed from 0.0 in CSR order (the reference
+// CSR_spmxv_seq order, hhp_cpu.c:445-455).
+extern "C" int hbs_host_csr_spmv(int m, const int *rowptr, const int *col, const double *val,
+                                 const double *x, double *y)
+{
+    if (m < 0 || !rowptr || (m > 0 && (!x || !y))) return lfail(HBS_ERR_ARG);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < m; ++i) {
+        double res = 0.0;
+        for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) res += val[j] * x[col[j]];
+        y[i] = res;
+    }
+    return HBS_OK;
+}
+
+extern "C" void hbs_host_shard_free(hbs_host_shard *s)
+{
+    if (!s) return;
+    for (hbs_host_csr *c : {&s->loc, &s->shr}) {
+        std::free(c->I);
+        std::free(c->J);
+        std::free(c->val);
+    }
+    std::free(s->gind.vals);
+    for (hbs_host_comm *c : {&s->send, &s->recv}) {
+        std::free(c->val);
+        std::free(c->J);
+        std::free(c->I);
+        std::free(c->ranks);
+    }
+    std::memset(s, 0, sizeof *s);
+}
+
+// One rank's SHARD_CSC, with the contents the reference's MPI_CSR_split_row
+// (hhp_matrix.c:1249-1338) gives that rank:
+//   rows      owned rows in ascending global order (internal_CSR_split_row,
+//             :820-894);
+//   loc       owned columns, ascending global = local index, SHARD CSC with
+//             row indices ascending inside a column (internal_CSR_loc_split
+//             :896-981 through CSR_to_CSC :1476-1535);
+//   shr       non-owned columns with at least one entry in the owned rows,
+//             ascending global;
+//   gind      [owned globals ; shr globals];
+//   recv      owners ascending, per owner its shr columns ascending
+//             (:1003-1071);
+//   send      peers q ascending; for q the local indices of the owned
+//             columns that q's rows reference, ascending -- the list the
+//             reference obtains by receiving q's recv block and mapping it
+//             through locmap (:1167-1232).
+// Built directly: one pass over all rows collects (peer, local column) pairs,
+// sorted and de-duplicated.  Errors the reference aborts on come back as
+// codes: a partition id out of range (:840-842) or a rank without rows or
+// entries (hhp_cpu.c:19-20, hhp_matrix.c:1484-1486) -> HBS_ERR_PARTITION.
+extern "C" int hbs_shard_build_host(int n, const int *rowptr, const int *col, const double *val,
+                                    const int *partvec, int nparts, int rank, hbs_host_shard *out)
+{
+    if (!out) return lfail(HBS_ERR_ARG);
+    std::memset(out, 0, sizeof *out);
+    if (n <= 0 || !rowptr || !col || !val || !partvec || nparts <= 0 || rank < 0 || rank >= nparts)
+        return lfail(HBS_ERR_ARG);
+    for (int i = 0; i < n; ++i)
+        if (partvec[i] < 0 || partvec[i] >= nparts) return lfail(HBS_ERR_PARTITION);
+    std::vector<int> locmap((size_t)n, -1), owned;
+    for (int i = 0; i < n; ++i)
+        if (partvec[i] == rank) {
+            locmap[(size_t)i] = (int)owned.size();
+            owned.push_back(i);
+        }
+    const int m_loc = (int)owned.size();
+    if (m_loc == 0) return lfail(HBS_ERR_PARTITION);
+    // shr columns: non-owned columns referenced by owned rows
+    std::vector<int> shrmap((size_t)n, -1);
+    long nnz_loc = 0, nnz_shr = 0;
+    std::vector<char> used((size_t)n, 0);
+    for (int lr = 0; lr < m_loc; ++lr) {
+        const int r = owned[(size_t)lr];
+        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            const int c = col[j];
+            if (locmap[(size_t)c] >= 0) {
+                ++nnz_loc;
+            } else {
+                ++nnz_shr;
+                used[(size_t)c] = 1;
+            }
+        }
+    }
+    if (nnz_loc + nnz_shr == 0) return lfail(HBS_ERR_PARTITION);
+    std::vector<int> shrcols;
+    for (int c = 0; c < n; ++c)
+        if (used[(size_t)c]) {
+            shrmap[(size_t)c] = (int)shrcols.size();
+            shrcols.push_back(c);
+        }
+    const int n_shr = (int)shrcols.size();
+
+    hbs_host_shard &s = *out;
+    s.loc.m = m_loc;
+    s.loc.n = m_loc;
+    s.loc.nnz = (int)nnz_loc;
+    s.shr.m = m_loc;
+    s.shr.n = n_shr;
+    s.shr.nnz = (int)nnz_shr;
+    s.loc.I = hmalloc<int>((size_t)m_loc + 1);
+    s.loc.J = hmalloc<int>((size_t)nnz_loc);
+    s.loc.val = hmalloc<double>((size_t)nnz_loc);
+    s.shr.I = hmalloc<int>((size_t)n_shr + 1);
+    s.shr.J = hmalloc<int>((size_t)nnz_shr);
+    s.shr.val = hmalloc<double>((size_t)nnz_shr);
+    s.gind.nvals = (unsigned)(m_loc + n_shr);
+    s.gind.vals = hmalloc<int>((size_t)m_loc + n_shr);
+    if (!s.loc.I || !s.loc.J || !s.loc.val || !s.shr.I || !s.shr.J || !s.shr.val || !s.gind.vals) {
+        hbs_host_shard_free(out);
+        return lfail(HBS_ERR_ALLOC);
+    }
+    std::copy(owned.begin(), owned.end(), s.gind.vals);
+    std::copy(shrcols.begin(), shrcols.end(), s.gind.vals + m_loc);
+    // CSC by counting sort over columns; rows visited in ascending order
+    std::fill(s.loc.I, s.loc.I + m_loc + 1, 0);
+    std::fill(s.shr.I, s.shr.I + n_shr + 1, 0);
+    for (int lr = 0; lr < m_loc; ++lr) {
+        const int r = owned[(size_t)lr];
+        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            const int c = col[j];
+            if (locmap[(size_t)c] >= 0) s.loc.I[locmap[(size_t)c] + 1]++;
+            else s.shr.I[shrmap[(size_t)c] + 1]++;
+        }
+    }
+    for (int c = 0; c < m_loc; ++c) s.loc.I[c + 1] += s.loc.I[c];
+    for (int c = 0; c < n_shr; ++c) s.shr.I[c + 1] += s.shr.I[c];
+    {
+        std::vector<int> fl(s.loc.I, s.loc.I + m_loc), fs(s.shr.I, s.shr.I + n_shr);
+        for (int lr = 0; lr < m_loc; ++lr) {
+            const int r = owned[(size_t)lr];
+            for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+                const int c = col[j];
+                if (locmap[(size_t)c] >= 0) {
+                    const int d = fl[(size_t)locmap[(size_t)c]]++;
+                    s.loc.J[d] = lr;
+                    s.loc.val[d] = val[j];
+                } else {
+                    const int d = fs[(size_t)shrmap[(size_t)c]]++;
+                    s.shr.J[d] = lr;
+                    s.shr.val[d] = val[j];
+                }
+            }
+        }
+    }
+    // recv: owners ascending; per owner its shr columns ascending
+    std::vector<int> ocount((size_t)nparts + 1, 0);
+    for (int k = 0; k < n_shr; ++k) ocount[(size_t)partvec[shrcols[(size_t)k]] + 1]++;
+    int rnum = 0;
+    for (int p = 0; p < nparts; ++p)
+        if (ocount[(size_t)p + 1]) ++rnum;
+    for (int p = 0; p < nparts; ++p) ocount[(size_t)p + 1] += ocount[(size_t)p];
+    s.recv.num = rnum;
+    s.recv.I = hmalloc<int>((size_t)rnum + 1);
+    s.recv.ranks = hmalloc<int>((size_t)rnum);
+    s.recv.J = hmalloc<int>((size_t)n_shr);
+    s.recv.val = static_cast<double *>(std::calloc((size_t)(n_shr ? n_shr : 1), sizeof(double)));
+    if (!s.recv.I || !s.recv.ranks || !s.recv.J || !s.recv.val) {
+        hbs_host_shard_free(out);
+        return lfail(HBS_ERR_ALLOC);
+    }
+    s.recv.I[0] = 0;
+    for (int p = 0, b = 0; p < nparts; ++p)
+        if (ocount[(size_t)p + 1] > ocount[(size_t)p]) {
+            s.recv.ranks[b] = p;
+            s.recv.I[b + 1] = ocount[(size_t)p + 1];
+            ++b;
+        }
+    {
+        std::vector<int> fo(ocount.begin(), ocount.end() - 1);
+        for (int k = 0; k < n_shr; ++k) s.recv.J[fo[(size_t)partvec[shrcols[(size_t)k]]]++] = k;
+    }
+    // send: (peer, local column) pairs referenced by the other ranks' rows
+    std::vector<unsigned long long> pairs;
+    for (int r = 0; r < n; ++r) {
+        const int q = partvec[r];
+        if (q == rank) continue;
+        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            const int lc = locmap[(size_t)col[j]];
+            if (lc >= 0) pairs.push_back(((unsigned long long)(unsigned)q << 32) | (unsigned)lc);
+        }
+    }
+    std::sort(pairs.begin(), pairs.end());
+    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+    int snum = 0;
+    for (size_t k = 0; k < pairs.size(); ++k)
+        if (k == 0 || (pairs[k] >> 32) != (pairs[k - 1] >> 32)) ++snum;
+    s.send.num = snum;
+    s.send.I = hmalloc<int>((size_t)snum + 1);
+    s.send.ranks = hmalloc<int>((size_t)snum);
+    s.send.J = hmalloc<int>(pairs.size());
+    s.send.val = static_cast<double *>(std::calloc(pairs.size() ? pairs.size() : 1, sizeof(double)));
+    if (!s.send.I || !s.send.ranks || !s.send.J || !s.send.val) {
+        hbs_host_shard_free(out);
+        return lfail(HBS_ERR_ALLOC);
+    }
+    s.send.I[0] = 0;
+    int b = -1;
+    for (size_t k = 0; k < pairs.size(); ++k) {
+        const int q = (int)(pairs[k] >> 32);
+        if (b < 0 || s.send.ranks[b] != q) {
+            ++b;
+            s.send.ranks[b] = q;
+        }
+        s.send.J[k] = (int)(pairs[k] & 0xffffffffu);
+        s.send.I[b + 1] = (int)k + 1;
+    }
+    return HBS_OK;
+}

@@ -1,0 +1,570 @@
+"""ctypes binding of libhbs.so (include/hbs.h).
+
+The library is the product; this module only marshals numpy arrays and
+handles.  It raises if libhbs.so is missing: there is no Python or CPU
+fallback for any device operation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libhbs.so")
+
+HBS_OK = 0
+ERRORS = {
+    -1: "HBS_ERR_ARG", -2: "HBS_ERR_HIP", -3: "HBS_ERR_ALLOC", -4: "HBS_ERR_IO",
+    -5: "HBS_ERR_FORMAT", -6: "HBS_ERR_PARTITION", -7: "HBS_ERR_BREAKDOWN",
+    -8: "HBS_ERR_NOCONV", -9: "HBS_ERR_NODEVICE",
+}
+HBS_CSR_F32 = 1
+HBS_CSR_TRANSPOSE = 2
+
+
+class HBSError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class HostCSR(C.Structure):        # CSR/CSC of hhp_common.h:43-58
+    _fields_ = [("I", C.POINTER(C.c_int)), ("J", C.POINTER(C.c_int)),
+                ("val", C.POINTER(C.c_double)), ("nnz", C.c_int), ("m", C.c_int), ("n", C.c_int)]
+
+
+class HostIVec(C.Structure):       # iVector
+    _fields_ = [("vals", C.POINTER(C.c_int)), ("nvals", C.c_uint)]
+
+
+class HostComm(C.Structure):       # COMM
+    _fields_ = [("val", C.POINTER(C.c_double)), ("J", C.POINTER(C.c_int)),
+                ("I", C.POINTER(C.c_int)), ("ranks", C.POINTER(C.c_int)), ("num", C.c_int)]
+
+
+class HostShard(C.Structure):      # SHARD_CSC
+    _fields_ = [("loc", HostCSR), ("shr", HostCSR), ("gind", HostIVec),
+                ("send", HostComm), ("recv", HostComm)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxit", C.c_int), ("fixed_iters", C.c_int),
+                ("check_every", C.c_int), ("use_graph", C.c_int), ("inner_maxit", C.c_int),
+                ("inner_tol", C.c_double), ("max_sweeps", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("stop", C.c_int), ("sweeps", C.c_int), ("status", C.c_int),
+                ("relres", C.c_double), ("rel_r0", C.c_double), ("time_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+_ipp = C.POINTER(_ip)
+_dpp = C.POINTER(_dp)
+
+_PROTOS = {
+    "hbs_last_error": (C.c_char_p, []),
+    "hbs_version": (C.c_char_p, []),
+    "hbs_device_count": (_i, [_ip]),
+    "hbs_ctx_create": (_i, [_i, _vp, C.POINTER(_vp)]),
+    "hbs_ctx_destroy": (_i, [_vp]),
+    "hbs_ctx_sync": (_i, [_vp]),
+    "hbs_ctx_stream": (_vp, [_vp]),
+    "hbs_dev_alloc": (_i, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "hbs_dev_free": (_i, [_vp, _vp]),
+    "hbs_h2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "hbs_d2h": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "hbs_d2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "hbs_dev_zero": (_i, [_vp, _vp, C.c_size_t]),
+    "hbs_csr_create": (_i, [_vp, _i, _i, _i, _ip, _ip, _dp, _i, C.POINTER(_vp)]),
+    "hbs_csr_destroy": (_i, [_vp]),
+    "hbs_csr_info": (_i, [_vp, _ip, _ip, _ip, _ip, _ip]),
+    "hbs_csr_device_arrays": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "hbs_csr_spmv_f64": (_i, [_vp, _d, _vp, _d, _vp]),
+    "hbs_csr_spmv_f32": (_i, [_vp, C.c_float, _vp, C.c_float, _vp]),
+    "hbs_csr_spmv_t_f64": (_i, [_vp, _vp, _vp]),
+    "hbs_csr_spmm_f64": (_i, [_vp, _i, _vp, _vp, _i]),
+    "hbs_csr_spmm_f32": (_i, [_vp, _i, _vp, _vp, _i]),
+    "hbs_dot_f64": (_i, [_vp, _i, _vp, _vp, _dp]),
+    "hbs_axpy_f64": (_i, [_vp, _i, _d, _vp, _vp]),
+    "hbs_scal_f64": (_i, [_vp, _i, _d, _vp]),
+    "hbs_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
+    "hbs_solver_create": (_i, [_vp, C.POINTER(_vp)]),
+    "hbs_solver_destroy": (_i, [_vp]),
+    "hbs_bicgstab_f64": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "hbs_bicgstab_mixed": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "hbs_adjoint_f64": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "hbs_host_free": (None, [_vp]),
+    "hbs_mm_read": (_i, [C.c_char_p, _ip, _ip, _ip, _ipp, _ipp, _dpp]),
+    "hbs_vector_read": (_i, [C.c_char_p, _i, _dp]),
+    "hbs_vector_write": (_i, [C.c_char_p, _i, _dp]),
+    "hbs_gen_convdiff2d": (_i, [_i, _i, _d, _ip, _ip, _ipp, _ipp, _dpp]),
+    "hbs_gen_convdiff3d": (_i, [_i, _i, _i, _d, _ip, _ip, _ipp, _ipp, _dpp]),
+    "hbs_gen_markov": (_i, [_i, _i, _i, _i, _d, C.c_uint64, _ip, _ipp, _ipp, _dpp]),
+    "hbs_gen_circuit": (_i, [_i, _d, _d, _i, _i, _i, _d, C.c_uint64, _ip, _ipp, _ipp, _dpp]),
+    "hbs_gen_fem": (_i, [_i, _i, _d, C.c_uint64, _ip, _ipp, _ipp, _dpp]),
+    "hbs_gen_uniform": (_i, [_i, _d, _d, C.c_uint64, _dp]),
+    "hbs_host_csr_spmv": (_i, [_i, _ip, _ip, _dp, _dp, _dp]),
+    "hbs_shard_build_host": (_i, [_i, _ip, _ip, _dp, _ip, _i, _i, C.POINTER(HostShard)]),
+    "hbs_host_shard_free": (None, [C.POINTER(HostShard)]),
+    "hbs_shard_create": (_i, [_vp, C.POINTER(HostShard), C.POINTER(_vp)]),
+    "hbs_shard_destroy": (_i, [_vp]),
+    "hbs_shard_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
+    "hbs_shard_pack": (_i, [_vp, _vp, _vp]),
+    "hbs_shard_unpack": (_i, [_vp, _vp, _vp]),
+    "hbs_shard_spmv": (_i, [_vp, _vp, _dp, _vp]),
+    "hbs_shard_dot_partial": (_i, [_vp, _vp, _vp, _dp]),
+    "hbs_shard_csr": (_i, [_vp, C.POINTER(_vp)]),
+    "hbs_timer_start": (_i, [_vp]),
+    "hbs_timer_stop": (_i, [_vp, _dp]),
+    "hbs_bench_spmv_f64": (_i, [_vp, _vp, _vp, _i, _dp]),
+    "hbs_bench_spmv_f32": (_i, [_vp, _vp, _vp, _i, _dp]),
+    "hbs_bench_copy": (_i, [_vp, _vp, _vp, C.c_size_t, _i, _dp]),
+    "hbs_set_spmv_plan": (_i, [_i, _i]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libhbs.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m hethyppar_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != HBS_OK:
+        raise HBSError(rc, lib().hbs_last_error().decode())
+    return rc
+
+
+def _ip_of(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _dp_of(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _take(ptr, n, dtype):
+    """Copy a library-malloc'd host array into numpy and free it."""
+    if n == 0:
+        out = np.zeros(0, dtype=dtype)
+    else:
+        out = np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+    lib().hbs_host_free(C.cast(ptr, _vp))
+    return out
+
+
+def _csr_out(call, *args):
+    m = C.c_int()
+    nnz = C.c_int()
+    rp, cc, vv = _ip(), _ip(), _dp()
+    check(call(*args, C.byref(m), C.byref(nnz), C.byref(rp), C.byref(cc), C.byref(vv)))
+    return (_take(rp, m.value + 1, np.int32), _take(cc, nnz.value, np.int32),
+            _take(vv, nnz.value, np.float64))
+
+
+# ---- host-side helpers ------------------------------------------------------
+
+def mm_read(path):
+    """Matrix Market file -> (m, n, rowptr, col, val), per-row file order."""
+    m, n, nnz = C.c_int(), C.c_int(), C.c_int()
+    rp, cc, vv = _ip(), _ip(), _dp()
+    check(lib().hbs_mm_read(os.fsencode(path), C.byref(m), C.byref(n), C.byref(nnz),
+                            C.byref(rp), C.byref(cc), C.byref(vv)))
+    return (m.value, n.value, _take(rp, m.value + 1, np.int32), _take(cc, nnz.value, np.int32),
+            _take(vv, nnz.value, np.float64))
+
+
+def vector_read(path, size):
+    out = np.zeros(size, dtype=np.float64)
+    check(lib().hbs_vector_read(os.fsencode(path), size, _dp_of(out)))
+    return out
+
+
+def vector_write(path, v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    check(lib().hbs_vector_write(os.fsencode(path), v.size, _dp_of(v)))
+
+
+def gen_convdiff2d(nx, ny, conv):
+    m = C.c_int()
+    rp, cc, vv = _ip(), _ip(), _dp()
+    nnz = C.c_int()
+    check(lib().hbs_gen_convdiff2d(nx, ny, conv, C.byref(m), C.byref(nnz), C.byref(rp), C.byref(cc),
+                                   C.byref(vv)))
+    return (_take(rp, m.value + 1, np.int32), _take(cc, nnz.value, np.int32),
+            _take(vv, nnz.value, np.float64))
+
+
+def gen_convdiff3d(nx, ny, nz, conv):
+    m = C.c_int()
+    rp, cc, vv = _ip(), _ip(), _dp()
+    nnz = C.c_int()
+    check(lib().hbs_gen_convdiff3d(nx, ny, nz, conv, C.byref(m), C.byref(nnz), C.byref(rp),
+                                   C.byref(cc), C.byref(vv)))
+    return (_take(rp, m.value + 1, np.int32), _take(cc, nnz.value, np.int32),
+            _take(vv, nnz.value, np.float64))
+
+
+def gen_markov(n, kmin=12, kmax=22, band=None, dom=1.05, seed=2):
+    if band is None:
+        band = max(1, int(round(0.02 * n)))
+    nnz = C.c_int()
+    rp, cc, vv = _ip(), _ip(), _dp()
+    check(lib().hbs_gen_markov(n, kmin, kmax, band, dom, seed, C.byref(nnz), C.byref(rp),
+                               C.byref(cc), C.byref(vv)))
+    return (_take(rp, n + 1, np.int32), _take(cc, nnz.value, np.int32),
+            _take(vv, nnz.value, np.float64))
+
+
+def gen_circuit(n, mean_nnz=4.2, zipf_s=2.1, nheavy=200, heavy_lo=1000, heavy_hi=50000,
+                dom=1.1, seed=3):
+    nnz = C.c_int()
+    rp, cc, vv = _ip(), _ip(), _dp()
+    check(lib().hbs_gen_circuit(n, mean_nnz, zipf_s, nheavy, heavy_lo, heavy_hi, dom, seed,
+                                C.byref(nnz), C.byref(rp), C.byref(cc), C.byref(vv)))
+    return (_take(rp, n + 1, np.int32), _take(cc, nnz.value, np.int32),
+            _take(vv, nnz.value, np.float64))
+
+
+def gen_fem(nblockrows, nblk_per_row=12, dom=1.1, seed=6):
+    nnz = C.c_int()
+    rp, cc, vv = _ip(), _ip(), _dp()
+    check(lib().hbs_gen_fem(nblockrows, nblk_per_row, dom, seed, C.byref(nnz), C.byref(rp),
+                            C.byref(cc), C.byref(vv)))
+    n = 3 * nblockrows
+    return (_take(rp, n + 1, np.int32), _take(cc, nnz.value, np.int32),
+            _take(vv, nnz.value, np.float64))
+
+
+def gen_uniform(n, lo=-1.0, hi=1.0, seed=1):
+    out = np.zeros(n, dtype=np.float64)
+    check(lib().hbs_gen_uniform(n, lo, hi, seed, _dp_of(out)))
+    return out
+
+
+def host_spmv(rowptr, col, val, x):
+    """y = A x summed in CSR order (host; used to form b = A x_target)."""
+    m = rowptr.size - 1
+    y = np.zeros(m, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    check(lib().hbs_host_csr_spmv(m, _ip_of(rowptr), _ip_of(col), _dp_of(val), _dp_of(x), _dp_of(y)))
+    return y
+
+
+def shard_build(rowptr, col, val, partvec, nparts, rank):
+    """One rank's SHARD_CSC (hbs_shard_build_host) as a dict of numpy arrays
+    plus the raw structure (kept alive in the dict under 'raw')."""
+    n = rowptr.size - 1
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    partvec = np.ascontiguousarray(partvec, dtype=np.int32)
+    raw = HostShard()
+    check(lib().hbs_shard_build_host(n, _ip_of(rowptr), _ip_of(col), _dp_of(val), _ip_of(partvec),
+                                     nparts, rank, C.byref(raw)))
+    return shard_to_numpy(raw)
+
+
+def _np(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def shard_to_numpy(raw):
+    def csc(c):
+        return {"I": _np(c.I, c.n + 1, np.int32), "J": _np(c.J, c.nnz, np.int32),
+                "val": _np(c.val, c.nnz, np.float64), "nnz": c.nnz, "m": c.m, "n": c.n}
+
+    def comm(c):
+        tot = _np(c.I, c.num + 1, np.int32)
+        cnt = int(tot[-1]) if c.num > 0 else 0
+        return {"I": tot, "J": _np(c.J, cnt, np.int32), "ranks": _np(c.ranks, c.num, np.int32),
+                "num": c.num}
+
+    d = {"loc": csc(raw.loc), "shr": csc(raw.shr), "gind": _np(raw.gind.vals, raw.gind.nvals, np.int32),
+         "send": comm(raw.send), "recv": comm(raw.recv), "raw": raw}
+    return d
+
+
+def shard_free(d):
+    raw = d.get("raw")
+    if raw is not None:
+        lib().hbs_host_shard_free(C.byref(raw))
+        d["raw"] = None
+
+
+def device_count():
+    n = C.c_int()
+    check(lib().hbs_device_count(C.byref(n)))
+    return n.value
+
+
+# ---- device objects -----------------------------------------------------------
+
+class Context:
+    """One device + one stream (hbs_ctx)."""
+
+    def __init__(self, device=0, stream=None):
+        h = _vp()
+        check(lib().hbs_ctx_create(device, stream, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().hbs_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().hbs_ctx_sync(self.h))
+
+    def empty(self, n, dtype=np.float64):
+        return DeviceArray(self, n, dtype)
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def dot(self, x, y):
+        out = C.c_double()
+        check(lib().hbs_dot_f64(self.h, x.n, x.ptr, y.ptr, C.byref(out)))
+        return out.value
+
+    def timer_start(self):
+        check(lib().hbs_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = C.c_double()
+        check(lib().hbs_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+
+class DeviceArray:
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = _vp()
+        check(lib().hbs_dev_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(lib().hbs_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(_vp), self.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        check(lib().hbs_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        check(lib().hbs_dev_zero(self.ctx.h, self.ptr, self.nbytes))
+
+    def offset(self, k):
+        """Raw device pointer k elements in."""
+        return _vp(self.ptr.value + k * self.dtype.itemsize)
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib().hbs_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceCSR:
+    def __init__(self, ctx, rowptr, col, val, ncols=None, flags=0):
+        self.ctx = ctx
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        m = rowptr.size - 1
+        n = m if ncols is None else ncols
+        h = _vp()
+        check(lib().hbs_csr_create(ctx.h, m, n, col.size, _ip_of(rowptr), _ip_of(col), _dp_of(val),
+                                   flags, C.byref(h)))
+        self.h = h
+        self.m, self.n, self.nnz = m, n, int(col.size)
+
+    @classmethod
+    def wrap(cls, ctx, handle, m, n, nnz):
+        obj = cls.__new__(cls)
+        obj.ctx, obj.h, obj.m, obj.n, obj.nnz = ctx, handle, m, n, nnz
+        obj._borrowed = True
+        return obj
+
+    def info(self):
+        v = [C.c_int() for _ in range(5)]
+        check(lib().hbs_csr_info(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(["m", "n", "nnz", "nblocks", "serial_rows"], [x.value for x in v]))
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0):
+        if x.dtype == np.float32:
+            check(lib().hbs_csr_spmv_f32(self.h, alpha, x.ptr, beta, y.ptr))
+        else:
+            check(lib().hbs_csr_spmv_f64(self.h, alpha, x.ptr, beta, y.ptr))
+
+    def spmv_t(self, x, y):
+        check(lib().hbs_csr_spmv_t_f64(self.h, x.ptr, y.ptr))
+
+    def spmm(self, X, Y, k, tile=0):
+        if X.dtype == np.float32:
+            check(lib().hbs_csr_spmm_f32(self.h, k, X.ptr, Y.ptr, tile))
+        else:
+            check(lib().hbs_csr_spmm_f64(self.h, k, X.ptr, Y.ptr, tile))
+
+    def bench_spmv(self, x, y, reps):
+        ms = C.c_double()
+        if x.dtype == np.float32:
+            check(lib().hbs_bench_spmv_f32(self.h, x.ptr, y.ptr, reps, C.byref(ms)))
+        else:
+            check(lib().hbs_bench_spmv_f64(self.h, x.ptr, y.ptr, reps, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h and not getattr(self, "_borrowed", False):
+            lib().hbs_csr_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_opts(**kw):
+    o = SolveOpts()
+    lib().hbs_solve_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Solver:
+    def __init__(self, A):
+        self.A = A
+        h = _vp()
+        check(lib().hbs_solver_create(A.h, C.byref(h)))
+        self.h = h
+
+    def _run(self, fn, b, x, opts, raise_on_error):
+        info = SolveInfo()
+        rc = fn(self.h, b.ptr, x.ptr, C.byref(opts), C.byref(info))
+        if rc != HBS_OK and raise_on_error:
+            check(rc)
+        return info
+
+    def bicgstab(self, b, x, opts=None, raise_on_error=True, **kw):
+        return self._run(lib().hbs_bicgstab_f64, b, x, opts or default_opts(**kw), raise_on_error)
+
+    def bicgstab_mixed(self, b, x, opts=None, raise_on_error=True, **kw):
+        return self._run(lib().hbs_bicgstab_mixed, b, x, opts or default_opts(**kw), raise_on_error)
+
+    def adjoint(self, g, x, lam, grad_val, opts=None, raise_on_error=True, **kw):
+        info = SolveInfo()
+        o = opts or default_opts(**kw)
+        rc = lib().hbs_adjoint_f64(self.h, g.ptr, x.ptr, lam.ptr,
+                                   grad_val.ptr if grad_val is not None else None,
+                                   C.byref(o), C.byref(info))
+        if rc != HBS_OK and raise_on_error:
+            check(rc)
+        return info
+
+    def close(self):
+        if self.h:
+            lib().hbs_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Shard:
+    """One rank's share on the device (hbs_shard)."""
+
+    def __init__(self, ctx, host_shard):
+        self.ctx = ctx
+        h = _vp()
+        check(lib().hbs_shard_create(ctx.h, C.byref(host_shard["raw"]), C.byref(h)))
+        self.h = h
+        v = [C.c_int() for _ in range(4)]
+        check(lib().hbs_shard_info(h, *[C.byref(x) for x in v]))
+        self.m_loc, self.n_shr, self.n_send, self.nnz = [x.value for x in v]
+
+    def pack(self, x_loc, send):
+        check(lib().hbs_shard_pack(self.h, x_loc.ptr, send.ptr))
+
+    def unpack(self, recv, x_ext):
+        check(lib().hbs_shard_unpack(self.h, recv.ptr, x_ext.ptr))
+
+    def spmv(self, x_ext, y, recv_host=None):
+        rh = None
+        if recv_host is not None:
+            recv_host = np.ascontiguousarray(recv_host, dtype=np.float64)
+            self._keep = recv_host
+            rh = _dp_of(recv_host)
+        check(lib().hbs_shard_spmv(self.h, x_ext.ptr, rh, y.ptr))
+
+    def dot_partial(self, a, b):
+        out = C.c_double()
+        check(lib().hbs_shard_dot_partial(self.h, a.ptr, b.ptr, C.byref(out)))
+        return out.value
+
+    def csr(self):
+        h = _vp()
+        check(lib().hbs_shard_csr(self.h, C.byref(h)))
+        return DeviceCSR.wrap(self.ctx, h, self.m_loc, self.m_loc + self.n_shr, self.nnz)
+
+    def close(self):
+        if self.h:
+            lib().hbs_shard_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_spmv_plan(block_nnz=2048, serial_max_avg=64):
+    check(lib().hbs_set_spmv_plan(block_nnz, serial_max_avg))

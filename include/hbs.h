@@ -1,0 +1,269 @@
+/*
+ * hbs.h -- native C-ABI of the MI355X BiCGStab / SpMV engine (libhbs.so).
+ *
+ * One rank's share of HetHypPar's BiCGStab solve on one AMD Instinct MI355X
+ * (gfx950).  Plain C types only: handles are opaque pointers, arrays are plain
+ * pointers with int sizes, every call returns an int status (HBS_OK == 0,
+ * negative on error; hbs_last_error() gives the message).  Nothing here calls
+ * exit().
+ *
+ * Pointer convention: "host" arguments are host memory; "dev" arguments are
+ * device memory (hbs_dev_alloc or any hipMalloc'd buffer).  Device work is
+ * queued on the context's stream and is asynchronous unless the call returns a
+ * host value.
+ *
+ * The reference-shaped API (device_csr_create, hhp_dp_vecop_P, ...) that a
+ * reference caller links against is in hhp_hip.h and hhp_dp_kernels.h; both
+ * are thin layers over this header.
+ */
+#ifndef HBS_H
+#define HBS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define HBS_OK              0
+#define HBS_ERR_ARG        -1   /* bad argument (null, size, shape)       */
+#define HBS_ERR_HIP        -2   /* HIP runtime error                      */
+#define HBS_ERR_ALLOC      -3   /* host or device allocation failed       */
+#define HBS_ERR_IO         -4   /* file cannot be opened                  */
+#define HBS_ERR_FORMAT     -5   /* malformed or unsupported file          */
+#define HBS_ERR_PARTITION  -6   /* partition vector out of range / empty  */
+#define HBS_ERR_BREAKDOWN  -7   /* BiCGStab breakdown (rho or (r0,v) == 0, non-finite) */
+#define HBS_ERR_NOCONV     -8   /* max iterations reached above tolerance */
+#define HBS_ERR_NODEVICE   -9   /* no HIP device visible                  */
+
+const char *hbs_last_error(void);
+const char *hbs_version(void);
+
+/* ---- context: one device + one stream -------------------------------- */
+typedef struct hbs_ctx_s *hbs_ctx;
+/* stream: a hipStream_t, or NULL to create a private non-blocking stream. */
+int hbs_ctx_create(int device, void *stream, hbs_ctx *out);
+int hbs_ctx_destroy(hbs_ctx ctx);
+int hbs_ctx_sync(hbs_ctx ctx);
+void *hbs_ctx_stream(hbs_ctx ctx);
+int hbs_device_count(int *out);
+
+/* device memory helpers (for C and ctypes callers without their own allocator) */
+int hbs_dev_alloc(hbs_ctx ctx, size_t bytes, void **dev);
+int hbs_dev_free(hbs_ctx ctx, void *dev);
+int hbs_h2d(hbs_ctx ctx, void *dev, const void *host, size_t bytes);   /* synchronous */
+int hbs_d2h(hbs_ctx ctx, void *host, const void *dev, size_t bytes);   /* synchronous */
+int hbs_d2d(hbs_ctx ctx, void *dst, const void *src, size_t bytes);    /* async */
+int hbs_dev_zero(hbs_ctx ctx, void *dev, size_t bytes);                /* async, fill kernel */
+
+/* ---- CSR matrix on the device ---------------------------------------- */
+typedef struct hbs_csr_s *hbs_csr;
+
+/* Deep-copies a host CSR (rowptr[m+1], col[nnz], val[nnz], 0-based int32
+ * indices, FP64 values) to HBM and builds the row-block plan of the CSR-stream
+ * SpMV.  flags: HBS_CSR_F32 also keeps an FP32 copy of the values (mixed
+ * precision); HBS_CSR_TRANSPOSE also keeps A^T as CSR (adjoint).
+ * Replaces device_csr_create (reference src/util/hhp_cuda.c:19-51). */
+#define HBS_CSR_F32       1
+#define HBS_CSR_TRANSPOSE 2
+int hbs_csr_create(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr,
+                   const int *col, const double *val, int flags, hbs_csr *out);
+int hbs_csr_destroy(hbs_csr a);
+/* m, n, nnz, number of row blocks, rows handled by serial (order-exact) blocks */
+int hbs_csr_info(hbs_csr a, int *m, int *n, int *nnz, int *nblocks, int *serial_rows);
+/* Pointers to the device arrays (rowptr, col, val) for callers that need them. */
+int hbs_csr_device_arrays(hbs_csr a, const int **rowptr, const int **col, const double **val);
+
+/* y = alpha*A*x + beta*y (beta == 0 never reads y).  Rows of short-row blocks
+ * are summed in CSR order with separately rounded products, which reproduces
+ * the reference CSR_spmxv_seq (hhp_cpu.c:445-455) bit for bit when alpha == 1
+ * and beta == 0.  Replaces device_csr_spmv (hhp_cuda.c:226-236). */
+int hbs_csr_spmv_f64(hbs_csr a, double alpha, const double *x_dev, double beta, double *y_dev);
+int hbs_csr_spmv_f32(hbs_csr a, float alpha, const float *x_dev, float beta, float *y_dev);
+/* y = A^T x (needs HBS_CSR_TRANSPOSE). */
+int hbs_csr_spmv_t_f64(hbs_csr a, const double *x_dev, double *y_dev);
+
+/* Batched: Y[m x k] = A * X[n x k], row-major blocks (ld = k).  tile = RHS
+ * columns per wavefront pass (0 = autotuned). */
+int hbs_csr_spmm_f64(hbs_csr a, int k, const double *X_dev, double *Y_dev, int tile);
+int hbs_csr_spmm_f32(hbs_csr a, int k, const float *X_dev, float *Y_dev, int tile);
+
+/* ---- BLAS-1 on device vectors (reference device_vector_* shapes) ------ */
+/* dot returns to host (synchronous), like device_vector_dot (hhp_cuda.c:193-195). */
+int hbs_dot_f64(hbs_ctx ctx, int n, const double *x_dev, const double *y_dev, double *out_host);
+int hbs_axpy_f64(hbs_ctx ctx, int n, double a, const double *x_dev, double *y_dev); /* y += a*x */
+int hbs_scal_f64(hbs_ctx ctx, int n, double a, double *x_dev);                       /* x *= a   */
+
+/* ---- BiCGStab -------------------------------------------------------- */
+typedef struct {
+    double tol;          /* stop when ||s|| or ||r|| < tol*||r0||            */
+    int maxit;           /* iteration cap                                    */
+    int fixed_iters;     /* 1: run exactly maxit iterations, no stop test    */
+    int check_every;     /* iterations per convergence poll (0 = default 8)  */
+    int use_graph;       /* 1: replay hipGraph-captured iteration chunks     */
+    int inner_maxit;     /* mixed: FP32 inner iteration cap per sweep        */
+    double inner_tol;    /* mixed: FP32 inner relative tolerance per sweep   */
+    int max_sweeps;      /* mixed: FP64 refinement sweeps cap                */
+} hbs_solve_opts;
+
+typedef struct {
+    int iters;           /* BiCGStab iterations executed (mixed: sum over sweeps) */
+    int stop;            /* 0 max-iter, 1 ||s|| test, 2 ||r|| test, 4 breakdown   */
+    int sweeps;          /* mixed: FP64 refinement sweeps                          */
+    int status;          /* HBS_OK, HBS_ERR_NOCONV or HBS_ERR_BREAKDOWN            */
+    double relres;       /* sqrt(||b-Ax||^2 / ||b||^2), FP64, computed on device  */
+    double rel_r0;       /* ||b-Ax|| / ||r0||                                      */
+    double time_ms;      /* device time of the solve (hipEvent)                    */
+} hbs_solve_info;
+
+void hbs_solve_opts_default(hbs_solve_opts *o);
+
+typedef struct hbs_solver_s *hbs_solver;
+/* Workspace for BiCGStab on A (vectors, device scalars, graphs).  Reusable. */
+int hbs_solver_create(hbs_csr a, hbs_solver *out);
+int hbs_solver_destroy(hbs_solver s);
+/* FP64 BiCGStab, the drop-in for the reference loop (bicgstab_cpu.c:172-255,
+ * bicgstab_gpu_dp.c:145-208) with the stop test of bicgstab_gpu.c:148-150,
+ * 231-236, 289-293.  b_dev, x_dev: device; x holds x0 on entry. */
+int hbs_bicgstab_f64(hbs_solver s, const double *b_dev, double *x_dev,
+                     const hbs_solve_opts *opts, hbs_solve_info *info);
+/* Mixed precision: FP32 BiCGStab sweeps (FP32 values and vectors, int32
+ * indices, compensated dots) inside FP64 iterative refinement; FP64 for
+ * r = b - A x, the scalar reductions and the convergence norm.  Needs
+ * HBS_CSR_F32. */
+int hbs_bicgstab_mixed(hbs_solver s, const double *b_dev, double *x_dev,
+                       const hbs_solve_opts *opts, hbs_solve_info *info);
+/* Adjoint: solve A^T lambda = g (g = dL/dx) with FP64 BiCGStab on the
+ * transpose (HBS_CSR_TRANSPOSE), then dL/db = lambda and
+ * dL/dA_ij = -lambda_i * x_j on A's pattern (grad_val[nnz], CSR order). */
+int hbs_adjoint_f64(hbs_solver s, const double *g_dev, const double *x_dev,
+                    double *lambda_dev, double *grad_val_dev,
+                    const hbs_solve_opts *opts, hbs_solve_info *info);
+
+/* ---- host-side layout (reference L2, src/util/hhp_matrix.c) ----------- */
+/* Host structures with the reference's field layout, so a reference caller's
+ * structs can be passed as they are (hhp_hip.h gives them the reference's
+ * names).  Arrays returned by the calls below are malloc'd by the library and
+ * released with hbs_host_free / hbs_host_shard_free. */
+typedef struct hbs_host_csr {      /* COO/CSR/CSC, hhp_common.h:43-58 */
+    int *I;                        /* CSR: row pointers [m+1]; SHARD CSC: column pointers [n+1] */
+    int *J;                        /* CSR: column indices;     SHARD CSC: row indices          */
+    double *val;
+    int nnz;
+    int m;
+    int n;
+} hbs_host_csr;
+typedef struct hbs_host_ivec {     /* iVector, hhp_common.h:12-15 */
+    int *vals;
+    unsigned int nvals;
+} hbs_host_ivec;
+typedef struct hbs_host_comm {     /* COMM, hhp_common.h:88-134 */
+    double *val;                   /* staging buffer [I[num]]                */
+    int *J;                        /* local indices [I[num]]                 */
+    int *I;                        /* per-peer offsets [num+1]               */
+    int *ranks;                    /* peer ranks, ascending [num]            */
+    int num;
+} hbs_host_comm;
+typedef struct hbs_host_shard {    /* SHARD_CSC, hhp_common.h:165-171 */
+    hbs_host_csr loc;              /* m_loc x m_loc, SHARD CSC               */
+    hbs_host_csr shr;              /* m_loc x n_shr, SHARD CSC               */
+    hbs_host_ivec gind;            /* global index of [loc cols ; shr cols]  */
+    hbs_host_comm send;
+    hbs_host_comm recv;
+} hbs_host_shard;
+void hbs_host_free(void *p);
+
+/* Matrix Market reader (reference readMatrixMarketFromFile,
+ * hhp_wrap_bora.c:13-102): per-row file order kept, duplicates kept. */
+int hbs_mm_read(const char *fname, int *m, int *n, int *nnz,
+                int **rowptr, int **col, double **val);
+/* whitespace separated doubles (reference vector_read, hhp_cpu.c:411-428) */
+int hbs_vector_read(const char *fname, int size, double *out);
+/* "%lf\n" per value (reference vector_write, hhp_cpu.c:430-442) */
+int hbs_vector_write(const char *fname, int size, const double *v);
+
+/* ---- synthetic workloads (SplitMix64 streams, one per row) ----------- */
+/* 2D 5-point convection-diffusion on nx*ny: diag 4+conv, west -1-conv,
+ * east/south/north -1 (Dirichlet neighbours dropped). */
+int hbs_gen_convdiff2d(int nx, int ny, double conv, int *m, int *nnz,
+                       int **rowptr, int **col, double **val);
+/* 3D 7-point: diag 6+conv, -x neighbour -1-conv, others -1. */
+int hbs_gen_convdiff3d(int nx, int ny, int nz, double conv, int *m, int *nnz,
+                       int **rowptr, int **col, double **val);
+/* Markov-like: per row k~U{kmin..kmax} distinct columns in a band of
+ * +-band around i (i excluded) plus one uniform column outside the band,
+ * values U[0,1], diagonal = dom * row sum. */
+int hbs_gen_markov(int n, int kmin, int kmax, int band, double dom, uint64_t seed,
+                   int *nnz, int **rowptr, int **col, double **val);
+/* Circuit-like: truncated-Zipf(zipf_s) off-diagonal counts calibrated to
+ * mean_nnz per row, nheavy rows with log-uniform [heavy_lo, heavy_hi] nnz,
+ * uniform columns, values N(0,1)*10^U(-3,3), diagonal = dom * sum|a_ij|. */
+int hbs_gen_circuit(int n, double mean_nnz, double zipf_s, int nheavy, int heavy_lo,
+                    int heavy_hi, double dom, uint64_t seed,
+                    int *nnz, int **rowptr, int **col, double **val);
+/* FEM-like: 3x3 blocks, nblk_per_row blocks per block row (diagonal, band,
+ * one random coupling), values N(0,1), diagonal shifted to dom * sum|a_ij|. */
+int hbs_gen_fem(int nblockrows, int nblk_per_row, double dom, uint64_t seed,
+                int *nnz, int **rowptr, int **col, double **val);
+/* x ~ U[lo,hi] from seed */
+int hbs_gen_uniform(int n, double lo, double hi, uint64_t seed, double *out);
+/* y = A x on the host in CSR order (the reference CSR_spmxv_seq order); used
+ * to form b = A*x_target for synthetic workloads. */
+int hbs_host_csr_spmv(int m, const int *rowptr, const int *col, const double *val,
+                      const double *x, double *y);
+
+/* ---- one rank's share: shard build, halo pack/unpack, shard SpMV ------ */
+/* Builds rank `rank`'s SHARD_CSC (loc, shr, gind, send, recv) from the global
+ * CSR and partition vector in one process; bit-identical to the reference's
+ * MPI_CSR_split_row (hhp_matrix.c:1249-1338). */
+int hbs_shard_build_host(int n, const int *rowptr, const int *col, const double *val,
+                         const int *partvec, int nparts, int rank,
+                         hbs_host_shard *out);
+void hbs_host_shard_free(hbs_host_shard *s);
+
+typedef struct hbs_shard_s *hbs_shard;
+/* Uploads one rank's share: loc and shr merged into one CSR over the extended
+ * vector [x_loc (loc.n) ; x_shr (shr.n)], plus send.J and recv.J. */
+int hbs_shard_create(hbs_ctx ctx, const hbs_host_shard *host, hbs_shard *out);
+int hbs_shard_destroy(hbs_shard s);
+int hbs_shard_info(hbs_shard s, int *m_loc, int *n_shr, int *n_send, int *nnz);
+/* Halo pack: send_dev[i] = x_loc_dev[send.J[i]] (reference hhp_cpu.c:533-538). */
+int hbs_shard_pack(hbs_shard s, const double *x_loc_dev, double *send_dev);
+/* Halo unpack: x_ext_dev[loc.n + recv.J[p]] = recv_dev[p] (reference CPU
+ * semantics, hhp_cpu.c:574-584). */
+int hbs_shard_unpack(hbs_shard s, const double *recv_dev, double *x_ext_dev);
+/* y_loc = loc*x_loc + shr*x_shr.  x_ext_dev holds x_loc in its first loc.n
+ * entries; recv_host (shr.n values in recv order, e.g. straight from MPI) is
+ * copied to the device and unpacked into x_ext_dev[loc.n..] first.  Pass
+ * recv_host = NULL when x_ext_dev already holds the halo.  Replaces
+ * MPI_device_SHARD_CSC_mpi_spmxv (hhp_cuda.c:259-352) minus the MPI transport. */
+int hbs_shard_spmv(hbs_shard s, double *x_ext_dev, const double *recv_host, double *y_dev);
+/* Partial dot of one rank (the reference then MPI_Allreduces it,
+ * hhp_cuda.c:184-191). */
+int hbs_shard_dot_partial(hbs_shard s, const double *a_dev, const double *b_dev, double *out_host);
+/* The merged local+halo CSR (m_loc x (m_loc + n_shr)) for SpMV benchmarks. */
+int hbs_shard_csr(hbs_shard s, hbs_csr *out);
+
+/* ---- timing helpers for benchmarks (hipEvents on the context stream) -- */
+int hbs_timer_start(hbs_ctx ctx);
+int hbs_timer_stop(hbs_ctx ctx, double *ms);
+/* Average device time of the last `reps` launches of the dominant SpMV
+ * kernel recorded with events around each launch. */
+int hbs_bench_spmv_f64(hbs_csr a, const double *x_dev, double *y_dev, int reps,
+                       double *avg_ms);
+int hbs_bench_spmv_f32(hbs_csr a, const float *x_dev, float *y_dev, int reps,
+                       double *avg_ms);
+int hbs_bench_copy(hbs_ctx ctx, const double *src_dev, double *dst_dev, size_t n,
+                   int reps, double *avg_ms);
+
+/* ---- tuning knobs (process-wide, for sweeps) -------------------------- */
+/* nnz per row block (<= 4096) and the average row length above which a
+ * block uses several lanes per row instead of the order-exact serial sum. */
+int hbs_set_spmv_plan(int block_nnz, int serial_max_avg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,45 @@
+"""The C-ABI library loads (no GPU needed) and exports every function the
+headers in include/ declare."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+from hethyppar_amd import hbs
+
+DECL = re.compile(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", re.M)
+
+
+def declared_functions():
+    names = {}
+    inc = os.path.join(ROOT, "include")
+    for f in sorted(os.listdir(inc)):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        text = re.sub(r"^\s*#[^\n]*", "", text, flags=re.M)
+        text = re.sub(r"typedef[^;]*;", "", text)
+        for m in DECL.finditer(text):
+            names.setdefault(m.group(1), f)
+    return names
+
+
+def test_headers_declare_something():
+    names = declared_functions()
+    assert "hbs_bicgstab_f64" in names and "hbs_shard_spmv" in names
+
+
+def test_every_declared_symbol_is_exported():
+    L = ctypes.CDLL(hbs.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_error_path_without_device():
+    # argument errors come back as codes with a message, never an exit
+    rc = hbs.lib().hbs_csr_destroy(None)
+    assert rc == 0
+    rc = hbs.lib().hbs_set_spmv_plan(100, 1)
+    assert rc == -1 and b"block_nnz" in hbs.lib().hbs_last_error()

@@ -1,0 +1,103 @@
+"""BiCGStab on the MI355X (FP64 drop-in and mixed FP32/FP64 refinement)
+against the oracle's restatement of the reference CPU solver.
+
+Bar (north star): iteration count within +-2 of the oracle's, relative
+residual sqrt(||b-Ax||^2/||b||^2) <= 1e-8 (FP64, with 1% slack for the
+stopping iterate), solution within 1e-6 relative of the oracle's.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from conftest import golden
+from hethyppar_amd import hbs
+from workloads import convdiff64, markov, sample320
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(ctx, rp, c, v, b, x0=None, mixed=False, flags=0, **kw):
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=flags | (hbs.HBS_CSR_F32 if mixed else 0))
+    S = hbs.Solver(A)
+    db = ctx.to_device(b)
+    dx = ctx.to_device(np.zeros_like(b) if x0 is None else x0)
+    info = (S.bicgstab_mixed if mixed else S.bicgstab)(db, dx, **kw)
+    return dx.download(), info
+
+
+def _check(x, info, xo, ito, tol_it=2):
+    print(f"gpu iters {info.iters} stop {info.stop} relres {info.relres:.3e} | oracle iters {ito}", flush=True)
+    assert info.status == 0
+    assert abs(info.iters - ito) <= tol_it
+    assert info.relres <= 1.01e-8
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+def test_convdiff64_fp64(ctx):
+    rp, c, v, xt, b = convdiff64()
+    xo, ito, rro, _ = orc.bicgstab(orc.CSR(rp, c, v), b, tol=1e-8, maxit=1000)
+    x, info = _solve(ctx, rp, c, v, b, tol=1e-8, maxit=1000)
+    _check(x, info, xo, ito)
+
+
+@pytest.mark.parametrize("graph,every", [(0, 1), (1, 4), (1, 8), (0, 16)])
+def test_convdiff64_deterministic_any_schedule(ctx, graph, every):
+    rp, c, v, xt, b = convdiff64()
+    x_ref, i_ref = _solve(ctx, rp, c, v, b, use_graph=1, check_every=8)
+    x, info = _solve(ctx, rp, c, v, b, use_graph=graph, check_every=every)
+    assert info.iters == i_ref.iters
+    assert np.array_equal(x, x_ref)
+
+
+def test_sample320_fixed_iterations_golden(ctx):
+    rp, c, v = sample320()
+    b = hbs.vector_read(golden("sample320", "B.txt"), 320)
+    x0 = hbs.vector_read(golden("sample320", "X_init.txt"), 320)
+    xo, ito, _, _ = orc.bicgstab(orc.CSR(rp, c, v), b, x0, maxit=20, fixed=True)
+    x, info = _solve(ctx, rp, c, v, b, x0=x0, maxit=20, fixed_iters=1)
+    assert info.iters == 20
+    assert np.max(np.abs(x - xo)) < 1e-9
+    for f in ("out_X_seq.txt", "out_X1.txt", "out_X_mpi_gpu.txt"):
+        assert np.max(np.abs(x - hbs.vector_read(golden("sample320", f), 320))) <= 1e-6 + 1e-9
+
+
+def test_markov_445k_fp64(ctx):
+    rp, c, v = markov(445000)
+    xt = hbs.gen_uniform(445000, -1.0, 1.0, 12)
+    b = hbs.host_spmv(rp, c, v, xt)
+    xo, ito, rro, _ = orc.bicgstab(orc.CSR(rp, c, v), b, tol=1e-8, maxit=1000)
+    x, info = _solve(ctx, rp, c, v, b, tol=1e-8, maxit=1000)
+    _check(x, info, xo, ito)
+
+
+def test_circuit_fp64_and_mixed(ctx):
+    rp, c, v = hbs.gen_circuit(150000, nheavy=10, heavy_lo=1000, heavy_hi=20000, seed=3)
+    xt = hbs.gen_uniform(150000, -1.0, 1.0, 13)
+    b = hbs.host_spmv(rp, c, v, xt)
+    xo, ito, rro, _ = orc.bicgstab(orc.CSR(rp, c, v), b, tol=1e-8, maxit=2000)
+    x, info = _solve(ctx, rp, c, v, b, tol=1e-8, maxit=2000)
+    _check(x, info, xo, ito)
+    xm, im = _solve(ctx, rp, c, v, b, mixed=True, tol=1e-8, maxit=4000)
+    print(f"mixed iters {im.iters} sweeps {im.sweeps} relres {im.relres:.3e}", flush=True)
+    assert im.status == 0 and im.relres <= 1.01e-8
+    assert np.linalg.norm(xm - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+def test_mixed_convdiff(ctx):
+    rp, c, v, xt, b = convdiff64()
+    xo, ito, _, _ = orc.bicgstab(orc.CSR(rp, c, v), b, tol=1e-8, maxit=1000)
+    x, info = _solve(ctx, rp, c, v, b, mixed=True, tol=1e-8, maxit=4000)
+    assert info.status == 0 and info.relres <= 1.01e-8 and info.sweeps >= 2
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+def test_not_converged_status(ctx):
+    rp, c, v, xt, b = convdiff64()
+    x, info = _solve(ctx, rp, c, v, b, tol=1e-12, maxit=5, raise_on_error=False)
+    assert info.status == -8 and info.iters == 5 and info.stop == 0
+
+
+def test_zero_rhs(ctx):
+    rp, c, v, xt, b = convdiff64()
+    x, info = _solve(ctx, rp, c, v, np.zeros_like(b), tol=1e-8, maxit=100)
+    assert info.iters == 0 and np.all(x == 0)

@@ -1,0 +1,145 @@
+"""SpMV / SpMM on the MI355X through the C-ABI against the oracle.
+
+Bar: rows of order-exact (serial) row blocks must equal the oracle's
+CSR_spmxv_seq bit for bit; rows summed by several lanes or split over chunks
+must lie within the summation bound 2*gamma_k*sum|a_ij x_j| of the oracle.
+FP32: |y32 - y64| <= 2*gamma32_k * sum|a x| (+ input rounding of a and x).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from conftest import golden
+from hethyppar_amd import hbs
+from workloads import convdiff64, markov, random_csr, sample, sample320, seq_bound
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_spmv(ctx, rp, c, v, x, ncols=None, flags=0):
+    A = hbs.DeviceCSR(ctx, rp, c, v, ncols=ncols, flags=flags)
+    dx = ctx.to_device(x)
+    dy = ctx.empty(rp.size - 1)
+    A.spmv(dx, dy)
+    y = dy.download()
+    return A, y
+
+
+def _cases():
+    rng = np.random.default_rng(7)
+    out = [("sample", *sample()), ("sample320", *sample320())]
+    rp, c, v, xt, b = convdiff64()
+    out.append(("convdiff64", rp, c, v))
+    out.append(("markov50k", *markov(50000)))
+    out.append(("random", *random_csr(3000, 3000, 0.004, 3)))
+    return out
+
+
+@pytest.mark.parametrize("case", _cases(), ids=lambda c: c[0])
+def test_spmv_bit_exact_serial_blocks(ctx, case):
+    name, rp, c, v = case
+    n = rp.size - 1
+    x = np.random.default_rng(1).uniform(-1, 1, n)
+    A, y = _run_spmv(ctx, rp, c, v, x)
+    info = A.info()
+    ref = orc.spmv(orc.CSR(rp, c, v), x)
+    print(name, info, flush=True)
+    assert info["serial_rows"] == n
+    assert np.array_equal(y, ref)
+
+
+def test_spmv_kat_x_mul(ctx):
+    rp, c, v = sample()
+    x = hbs.vector_read(golden("sample", "X_mul.txt"), 16)
+    _, y = _run_spmv(ctx, rp, c, v, x)
+    assert np.max(np.abs(y - np.arange(1, 17))) < 1e-13
+
+
+@pytest.mark.parametrize("block_nnz,serial", [(256, 0), (512, 4), (2048, 64), (4096, 1000)])
+def test_spmv_plans_power_law(ctx, block_nnz, serial):
+    """Power-law rows: chunked long rows, lane-parallel blocks, serial blocks."""
+    rp, c, v = hbs.gen_circuit(60000, nheavy=12, heavy_lo=300, heavy_hi=9000, seed=11)
+    x = np.random.default_rng(2).uniform(-1, 1, rp.size - 1)
+    hbs.set_spmv_plan(block_nnz, serial)
+    try:
+        A, y = _run_spmv(ctx, rp, c, v, x)
+    finally:
+        hbs.set_spmv_plan(2048, 64)
+    ref = orc.spmv(orc.CSR(rp, c, v), x)
+    bound = seq_bound(rp, c, v, x)
+    assert np.all(np.abs(y - ref) <= bound + 1e-300)
+
+
+def test_spmv_alpha_beta_and_empty_rows(ctx):
+    rp, c, v = random_csr(500, 400, 0.01, 9)   # has empty rows
+    assert (np.diff(rp) == 0).any()
+    x = np.random.default_rng(3).standard_normal(400)
+    y0 = np.random.default_rng(4).standard_normal(500)
+    A = hbs.DeviceCSR(ctx, rp, c, v, ncols=400)
+    dx, dy = ctx.to_device(x), ctx.to_device(y0)
+    A.spmv(dx, dy, alpha=-2.0, beta=0.5)
+    ref = orc.spmv(orc.CSR(rp, c, v, ncols=400), x)
+    assert np.allclose(dy.download(), -2.0 * ref + 0.5 * y0, rtol=1e-14, atol=1e-14)
+
+
+def test_spmv_f32(ctx):
+    rp, c, v = markov(30000)
+    x = np.random.default_rng(5).uniform(-1, 1, rp.size - 1)
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    x32 = x.astype(np.float32)
+    dx, dy = ctx.to_device(x32), ctx.empty(rp.size - 1, np.float32)
+    A.spmv(dx, dy)
+    y32 = dy.download().astype(np.float64)
+    ref = orc.spmv(orc.CSR(rp, c, v.astype(np.float32).astype(np.float64)), x32.astype(np.float64))
+    u = np.finfo(np.float32).eps / 2
+    k = np.diff(rp)
+    bound = 2 * (2 * k * u / (1 - k * u)) * orc.spmv(orc.CSR(rp, c, np.abs(v)), np.abs(x))
+    assert np.all(np.abs(y32 - ref) <= bound * 1.01 + 1e-30)
+
+
+def test_spmv_transpose(ctx):
+    rp, c, v = random_csr(700, 700, 0.01, 12, empty_rows=False)
+    x = np.random.default_rng(6).standard_normal(700)
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_TRANSPOSE)
+    dx, dy = ctx.to_device(x), ctx.empty(700)
+    A.spmv_t(dx, dy)
+    import scipy.sparse as sp
+    AT = sp.csr_matrix((v, c, rp), shape=(700, 700)).T.tocsr()
+    AT.sort_indices()
+    ref = orc.spmv(orc.CSR(AT.indptr, AT.indices, AT.data), x)
+    assert np.array_equal(dy.download(), ref)
+
+
+@pytest.mark.parametrize("k,tile", [(64, 0), (64, 64), (64, 16), (8, 0), (5, 4)])
+def test_spmm_columns_bit_exact(ctx, k, tile):
+    rp, c, v, xt, b = convdiff64()
+    n = rp.size - 1
+    X = np.random.default_rng(k).uniform(-1, 1, (n, k))
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    dX, dY = ctx.to_device(X.ravel()), ctx.empty(n * k)
+    A.spmm(dX, dY, k, tile)
+    Y = dY.download().reshape(n, k)
+    Ao = orc.CSR(rp, c, v)
+    for j in range(k):
+        assert np.array_equal(Y[:, j], orc.spmv(Ao, X[:, j]))
+
+
+def test_spmm_f32(ctx):
+    rp, c, v, xt, b = convdiff64()
+    n, k = rp.size - 1, 32
+    X = np.random.default_rng(1).uniform(-1, 1, (n, k)).astype(np.float32)
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    dX, dY = ctx.to_device(X.ravel()), ctx.empty(n * k, np.float32)
+    A.spmm(dX, dY, k, 0)
+    Y = dY.download().reshape(n, k).astype(np.float64)
+    ref = (orc.spmv(orc.CSR(rp, c, v), X[:, 3].astype(np.float64)))
+    assert np.allclose(Y[:, 3], ref, rtol=1e-5, atol=1e-5)
+
+
+def test_dot_and_blas1(ctx):
+    x = np.random.default_rng(1).standard_normal(100003)
+    y = np.random.default_rng(2).standard_normal(100003)
+    dx, dy = ctx.to_device(x), ctx.to_device(y)
+    d = ctx.dot(dx, dy)
+    assert abs(d - float(x @ y)) <= 1e-12 * float(np.abs(x) @ np.abs(y))
+    assert ctx.dot(dx, dy) == d   # deterministic reduction order
