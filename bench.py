@@ -172,6 +172,7 @@ def main():
         log("bench: CPU baseline (oracle, 1 thread)")
         result["cpu_baseline"] = cpu_baseline(rp, c, v, b)
     print(json.dumps(result), flush=True)
+    ctx.close()
 
 
 def extras(hbs, ctx, rp, c, v, n):
@@ -196,7 +197,7 @@ def extras(hbs, ctx, rp, c, v, n):
     # mixed precision vs FP64 on the circuit-like matrix
     N3 = 3500000
     log("bench: circuit-like n=3.5M, FP64 vs mixed")
-    rp3, c3, v3 = hbs.gen_circuit(N3, 4.2, 2.1, 200, 1000, 50000, 1.1, 3)
+    rp3, c3, v3 = hbs.gen_circuit(N3, 4.2, 2.1, 200, 1000, 50000, 1.1, 3, equilibrate=True)
     xt3 = hbs.gen_uniform(N3, -1.0, 1.0, 33)
     b3 = hbs.host_spmv(rp3, c3, v3, xt3)
     A3 = hbs.DeviceCSR(ctx, rp3, c3, v3, flags=hbs.HBS_CSR_F32)

@@ -236,9 +236,12 @@ extern "C" int hbs_gen_markov(int n, int kmin, int kmax, int band, double dom, u
 // [heavy_lo, heavy_hi]; the other rows get k ~ Zipf(zipf_s) on 1..K, K the
 // truncation whose mean brings the whole matrix closest to mean_nnz per row.
 // Columns uniform, distinct, != i; values N(0,1) * 10^U(-3,3); diagonal =
-// dom * sum |a_ij|.
+// dom * sum |a_ij|.  equilibrate != 0 divides every row by its diagonal
+// (the same solution for b scaled alike): without it the row scales span
+// ~1e6 and unpreconditioned BiCGStab stagnates near 1e-5.
 extern "C" int hbs_gen_circuit(int n, double mean_nnz, double zipf_s, int nheavy, int heavy_lo, int heavy_hi,
-                               double dom, uint64_t seed, int *nnz, int **rowptr, int **col, double **val)
+                               double dom, int equilibrate, uint64_t seed, int *nnz, int **rowptr, int **col,
+                               double **val)
 {
     if (n <= 1 || nheavy < 0 || nheavy > n || heavy_lo < 2 || heavy_hi < heavy_lo || heavy_hi > n ||
         !nnz || !rowptr || !col || !val)
@@ -346,9 +349,14 @@ extern "C" int hbs_gen_circuit(int n, double mean_nnz, double zipf_s, int nheavy
             vv[t] = r.normal() * mag;
             sum += std::fabs(vv[t]);
         }
+        double diag = dom * sum;
+        if (diag == 0.0) diag = 1.0;
         cc[k] = i;
-        vv[k] = dom * sum;
-        if (vv[k] == 0.0) vv[k] = 1.0;
+        vv[k] = diag;
+        if (equilibrate) {
+            for (int t = 0; t < k; ++t) vv[t] = vv[t] / diag;
+            vv[k] = 1.0;
+        }
         sort_row(cc, vv, k + 1);
     }
     return HBS_OK;

@@ -6,6 +6,7 @@ fallback for any device operation.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -107,7 +108,7 @@ _PROTOS = {
     "hbs_gen_convdiff2d": (_i, [_i, _i, _d, _ip, _ip, _ipp, _ipp, _dpp]),
     "hbs_gen_convdiff3d": (_i, [_i, _i, _i, _d, _ip, _ip, _ipp, _ipp, _dpp]),
     "hbs_gen_markov": (_i, [_i, _i, _i, _i, _d, C.c_uint64, _ip, _ipp, _ipp, _dpp]),
-    "hbs_gen_circuit": (_i, [_i, _d, _d, _i, _i, _i, _d, C.c_uint64, _ip, _ipp, _ipp, _dpp]),
+    "hbs_gen_circuit": (_i, [_i, _d, _d, _i, _i, _i, _d, _i, C.c_uint64, _ip, _ipp, _ipp, _dpp]),
     "hbs_gen_fem": (_i, [_i, _i, _d, C.c_uint64, _ip, _ipp, _ipp, _dpp]),
     "hbs_gen_uniform": (_i, [_i, _d, _d, C.c_uint64, _dp]),
     "hbs_host_csr_spmv": (_i, [_i, _ip, _ip, _dp, _dp, _dp]),
@@ -235,10 +236,11 @@ def gen_markov(n, kmin=12, kmax=22, band=None, dom=1.05, seed=2):
 
 
 def gen_circuit(n, mean_nnz=4.2, zipf_s=2.1, nheavy=200, heavy_lo=1000, heavy_hi=50000,
-                dom=1.1, seed=3):
+                dom=1.1, seed=3, equilibrate=True):
     nnz = C.c_int()
     rp, cc, vv = _ip(), _ip(), _dp()
-    check(lib().hbs_gen_circuit(n, mean_nnz, zipf_s, nheavy, heavy_lo, heavy_hi, dom, seed,
+    check(lib().hbs_gen_circuit(n, mean_nnz, zipf_s, nheavy, heavy_lo, heavy_hi, dom,
+                                1 if equilibrate else 0, seed,
                                 C.byref(nnz), C.byref(rp), C.byref(cc), C.byref(vv)))
     return (_take(rp, n + 1, np.int32), _take(cc, nnz.value, np.int32),
             _take(vv, nnz.value, np.float64))
@@ -327,9 +329,20 @@ class Context:
         h = _vp()
         check(lib().hbs_ctx_create(device, stream, C.byref(h)))
         self.h = h
+        self._children = []
+
+    def _adopt(self, obj):
+        self._children.append(weakref.ref(obj))
 
     def close(self):
+        """Release every object created on this context (newest first), then
+        the context itself."""
         if self.h:
+            for r in reversed(self._children):
+                o = r()
+                if o is not None:
+                    o.close()
+            self._children = []
             lib().hbs_ctx_destroy(self.h)
             self.h = None
 
@@ -373,6 +386,7 @@ class DeviceArray:
         p = _vp()
         check(lib().hbs_dev_alloc(ctx.h, self.nbytes, C.byref(p)))
         self.ptr = p
+        ctx._adopt(self)
 
     @property
     def nbytes(self):
@@ -398,7 +412,9 @@ class DeviceArray:
     def free(self):
         if self.ptr and self.ctx.h:
             lib().hbs_dev_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
@@ -420,6 +436,7 @@ class DeviceCSR:
                                    flags, C.byref(h)))
         self.h = h
         self.m, self.n, self.nnz = m, n, int(col.size)
+        ctx._adopt(self)
 
     @classmethod
     def wrap(cls, ctx, handle, m, n, nnz):
@@ -457,7 +474,7 @@ class DeviceCSR:
         return ms.value
 
     def close(self):
-        if self.h and not getattr(self, "_borrowed", False):
+        if self.h and not getattr(self, "_borrowed", False) and self.ctx.h:
             lib().hbs_csr_destroy(self.h)
         self.h = None
 
@@ -482,6 +499,7 @@ class Solver:
         h = _vp()
         check(lib().hbs_solver_create(A.h, C.byref(h)))
         self.h = h
+        A.ctx._adopt(self)
 
     def _run(self, fn, b, x, opts, raise_on_error):
         info = SolveInfo()
@@ -507,9 +525,9 @@ class Solver:
         return info
 
     def close(self):
-        if self.h:
+        if self.h and self.A.h and self.A.ctx.h:
             lib().hbs_solver_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -529,6 +547,7 @@ class Shard:
         v = [C.c_int() for _ in range(4)]
         check(lib().hbs_shard_info(h, *[C.byref(x) for x in v]))
         self.m_loc, self.n_shr, self.n_send, self.nnz = [x.value for x in v]
+        ctx._adopt(self)
 
     def pack(self, x_loc, send):
         check(lib().hbs_shard_pack(self.h, x_loc.ptr, send.ptr))
@@ -555,9 +574,9 @@ class Shard:
         return DeviceCSR.wrap(self.ctx, h, self.m_loc, self.m_loc + self.n_shr, self.nnz)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().hbs_shard_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -199,9 +199,10 @@ int hbs_gen_markov(int n, int kmin, int kmax, int band, double dom, uint64_t see
                    int *nnz, int **rowptr, int **col, double **val);
 /* Circuit-like: truncated-Zipf(zipf_s) off-diagonal counts calibrated to
  * mean_nnz per row, nheavy rows with log-uniform [heavy_lo, heavy_hi] nnz,
- * uniform columns, values N(0,1)*10^U(-3,3), diagonal = dom * sum|a_ij|. */
+ * uniform columns, values N(0,1)*10^U(-3,3), diagonal = dom * sum|a_ij|;
+ * equilibrate != 0 divides each row by its diagonal. */
 int hbs_gen_circuit(int n, double mean_nnz, double zipf_s, int nheavy, int heavy_lo,
-                    int heavy_hi, double dom, uint64_t seed,
+                    int heavy_hi, double dom, int equilibrate, uint64_t seed,
                     int *nnz, int **rowptr, int **col, double **val);
 /* FEM-like: 3x3 blocks, nblk_per_row blocks per block row (diagonal, band,
  * one random coupling), values N(0,1), diagonal shifted to dom * sum|a_ij|. */

@@ -78,17 +78,28 @@ def test_circuit_fp64_and_mixed(ctx):
     x, info = _solve(ctx, rp, c, v, b, tol=1e-8, maxit=2000)
     _check(x, info, xo, ito)
     xm, im = _solve(ctx, rp, c, v, b, mixed=True, tol=1e-8, maxit=4000)
-    print(f"mixed iters {im.iters} sweeps {im.sweeps} relres {im.relres:.3e}", flush=True)
+    _check_mixed(xm, im, xo, xt)
+
+
+def _check_mixed(xm, im, xo, xt):
+    """Mixed precision follows another iteration path, so its iterate is not
+    the oracle's; the bar is the FP64 residual (<= 1e-8) and a forward error
+    no worse than the FP64 oracle solve's (x_target known)."""
+    e_m = np.linalg.norm(xm - xt) / np.linalg.norm(xt)
+    e_o = np.linalg.norm(xo - xt) / np.linalg.norm(xt)
+    d = np.linalg.norm(xm - xo) / np.linalg.norm(xo)
+    print(f"mixed iters {im.iters} sweeps {im.sweeps} relres {im.relres:.3e} | "
+          f"err vs target {e_m:.2e} (oracle {e_o:.2e}), vs oracle {d:.2e}", flush=True)
     assert im.status == 0 and im.relres <= 1.01e-8
-    assert np.linalg.norm(xm - xo) / np.linalg.norm(xo) <= 1e-6
+    assert e_m <= 2 * max(e_o, 1e-7)
 
 
 def test_mixed_convdiff(ctx):
     rp, c, v, xt, b = convdiff64()
     xo, ito, _, _ = orc.bicgstab(orc.CSR(rp, c, v), b, tol=1e-8, maxit=1000)
     x, info = _solve(ctx, rp, c, v, b, mixed=True, tol=1e-8, maxit=4000)
-    assert info.status == 0 and info.relres <= 1.01e-8 and info.sweeps >= 2
-    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+    assert info.sweeps >= 2
+    _check_mixed(x, info, xo, xt)
 
 
 def test_not_converged_status(ctx):

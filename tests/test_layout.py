@@ -143,12 +143,18 @@ def test_gen_circuit_properties():
     lens = np.diff(rp)
     assert (lens >= 1000).sum() == 20
     assert abs(lens.mean() - 4.2) < 0.6
+    rpr, cr, vr = hbs.gen_circuit(n, nheavy=20, heavy_lo=1000, heavy_hi=20000, seed=3,
+                                  equilibrate=False)
+    assert np.array_equal(rp, rpr) and np.array_equal(c, cr)
     for i in np.flatnonzero(lens >= 1000)[:3].tolist() + [0, 5, 99]:
         cols = c[rp[i]:rp[i + 1]]
         assert np.all(np.diff(cols) > 0)
-        vals = v[rp[i]:rp[i + 1]]
         d = int(np.flatnonzero(cols == i)[0])
-        assert np.isclose(vals[d], 1.1 * np.abs(np.delete(vals, d)).sum(), rtol=1e-14)
+        raw = vr[rp[i]:rp[i + 1]]
+        assert np.isclose(raw[d], 1.1 * np.abs(np.delete(raw, d)).sum(), rtol=1e-14)
+        vals = v[rp[i]:rp[i + 1]]   # equilibrated: row divided by its diagonal
+        assert vals[d] == 1.0
+        assert np.allclose(vals, raw / raw[d], rtol=0, atol=0)
 
 
 def test_gen_fem_properties():
