@@ -1120,6 +1120,8 @@ struct hbs_shard_s {
     int *send_J = nullptr;        // device [n_send]
     int *recv_src = nullptr;      // device [n_shr]: recv position of shr column c
     double *stage = nullptr;      // device [n_shr]: halo as received
+    double *x_ext = nullptr;      // device [m_loc + n_shr], for split x_loc / x_shr callers
+    double *d_send = nullptr;     // device [n_send], packed halo before D2H
 };
 
 extern "C" int hbs_shard_destroy(hbs_shard s)
@@ -1131,6 +1133,8 @@ extern "C" int hbs_shard_destroy(hbs_shard s)
     dfree(s->send_J);
     dfree(s->recv_src);
     dfree(s->stage);
+    dfree(s->x_ext);
+    dfree(s->d_send);
     delete s;
     return HBS_OK;
 }
@@ -1264,4 +1268,50 @@ extern "C" int hbs_shard_csr(hbs_shard s, hbs_csr *out)
     if (!s || !out) return fail(HBS_ERR_ARG, "null argument");
     *out = s->merged;
     return HBS_OK;
+}
+
+namespace {
+int shard_scratch(hbs_shard s)
+{
+    if (!s->x_ext) CHK(dalloc(&s->x_ext, (size_t)s->m_loc + s->n_shr));
+    if (!s->d_send) CHK(dalloc(&s->d_send, (size_t)std::max(s->n_send, 1)));
+    return HBS_OK;
+}
+}  // namespace
+
+extern "C" int hbs_shard_pack_host(hbs_shard s, const double *x_loc, double *send_host)
+{
+    if (!s || (s->n_send > 0 && (!x_loc || !send_host))) return fail(HBS_ERR_ARG, "bad argument");
+    if (s->n_send == 0) return HBS_OK;
+    DevGuard g(s->ctx->device);
+    CHK(shard_scratch(s));
+    CHK(hbs_shard_pack(s, x_loc, s->d_send));
+    HIPCHK(hipMemcpyAsync(send_host, s->d_send, sizeof(double) * s->n_send, hipMemcpyDeviceToHost,
+                          s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return HBS_OK;
+}
+
+extern "C" int hbs_shard_spmv2(hbs_shard s, const double *x_loc, const double *recv_host, double *x_shr,
+                               double alpha, double beta, double *y)
+{
+    if (!s || !x_loc || !y) return fail(HBS_ERR_ARG, "bad argument");
+    DevGuard g(s->ctx->device);
+    CHK(shard_scratch(s));
+    hipStream_t q = s->ctx->stream;
+    HIPCHK(hipMemcpyAsync(s->x_ext, x_loc, sizeof(double) * s->m_loc, hipMemcpyDeviceToDevice, q));
+    if (s->n_shr > 0) {
+        if (recv_host) {
+            HIPCHK(hipMemcpyAsync(s->stage, recv_host, sizeof(double) * s->n_shr, hipMemcpyHostToDevice, q));
+            CHK(hbs_shard_unpack(s, s->stage, s->x_ext));
+            if (x_shr)
+                HIPCHK(hipMemcpyAsync(x_shr, s->x_ext + s->m_loc, sizeof(double) * s->n_shr,
+                                      hipMemcpyDeviceToDevice, q));
+        } else {
+            if (!x_shr) return fail(HBS_ERR_ARG, "no halo: pass recv_host or x_shr");
+            HIPCHK(hipMemcpyAsync(s->x_ext + s->m_loc, x_shr, sizeof(double) * s->n_shr,
+                                  hipMemcpyDeviceToDevice, q));
+        }
+    }
+    return hbs_csr_spmv_f64(s->merged, alpha, s->x_ext, beta, y);
 }

@@ -246,6 +246,15 @@ int hbs_shard_spmv(hbs_shard s, double *x_ext_dev, const double *recv_host, doub
 int hbs_shard_dot_partial(hbs_shard s, const double *a_dev, const double *b_dev, double *out_host);
 /* The merged local+halo CSR (m_loc x (m_loc + n_shr)) for SpMV benchmarks. */
 int hbs_shard_csr(hbs_shard s, hbs_csr *out);
+/* Pack straight to host memory (what the reference then MPI_Isends,
+ * hhp_cpu.c:533-547); synchronous. */
+int hbs_shard_pack_host(hbs_shard s, const double *x_loc_dev, double *send_host);
+/* Shard SpMV for callers that keep x_loc and x_shr in separate device
+ * buffers (the reference's dX / dX_shr): y = alpha*(loc*x_loc + shr*x_shr) +
+ * beta*y.  With recv_host the halo is copied in, unpacked through recv.J and
+ * also written to x_shr_dev (if not NULL); without it x_shr_dev is the halo. */
+int hbs_shard_spmv2(hbs_shard s, const double *x_loc_dev, const double *recv_host,
+                    double *x_shr_dev, double alpha, double beta, double *y_dev);
 
 /* ---- timing helpers for benchmarks (hipEvents on the context stream) -- */
 int hbs_timer_start(hbs_ctx ctx);

@@ -1,0 +1,169 @@
+"""The reference-shaped device API (include/hhp_hip.h, include/hhp_dp_kernels.h)
+called the way the reference's drivers call it (structs by value), checked
+against the oracle and against numpy's element-wise arithmetic."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from conftest import golden
+from hethyppar_amd import hbs
+from workloads import convdiff64, partition
+
+pytestmark = pytest.mark.gpu
+
+_dp = C.POINTER(C.c_double)
+
+
+class Vector(C.Structure):
+    _fields_ = [("vals", _dp), ("nvals", C.c_uint)]
+
+
+class DeviceVector(C.Structure):
+    _fields_ = [("desc", C.c_void_p), ("vals", _dp), ("nvals", C.c_uint)]
+
+
+class DeviceCSR(C.Structure):
+    _fields_ = [("desc", C.c_void_p), ("data", hbs.HostCSR)]
+
+
+class DeviceShard(C.Structure):
+    _fields_ = [("loc", DeviceCSR), ("shr", DeviceCSR), ("gind", hbs.HostIVec),
+                ("send", hbs.HostComm), ("recv", hbs.HostComm), ("dev", C.c_void_p)]
+
+
+def L():
+    lib = hbs.lib()
+    lib.device_csr_create.argtypes = [hbs.HostCSR, C.POINTER(DeviceCSR)]
+    lib.device_csr_destroy.argtypes = [C.POINTER(DeviceCSR)]
+    lib.device_vector_init.argtypes = [C.c_size_t, C.POINTER(DeviceVector)]
+    lib.device_vector_destroy.argtypes = [C.POINTER(DeviceVector)]
+    lib.device_vector_toGPU.argtypes = [Vector, DeviceVector]
+    lib.device_vector_toCPU.argtypes = [DeviceVector, Vector]
+    lib.device_csr_spmv.argtypes = [C.c_void_p, DeviceCSR, DeviceVector, DeviceVector, C.c_double,
+                                    C.c_double, C.c_void_p]
+    lib.device_vector_dot.argtypes = [C.c_void_p, DeviceVector, DeviceVector, _dp]
+    lib.device_vector_axpy.argtypes = [C.c_void_p, DeviceVector, C.c_double, DeviceVector]
+    lib.device_vector_scale.argtypes = [C.c_void_p, C.c_double, DeviceVector]
+    lib.device_SHARD_CSC_create.argtypes = [hbs.HostShard, C.POINTER(DeviceShard)]
+    lib.device_SHARD_CSC_destroy.argtypes = [C.POINTER(DeviceShard)]
+    lib.device_SHARD_CSC_spmxv.argtypes = [DeviceShard, Vector, DeviceVector, DeviceVector, DeviceVector,
+                                           C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]
+    lib.hhp_default_handle.restype = C.c_void_p
+    for f in ("hhp_dp_vecop_P", "hhp_dp_vecop_S", "hhp_dp_vecop_XR", "hhp_dp_update_beta",
+              "hhp_dp_update_alpha", "hhp_dp_update_omega", "hhp_dp_gather", "hhp_dp_add"):
+        getattr(lib, f).restype = None
+    return lib
+
+
+def host_vec(a):
+    return Vector(a.ctypes.data_as(_dp), a.size)
+
+
+def dvec(lib, a):
+    v = DeviceVector()
+    assert lib.device_vector_init(a.size, C.byref(v)) == 0
+    assert lib.device_vector_toGPU(host_vec(a), v) == 0
+    return v
+
+
+def down(lib, v):
+    out = np.zeros(v.nvals)
+    assert lib.device_vector_toCPU(v, host_vec(out)) == 0
+    return out
+
+
+def test_device_csr_spmv_and_blas1():
+    lib = L()
+    rp, c, v, xt, b = convdiff64()
+    csr = hbs.HostCSR(rp.ctypes.data_as(C.POINTER(C.c_int)), c.ctypes.data_as(C.POINTER(C.c_int)),
+                      v.ctypes.data_as(_dp), c.size, rp.size - 1, rp.size - 1)
+    dA = DeviceCSR()
+    assert lib.device_csr_create(csr, C.byref(dA)) == 0
+    assert dA.data.nnz == c.size and dA.data.m == rp.size - 1
+    dx = dvec(lib, xt)
+    dy = dvec(lib, np.zeros(rp.size - 1))
+    assert lib.device_csr_spmv(None, dA, dx, dy, 1.0, 0.0, None) == 0
+    assert np.array_equal(down(lib, dy), orc.spmv(orc.CSR(rp, c, v), xt))
+    d = C.c_double()
+    assert lib.device_vector_dot(None, dy, dx, C.byref(d)) == 0
+    y = down(lib, dy)
+    assert abs(d.value - y @ xt) <= 1e-12 * (np.abs(y) @ np.abs(xt))
+    assert lib.device_vector_axpy(None, dx, -0.5, dy) == 0        # dy += -0.5 dx
+    assert lib.device_vector_scale(None, 3.0, dy) == 0            # dy *= 3
+    assert np.allclose(down(lib, dy), 3.0 * (y - 0.5 * xt), rtol=1e-15, atol=1e-15)
+    for vv in (dx, dy):
+        assert lib.device_vector_destroy(C.byref(vv)) == 0
+    assert lib.device_csr_destroy(C.byref(dA)) == 0
+
+
+def test_dp_kernels_match_elementwise_arithmetic():
+    lib = L()
+    n = 10007
+    rng = np.random.default_rng(1)
+    P, V, R, S, T, X = (rng.standard_normal(n) for _ in range(6))
+    sc = np.array([0.7, -1.3, 2.5, 0.4, 1.9, -0.2, 0.0, 0.0, 0.0, 0.0])
+    dP, dV, dR, dS, dT, dX = (dvec(lib, a) for a in (P, V, R, S, T, X))
+    dsc = dvec(lib, sc)
+    st = C.c_void_p(hbs.lib().hbs_ctx_stream(lib.hhp_default_handle()))   # same stream as the copies
+    p = lambda k: C.cast(C.c_void_p(C.cast(dsc.vals, C.c_void_p).value + 8 * k), _dp)
+    om, be, na = p(0), p(1), p(2)
+    lib.hhp_dp_vecop_P(dP.vals, dV.vals, dR.vals, om, be, n, st)
+    lib.hhp_dp_vecop_S(dS.vals, dR.vals, dV.vals, na, n, st)
+    al, om2, nom = p(3), p(4), p(5)
+    lib.hhp_dp_vecop_XR(dX.vals, dR.vals, dP.vals, dS.vals, dT.vals, al, om2, nom, n, st)
+    Pn = (P - sc[0] * V) * sc[1] + R
+    Sn = R + sc[2] * V
+    Xn = X + (sc[3] * Pn + sc[4] * Sn)
+    Rn = Sn + sc[5] * T
+    assert np.array_equal(down(lib, dP), Pn)
+    assert np.array_equal(down(lib, dS), Sn)
+    assert np.array_equal(down(lib, dX), Xn)
+    assert np.array_equal(down(lib, dR), Rn)
+    # beta = (rho_new/rho_old)*(alpha/omega); rho_old <- rho_new
+    s2 = np.array([0.0, 3.0, 6.0, 0.5, 0.25, 0.0, 0.0])
+    d2 = dvec(lib, s2)
+    q = lambda k: C.cast(C.c_void_p(C.cast(d2.vals, C.c_void_p).value + 8 * k), _dp)
+    lib.hhp_dp_update_beta(q(0), q(1), q(2), q(3), q(4), st)
+    lib.hhp_dp_update_alpha(q(5), q(6), q(1), q(4), st)
+    out = down(lib, d2)
+    assert out[0] == (6.0 / 3.0) * (0.5 / 0.25) and out[1] == 6.0
+    assert out[5] == 6.0 / 0.25 and out[6] == -(6.0 / 0.25)
+    idx = rng.integers(0, n, 999).astype(np.int32)
+    didx = dvec(lib, idx.astype(np.float64))   # room for 999 ints
+    hbs.check(hbs.lib().hbs_h2d(lib.hhp_default_handle(), C.cast(didx.vals, C.c_void_p),
+                                idx.ctypes.data_as(C.c_void_p), idx.nbytes))
+    dout = dvec(lib, np.zeros(999))
+    lib.hhp_dp_gather(dout.vals, dR.vals, C.cast(didx.vals, C.POINTER(C.c_int)), 999, st)
+    assert np.array_equal(down(lib, dout), Rn[idx])
+
+
+def test_device_shard_spmxv_host_halo():
+    lib = L()
+    _, _, rp, c, v = hbs.mm_read(golden("sample320", "A.mtx"))
+    pv = partition("sample320", "row_part_16")
+    x = np.random.default_rng(2).uniform(-1, 1, 320)
+    for r in (0, 7, 15):
+        hs = hbs.shard_build(rp, c, v, pv, 16, r)
+        raw = hs["raw"]
+        dA = DeviceShard()
+        assert lib.device_SHARD_CSC_create(raw, C.byref(dA)) == 0
+        m, ns = hs["loc"]["m"], hs["shr"]["n"]
+        x_loc = x[hs["gind"][:m]]
+        recv_buf = x[hs["gind"][m:][hs["recv"]["J"]]]
+        for k in range(ns):      # the halo as MPI would have delivered it
+            raw.recv.val[k] = recv_buf[k]
+        dX = dvec(lib, x_loc)
+        dXs = dvec(lib, np.zeros(max(ns, 1)))
+        dY = dvec(lib, np.zeros(m))
+        assert lib.device_SHARD_CSC_spmxv(dA, Vector(), dX, dXs, dY, None, 1.0, 0.0, None, None, None) == 0
+        assert np.array_equal(down(lib, dY), orc.shard_spmv(hs, x_loc, recv_buf))
+        nsend = int(hs["send"]["I"][-1]) if hs["send"]["num"] else 0
+        sent = np.array([raw.send.val[k] for k in range(nsend)])
+        assert np.array_equal(sent, orc.shard_pack(hs, x_loc))
+        if ns:
+            assert np.array_equal(down(lib, dXs)[:ns], x[hs["gind"][m:]])
+        assert lib.device_SHARD_CSC_destroy(C.byref(dA)) == 0
+        hbs.shard_free(hs)
