@@ -16,33 +16,39 @@ constexpr int kThreads = 256;          // workgroup size of every kernel (4 wave
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxBlockNnz = 4096;     // upper bound of nnz per SpMV row block
 constexpr int kMaxDots = 2;            // reductions fused into one pass
+constexpr int kMaxGroups = 1024;       // workgroups of a reducing kernel = partials a consumer sums
 
-// One SpMV row block.  Normal block: rows [r0, r1) whose entries [j0, j1) fit
-// in LDS.  Chunk block: entries [j0, j1) of one long row r0 (r1 == r0 + 1).
+// One SpMV row block: rows [r0, r1) with entries [j0, j1).
 struct alignas(16) RowBlock {
     int r0, r1, j0, j1;
 };
-// chunk < 0: normal block; lanes = lanes per row in the reduction (1 = the
-// order-exact serial sum).  chunk >= 0: index of this chunk within its row,
-// nchunks chunks starting at block `first`.  sys: system the rows belong to.
+// kind 0: the block's entries fit in LDS; lanes = lanes per row in the
+// reduction (1 = the order-exact serial sum).  kind 1: one long row r0 whose
+// value is the sum, in order, of the chunk sums [first, first + nchunks).
 struct alignas(16) BlockAux {
-    int chunk, nchunks, first;
-    int lanes_sys;  // lanes | sys << 8
+    int kind, first, nchunks, lanes;
 };
-__host__ __device__ inline int aux_lanes(const BlockAux &a) { return a.lanes_sys & 0xff; }
-__host__ __device__ inline int aux_sys(const BlockAux &a) { return a.lanes_sys >> 8; }
-
-// A contiguous element range [e0, e1) of one system, processed by one
-// workgroup of the streaming vector kernels.
+// A piece [j0, j1) of a long row, summed by one workgroup of k_chunks.
+struct alignas(16) Chunk {
+    int j0, j1, row, sys;
+};
+// SpMV workgroup p handles row blocks [rb0, rb1), all of system sys.
+struct alignas(16) Group {
+    int rb0, rb1, sys, pad;
+};
+// Vector workgroup: elements [e0, e1) of system sys.
 struct alignas(16) VecBlock {
     int sys, e0, e1, pad;
 };
-// Workgroup ranges of one system: SpMV blocks [b0, b1), vector blocks [v0, v1).
+// Per system: SpMV workgroups [g0, g1) and vector workgroups [v0, v1).
 struct alignas(16) SysRange {
-    int b0, b1, v0, v1;
+    int g0, g1, v0, v1;
 };
 
-// Device-resident BiCGStab scalars, one per system.
+// Device-resident BiCGStab scalars of one system.  A solve keeps two copies
+// (slots A and B): every kernel that updates scalars reads one slot and its
+// first workgroup of the system writes the other, so no workgroup ever reads
+// a value another workgroup of the same kernel is writing.
 struct alignas(16) SolverState {
     double rho, alpha, omega, beta;
     double rho_next;     // (r, r0) of the newest r
@@ -55,7 +61,7 @@ struct alignas(16) SolverState {
     double resid2;       // ||b - Ax||^2 of the last residual pass
     double out;          // generic reduction result
     int iter, maxit, done, stop;
-    int s_conv, fixed, pad0, pad1;
+    int s_conv, fixed, pending, pad;   // pending: the (r,r0), ||r||^2 partials await FIN_XR
 };
 
 enum Epi : int {
@@ -63,43 +69,31 @@ enum Epi : int {
     EPI_DOT1 = 1,    // y = Ax ; d0 = (w, y)
     EPI_DOT2 = 2,    // y = Ax ; d0 = (y, w), d1 = (y, y)
     EPI_RESID = 3,   // y = w - Ax ; d0 = (y, y)
+    EPI_RESID2 = 4,  // y = w - Ax ; d0 = d1 = (y, y)   (start of a solve)
 };
 
 enum Fin : int {
     FIN_NONE = 0,
     FIN_ALPHA = 1,   // rv = d0, alpha = rho / rv
     FIN_OMEGA = 2,   // ts = d0, tt = d1, omega = ts / tt
-    FIN_INIT = 3,    // rr0 = d0: threshold, first beta
-    FIN_RESID = 4,   // resid2 = d0
-    FIN_S = 5,       // ss = d0: ||s|| stop test
-    FIN_XR = 6,      // rho_next = d0, rr = d1: ||r|| stop test, next beta
-    FIN_STORE = 7,   // out = d0
-};
-
-// Reduction scratch shared by the kernels of one plan.  Counters reset
-// themselves (the last arriving workgroup writes 0), so the scratch needs no
-// clearing between launches or graph replays.
-struct Scratch {
-    double *part = nullptr;       // [nslots * kMaxDots] per-workgroup partials
-    double *chunk = nullptr;      // [nslots] long-row chunk sums
-    unsigned *ctr_sys = nullptr;  // [nsys] per-system arrival counters
-    unsigned *ctr_row = nullptr;  // [nslots] long-row arrival counters
-    unsigned *ndone = nullptr;    // [1] systems finished
+    FIN_XR = 3,      // rho_next = d0, rr = d1: iteration count, stop tests, next beta
+    FIN_S = 4,       // ss = d0: ||s|| stop test
+    FIN_RESID = 5,   // resid2 = d0
+    FIN_STORE = 6,   // out = d0
 };
 
 struct SpmvPlan {
     int nblk = 0;
+    int ngroups = 0;
+    int nchunks = 0;
     int nsys = 1;
     int block_nnz = 0;
     int serial_rows = 0;
     int max_row = 0;
     RowBlock *blocks = nullptr;   // device [nblk]
     BlockAux *aux = nullptr;      // device [nblk]
-};
-
-struct VecPlan {
-    int nvb = 0;
-    VecBlock *blocks = nullptr;   // device [nvb]
+    Chunk *chunks = nullptr;      // device [nchunks]
+    Group *groups = nullptr;      // device [ngroups]
 };
 
 // Launch parameters of the SpMV kernels.
@@ -107,8 +101,9 @@ template <typename T>
 struct SpmvArgs {
     const RowBlock *blocks;
     const BlockAux *aux;
+    const Group *groups;
+    int ngroups;
     const SysRange *sys;
-    int nblk;
     const int *rowptr;
     const int *col;
     const T *val;
@@ -116,11 +111,26 @@ struct SpmvArgs {
     T *y;
     const T *w;
     T alpha, beta;
-    Scratch sc;
-    SolverState *st;      // [nsys] or null
-    int fin;
-    int check_done;       // iteration kernels: return if the system is done
-    int skip_on_sconv;    // ... or if the ||s|| test fired this iteration
+    const double *chunk_sums;   // sums of the long-row chunks (k_chunks, same x)
+    double *part_out;           // [ngroups * kMaxDots] partials of the fused dots
+    const double *part_in;      // partials of the previous vector kernel (FIN_S)
+    int nparts_in;
+    const SolverState *st_in;   // null: no state
+    SolverState *st_out;
+    int fin_in;                 // FIN_NONE or FIN_S
+    int check_done;             // return when the system is done (or ||s|| fired)
+    SolverState *st_mark;       // start of a solve: set pending in this slot
+};
+
+template <typename T>
+struct ChunkArgs {
+    const Chunk *chunks;
+    int nchunks;
+    const int *col;
+    const T *val;
+    const T *x;
+    double *chunk_sums;
+    const SolverState *st;      // null: always run
 };
 
 // Launch parameters of the vector kernels of one BiCGStab iteration.
@@ -131,42 +141,63 @@ struct VecArgs {
     int nvb;
     T *X, *R, *P, *V, *S, *T_;
     const T *R0;
-    SolverState *st;
-    Scratch sc;
+    const SolverState *st_in;
+    SolverState *st_out;
+    const double *part_in;      // partials the prologue applies
+    int nparts_in;
+    int in_spmv;                // 1: part_in from an SpMV (ranges g0..g1), 0: from a vector kernel
+    double *part_out;           // [nvb * kMaxDots]
 };
 
 // ---- host-side planning (hbs_plan.cpp) ----
-// Row blocks of rows [row0, row0 + m) of one system; appended to blocks/aux.
+struct HostPlan {
+    std::vector<RowBlock> blocks;
+    std::vector<BlockAux> aux;
+    std::vector<Chunk> chunks;
+    std::vector<Group> groups;
+    int serial_rows = 0;
+    int max_row = 0;
+};
+// Row blocks of rows [row0, row0 + m) of system sys, appended to the plan.
 void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max_avg, int sys,
-               std::vector<RowBlock> &blocks, std::vector<BlockAux> &aux, int *serial_rows,
-               int *max_row);
+               HostPlan &plan);
+// Split the row blocks [rb0, rb1) of one system over ngroups workgroups.
+void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan);
 // Vector blocks of elements [e0, e0 + n) of one system.
 void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock> &blocks);
+int vec_elems_per_block(int n);
 
 // ---- launchers (hbs_kernels.hip) ----
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, hipStream_t s);
 hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, hipStream_t s);
+hipError_t launch_chunks(const ChunkArgs<double> &a, int block_nnz, hipStream_t s);
+hipError_t launch_chunks(const ChunkArgs<float> &a, int block_nnz, hipStream_t s);
 hipError_t launch_vec_P(const VecArgs<double> &a, hipStream_t s);
 hipError_t launch_vec_S(const VecArgs<double> &a, hipStream_t s);
 hipError_t launch_vec_XR(const VecArgs<double> &a, hipStream_t s);
 hipError_t launch_vec_P(const VecArgs<float> &a, hipStream_t s);
 hipError_t launch_vec_S(const VecArgs<float> &a, hipStream_t s);
 hipError_t launch_vec_XR(const VecArgs<float> &a, hipStream_t s);
+// Apply the pending FIN_XR of every system in place (one workgroup per system).
+hipError_t launch_apply_xr(SolverState *st, const SysRange *sys, int nsys, const double *part, int nvb,
+                           hipStream_t s);
+// Reduce partials into st[sys] with fin (FIN_RESID / FIN_STORE), one workgroup per system.
+hipError_t launch_finalize(SolverState *st, const SysRange *sys, int nsys, const double *part,
+                           int nparts, int in_spmv, int fin, hipStream_t s);
 // R0 = R ; P = V = 0 (FP64, after the residual SpMV wrote R)
 hipError_t launch_init_vectors(const VecArgs<double> &a, hipStream_t s);
-// Mixed sweep start: R = R0 = (float)(r64 / ||r64||), X = P = V = 0, d0 = ||R||^2
-// -> FIN_INIT on the inner state; the inner state's scale = ||r64||.
-hipError_t launch_ir_start(const VecArgs<float> &a, const double *r64,
-                           const SolverState *outer, hipStream_t s);
-// Mixed sweep end: x64 += scale * (double)x32.
+// Mixed sweep start: R = R0 = (float)(r64 / ||r64||), X = P = V = 0, partials
+// d0 = d1 = ||R||^2 in part_out; st_out (the inner slot A) gets pending and
+// scale = ||r64|| from outer.resid2.
+hipError_t launch_ir_start(const VecArgs<float> &a, const double *r64, const SolverState *outer,
+                           hipStream_t s);
+// Mixed sweep end: x64 += scale * (double)x32 (scale from st_in).
 hipError_t launch_ir_update(const VecArgs<float> &a, double *x64, hipStream_t s);
-// Elementwise helpers over the vector plan (one system or many).
-hipError_t launch_dot(const VecArgs<double> &a, const double *x, const double *y, int fin,
-                      hipStream_t s);
-hipError_t launch_state_init(SolverState *st, int nsys, double tol, int maxit, int fixed,
-                             unsigned *ndone, hipStream_t s);
+// d0 = (x, y) partials into part_out.
+hipError_t launch_dot(const VecArgs<double> &a, const double *x, const double *y, hipStream_t s);
+hipError_t launch_state_init(SolverState *st, int nstates, double tol, int maxit, int fixed, hipStream_t s);
 
-// BLAS-1 helpers on raw device pointers (single system).
+// BLAS-1 helpers on raw device pointers.
 hipError_t launch_axpy_f64(int n, double a, const double *x, double *y, hipStream_t s);
 hipError_t launch_scal_f64(int n, double a, double *x, hipStream_t s);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);

@@ -1,25 +1,26 @@
 // hbs_kernels.hip -- hand-written CDNA4 (gfx950) kernels of the BiCGStab/SpMV engine.
 //
-// Everything here is HBM-bandwidth bound (no dense contraction, so no MFMA):
-// the kernels are written for coalesced 16-byte streams, wave64 reductions and
-// XCD-aware workgroup placement.
+// Everything here is bound by memory traffic (no dense contraction, so no
+// MFMA): coalesced 16-byte streams, wave64 reductions, XCD-aware placement.
 //
-//  * k_spmv       CSR-stream SpMV.  A workgroup owns a row block whose entries
-//                 fit in LDS: phase 1 streams col/val coalesced and writes one
-//                 product per entry to LDS; phase 2 sums each row.  Short-row
-//                 blocks sum serially in CSR order (bit-identical to the
-//                 reference CSR_spmxv_seq, hhp_cpu.c:445-455); blocks of longer
-//                 rows use 2..64 lanes per row; rows longer than a block are
-//                 split into chunks whose sums are combined in chunk order by
-//                 the last chunk to finish.  Epilogues fuse the dot products
-//                 BiCGStab needs after each SpMV.
-//  * k_vec_*      the three BiCGStab vector updates, each fused with the
-//                 reductions that follow it (reference k_vecop_P/S/XR,
-//                 hhp_dp_kernels.cu:51-74, plus the dots of
-//                 bicgstab_cpu.c:203,229,242).
-//  * reductions   deterministic: every workgroup stores its partial, the last
-//                 one to arrive (atomic counter) sums the partials in a fixed
-//                 order and updates the device-resident scalars.
+//  * k_spmv     CSR-stream SpMV.  A workgroup owns a contiguous run of row
+//               blocks whose entries fit in LDS: phase 1 streams col/val
+//               coalesced and stores one product per entry in LDS; phase 2
+//               sums each row.  Short-row blocks sum serially in CSR order
+//               (bit-identical to the reference CSR_spmxv_seq,
+//               hhp_cpu.c:445-455); longer rows use 2..64 lanes; rows longer
+//               than a block are cut into chunks summed by k_chunks and
+//               combined here in chunk order.  Epilogues fuse the dot
+//               products BiCGStab needs after each SpMV.
+//  * k_vec_*    the three BiCGStab vector updates (reference k_vecop_P/S/XR,
+//               hhp_dp_kernels.cu:51-74), each fused with the reductions of
+//               bicgstab_cpu.c:203,219,229,242 that follow it.
+//  * reductions atomic-free and deterministic: a producer writes one partial
+//               per workgroup (<= 1024); every workgroup of the consumer
+//               kernel sums them in the same fixed order, so all of them
+//               derive bit-identical scalars (alpha, omega, beta, stop
+//               decisions) in their prologue.  The first workgroup of a
+//               system stores the updated scalars in the other state slot.
 #include <hip/hip_runtime.h>
 
 #include "hbs_internal.h"
@@ -29,28 +30,19 @@ namespace {
 
 constexpr int kXcds = 8;   // MI355X: 8 XCDs, workgroups dispatched round robin
 
-// Give each XCD a contiguous range of logical blocks so neighbouring row
-// blocks (which touch neighbouring x entries) share one XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nblk)
+// Give each XCD a contiguous range of logical workgroups so neighbouring row
+// blocks (which gather neighbouring x entries) share one XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int n)
 {
-    const int q = nblk / kXcds, r = nblk % kXcds;
+    const int q = n / kXcds, r = n % kXcds;
     const int x = bid % kXcds, k = bid / kXcds;
     return x * q + (x < r ? x : r) + k;
 }
 
-__device__ __forceinline__ double ld_agent(const double *p)
-{
-    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(double *p, double v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Per-thread dot accumulators.  FP64: plain sequential sum of rounded
-// products.  FP32: Dot2 of Ogita, Rump and Oishi (2005) -- TwoProduct by FMA
-// and Knuth's TwoSum, the error terms summed in a second float -- handed to
-// the FP64 workgroup / grid reduction.
+// Per-thread dot accumulators.  FP64: sequential sum of rounded products.
+// FP32: Dot2 of Ogita, Rump and Oishi (2005) -- TwoProduct by FMA and Knuth's
+// TwoSum, error terms kept in a second float -- handed on as FP64 to the
+// workgroup and grid sums.
 template <typename T>
 struct Acc;
 template <>
@@ -120,90 +112,69 @@ __device__ __forceinline__ T block_sum1(T v, T *red)
     return t;
 }
 
-// Thread 0 has stored this workgroup's partials; count the arrival and, in
-// the last workgroup of the system, sum partials [b0, b1) in a fixed order.
+// Sum of partials [p0, p1) (stride nparts between the ND rows), fixed order.
 template <int ND>
-__device__ bool arrive_and_reduce(double (&v)[ND], const double *part, int nslots, unsigned *ctr,
-                                  int b0, int b1, double *red, int *s_flag)
+__device__ __forceinline__ void reduce_parts(const double *part, int nparts, int p0, int p1, double (&v)[ND],
+                                             double *red)
 {
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(ctr, 1u);
-        const int last = prev == (unsigned)(b1 - b0 - 1);
-        if (last) __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        *s_flag = last;
-    }
-    __syncthreads();
-    const int last = *s_flag;
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         double acc = 0.0;
-        for (int i = b0 + (int)threadIdx.x; i < b1; i += kThreads) acc += ld_agent(&part[(size_t)d * nslots + i]);
+        for (int i = p0 + (int)threadIdx.x; i < p1; i += kThreads) acc += part[(size_t)d * nparts + i];
         v[d] = acc;
     }
     block_sum<ND>(v, red);
-    return true;
 }
 
-__device__ __forceinline__ void mark_done(SolverState *st, int stop, unsigned *ndone)
+__device__ __forceinline__ void set_done(SolverState &s, int stop)
 {
-    st->done = 1;
-    st->stop = stop;
-    if (ndone) atomicAdd(ndone, 1u);
+    s.done = 1;
+    s.stop = stop;
 }
 
-// Scalar updates of one system (thread 0 of the last workgroup).
-// Order of operations follows bicgstab_cpu.c:203-229.
-template <int ND>
-__device__ void finalize(SolverState *st, int fin, const double (&v)[ND], unsigned *ndone)
+// Scalar updates (identical in every workgroup).  Order of operations of
+// bicgstab_cpu.c:203-229; stop tests of bicgstab_gpu.c:148-150,231-236,289-293.
+__device__ void apply_fin(SolverState &s, int fin, double d0, double d1)
 {
-    const double d0 = ND > 0 ? v[0] : 0.0;
-    const double d1 = ND > 1 ? v[1] : 0.0;
     switch (fin) {
     case FIN_ALPHA:
-        st->rv = d0;
-        st->alpha = st->rho / d0;
-        if (d0 == 0.0 || !isfinite(st->alpha)) mark_done(st, 4, ndone);
+        s.rv = d0;
+        s.alpha = s.rho / d0;
+        if (d0 == 0.0 || !isfinite(s.alpha)) set_done(s, 4);
         break;
     case FIN_OMEGA:
-        st->ts = d0;
-        st->tt = d1;
-        st->omega = d0 / d1;
-        if (d1 == 0.0 || !isfinite(st->omega)) mark_done(st, 4, ndone);
-        break;
-    case FIN_INIT:
-        st->rr0 = d0;
-        st->thr = st->tol * sqrt(d0);
-        st->rho_next = d0;
-        if (st->done) break;
-        if (d0 == 0.0) { mark_done(st, 2, ndone); break; }
-        if (!isfinite(d0)) { mark_done(st, 4, ndone); break; }
-        st->beta = (d0 / st->rho) * (st->alpha / st->omega);
-        st->rho = d0;
-        break;
-    case FIN_RESID:
-        st->resid2 = d0;
-        break;
-    case FIN_S:
-        st->ss = d0;
-        if (!st->fixed && sqrt(d0) < st->thr) st->s_conv = 1;
+        s.ts = d0;
+        s.tt = d1;
+        s.omega = d0 / d1;
+        if (d1 == 0.0 || !isfinite(s.omega)) set_done(s, 4);
         break;
     case FIN_XR:
-        st->iter += 1;
-        if (st->s_conv) { mark_done(st, 1, ndone); break; }
-        st->rho_next = d0;
-        st->rr = d1;
-        if (!st->fixed && sqrt(d1) < st->thr) { mark_done(st, 2, ndone); break; }
-        if (st->iter >= st->maxit) { mark_done(st, 0, ndone); break; }
-        if (d0 == 0.0 || !isfinite(d0)) { mark_done(st, 4, ndone); break; }
-        st->beta = (d0 / st->rho) * (st->alpha / st->omega);
-        st->rho = d0;
+        // end of iteration `iter` (or, with iter == -1, the initial residual)
+        s.pending = 0;
+        s.iter += 1;
+        if (s.iter == 0) {
+            s.rr0 = d1;
+            s.thr = s.tol * sqrt(d1);
+        }
+        if (s.s_conv) { set_done(s, 1); break; }
+        s.rho_next = d0;
+        s.rr = d1;
+        if (s.iter > 0 && !s.fixed && sqrt(d1) < s.thr) { set_done(s, 2); break; }
+        if (d1 == 0.0) { set_done(s, 2); break; }
+        if (s.iter >= s.maxit) { set_done(s, 0); break; }
+        if (d0 == 0.0 || !isfinite(d0)) { set_done(s, 4); break; }
+        s.beta = (d0 / s.rho) * (s.alpha / s.omega);
+        s.rho = d0;
+        break;
+    case FIN_S:
+        s.ss = d0;
+        if (!s.fixed && sqrt(d0) < s.thr) s.s_conv = 1;
+        break;
+    case FIN_RESID:
+        s.resid2 = d0;
         break;
     case FIN_STORE:
-        st->out = d0;
+        s.out = d0;
         break;
     default:
         break;
@@ -212,7 +183,7 @@ __device__ void finalize(SolverState *st, int fin, const double (&v)[ND], unsign
 
 template <int EPI>
 struct EpiDots {
-    static constexpr int n = EPI == EPI_DOT2 ? 2 : (EPI == EPI_AXPBY ? 0 : 1);
+    static constexpr int n = (EPI == EPI_DOT2 || EPI == EPI_RESID2) ? 2 : (EPI == EPI_AXPBY ? 0 : 1);
 };
 
 // Row epilogue: write y[r] and feed the fused dots.
@@ -234,6 +205,7 @@ __device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, Acc<
         const T out = a.w[r] - acc;
         a.y[r] = out;
         d[0].add(out, out);
+        if constexpr (EPI == EPI_RESID2) d[1].add(out, out);
     }
 }
 
@@ -243,32 +215,50 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
     extern __shared__ __align__(16) unsigned char smem[];
     T *lds = reinterpret_cast<T *>(smem);
     __shared__ double red[kMaxDots * kWaves];
-    __shared__ T tred[kWaves];
-    __shared__ int s_flag;
     constexpr int ND = EpiDots<EPI>::n;
 
-    const int b = xcd_remap(blockIdx.x, a.nblk);
-    const RowBlock rb = a.blocks[b];
-    const BlockAux ax = a.aux[b];
-    const int sys = aux_sys(ax);
-    SolverState *st = a.st ? a.st + sys : nullptr;
-    if (st && a.check_done && (st->done || (a.skip_on_sconv && st->s_conv))) return;
-
-    // phase 1: coalesced col/val streams, x gathered, one product per entry
-    const int nz = rb.j1 - rb.j0;
-    const int *__restrict__ colp = a.col + rb.j0;
-    const T *__restrict__ valp = a.val + rb.j0;
-    const T *__restrict__ x = a.x;
-#pragma unroll 4
-    for (int k = threadIdx.x; k < nz; k += kThreads) lds[k] = valp[k] * x[colp[k]];
-    __syncthreads();
+    const int p = xcd_remap(blockIdx.x, a.ngroups);
+    const Group g = a.groups[p];
+    if (a.st_in) {
+        SolverState s = a.st_in[g.sys];
+        if (a.check_done && s.done) {
+            if (a.st_out && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_out[g.sys] = s;
+            return;
+        }
+        if (a.fin_in == FIN_S) {
+            const SysRange sr = a.sys[g.sys];
+            double v[1];
+            reduce_parts<1>(a.part_in, a.nparts_in, sr.v0, sr.v1, v, red);
+            apply_fin(s, FIN_S, v[0], 0.0);
+            if (p == sr.g0 && threadIdx.x == 0) a.st_out[g.sys] = s;
+            if (s.s_conv) return;
+        }
+    }
+    if (a.st_mark && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_mark[g.sys].pending = 1;
 
     Acc<T> d[kMaxDots];
-    bool write_slot = true;
-    int slot = b;
-    if (ax.chunk < 0) {
+    const T *__restrict__ x = a.x;
+    for (int b = g.rb0; b < g.rb1; ++b) {
+        const RowBlock rb = a.blocks[b];
+        const BlockAux ax = a.aux[b];
+        if (ax.kind == 1) {
+            // long row: ordered sum of its chunk sums
+            if (threadIdx.x == 0) {
+                T total = T(0);
+                for (int c = 0; c < ax.nchunks; ++c) total += (T)a.chunk_sums[ax.first + c];
+                row_out<T, EPI>(a, rb.r0, total, d);
+            }
+            continue;
+        }
+        // phase 1: coalesced col/val streams, x gathered, one product per entry
+        const int nz = rb.j1 - rb.j0;
+        const int *__restrict__ colp = a.col + rb.j0;
+        const T *__restrict__ valp = a.val + rb.j0;
+#pragma unroll 4
+        for (int k = threadIdx.x; k < nz; k += kThreads) lds[k] = valp[k] * x[colp[k]];
+        __syncthreads();
         const int nr = rb.r1 - rb.r0;
-        const int lanes = aux_lanes(ax);
+        const int lanes = ax.lanes;
         if (lanes <= 1) {
             // order-exact: res = 0; res += p_j in CSR order
             for (int t = threadIdx.x; t < nr; t += kThreads) {
@@ -279,12 +269,12 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
                 row_out<T, EPI>(a, r, acc, d);
             }
         } else {
-            const int g = threadIdx.x / lanes, l = threadIdx.x & (lanes - 1);
+            const int gi = threadIdx.x / lanes, l = threadIdx.x & (lanes - 1);
             const int ng = kThreads / lanes;
             for (int t0 = 0; t0 < nr; t0 += ng) {
-                const int t = t0 + g;
+                const int t = t0 + gi;
                 T acc = T(0);
-                int r = rb.r0 + t;
+                const int r = rb.r0 + t;
                 if (t < nr) {
                     const int s = a.rowptr[r] - rb.j0, e = a.rowptr[r + 1] - rb.j0;
                     for (int q = s + l; q < e; q += lanes) acc += lds[q];
@@ -293,41 +283,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
                 if (t < nr && l == 0) row_out<T, EPI>(a, r, acc, d);
             }
         }
-    } else {
-        // chunk of a long row: workgroup sum, then the last chunk combines
-        T acc = T(0);
-        for (int k = threadIdx.x; k < nz; k += kThreads) acc += lds[k];
-        acc = block_sum1<T>(acc, tred);
-        if (threadIdx.x == 0) {
-            st_agent(&a.sc.chunk[b], (double)acc);
-            __threadfence();
-            const unsigned prev = atomicAdd(&a.sc.ctr_row[ax.first], 1u);
-            const int last = prev == (unsigned)(ax.nchunks - 1);
-            if (last) __hip_atomic_store(&a.sc.ctr_row[ax.first], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_flag = last;
-        }
-        __syncthreads();
-        const int last = s_flag;
-        __syncthreads();
-        if (last) {
-            __threadfence();
-            if (threadIdx.x == 0) {
-                T total = T(0);
-                for (int c = 0; c < ax.nchunks; ++c) total += (T)ld_agent(&a.sc.chunk[ax.first + c]);
-                row_out<T, EPI>(a, rb.r0, total, d);
-            }
-        }
-        // the row's dot contribution lives in the slot of its first chunk,
-        // written only by the finalizing chunk; other chunks hold zero
-        if (last) {
-            slot = ax.first;
-            if (ax.chunk != 0 && threadIdx.x == 0 && ND > 0) {
-#pragma unroll
-                for (int q = 0; q < ND; ++q) st_agent(&a.sc.part[(size_t)q * a.nblk + b], 0.0);
-            }
-        } else if (ax.chunk == 0) {
-            write_slot = false;
-        }
+        __syncthreads();   // LDS is reused by the next row block
     }
 
     if constexpr (ND > 0) {
@@ -336,16 +292,23 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
         for (int q = 0; q < ND; ++q) v[q] = d[q].get();
         block_sum<ND>(v, red);
         if (threadIdx.x == 0) {
-            if (write_slot) {
 #pragma unroll
-                for (int q = 0; q < ND; ++q) st_agent(&a.sc.part[(size_t)q * a.nblk + slot], v[q]);
-            }
-        }
-        const SysRange sr = a.sys[sys];
-        if (arrive_and_reduce<ND>(v, a.sc.part, a.nblk, &a.sc.ctr_sys[sys], sr.b0, sr.b1, red, &s_flag)) {
-            if (threadIdx.x == 0 && st) finalize<ND>(st, a.fin, v, a.sc.ndone);
+            for (int q = 0; q < ND; ++q) a.part_out[(size_t)q * a.ngroups + p] = v[q];
         }
     }
+}
+
+// One workgroup per chunk of a long row: ordered tree sum of its products.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_chunks(ChunkArgs<T> a)
+{
+    __shared__ T red[kWaves];
+    const Chunk c = a.chunks[blockIdx.x];
+    if (a.st && a.st[c.sys].done) return;
+    T acc = T(0);
+    for (int j = c.j0 + (int)threadIdx.x; j < c.j1; j += kThreads) acc += a.val[j] * a.x[a.col[j]];
+    acc = block_sum1<T>(acc, red);
+    if (threadIdx.x == 0) a.chunk_sums[blockIdx.x] = (double)acc;
 }
 
 // ---- streaming vector kernels --------------------------------------------
@@ -367,7 +330,6 @@ __device__ __forceinline__ void stp(T *p, const Pk<T> &x)
 }
 
 // Visit [e0, e1): scalar head up to a 16-byte boundary, 16-byte packs, tail.
-// f(i) handles element i; fv(i) handles the pack starting at i.
 template <typename T, typename F, typename FV>
 __device__ __forceinline__ void visit(int e0, int e1, F f, FV fv)
 {
@@ -379,9 +341,25 @@ __device__ __forceinline__ void visit(int e0, int e1, F f, FV fv)
     if ((int)threadIdx.x < e1 - a1) f(a1 + threadIdx.x);
 }
 
+template <typename T>
+__device__ __forceinline__ bool first_of_sys(const VecArgs<T> &a, const VecBlock &vb)
+{
+    return (int)blockIdx.x == a.sys[vb.sys].v0;
+}
+
 template <int ND, typename T>
-__device__ __forceinline__ void vec_reduce(const VecArgs<T> &a, const VecBlock &vb, Acc<T> *d,
-                                           int fin, double *red, int *s_flag)
+__device__ __forceinline__ void apply_in(const VecArgs<T> &a, const VecBlock &vb, SolverState &s, int fin,
+                                         double *red)
+{
+    const SysRange sr = a.sys[vb.sys];
+    double v[ND];
+    if (a.in_spmv) reduce_parts<ND>(a.part_in, a.nparts_in, sr.g0, sr.g1, v, red);
+    else reduce_parts<ND>(a.part_in, a.nparts_in, sr.v0, sr.v1, v, red);
+    apply_fin(s, fin, v[0], ND > 1 ? v[ND - 1] : 0.0);
+}
+
+template <int ND, typename T>
+__device__ __forceinline__ void write_parts(const VecArgs<T> &a, Acc<T> *d, double *red)
 {
     double v[ND];
 #pragma unroll
@@ -389,22 +367,23 @@ __device__ __forceinline__ void vec_reduce(const VecArgs<T> &a, const VecBlock &
     block_sum<ND>(v, red);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < ND; ++q) st_agent(&a.sc.part[(size_t)q * a.nvb + blockIdx.x], v[q]);
-    }
-    const SysRange sr = a.sys[vb.sys];
-    if (arrive_and_reduce<ND>(v, a.sc.part, a.nvb, &a.sc.ctr_sys[vb.sys], sr.v0, sr.v1, red, s_flag)) {
-        if (threadIdx.x == 0) finalize<ND>(a.st + vb.sys, fin, v, a.sc.ndone);
+        for (int q = 0; q < ND; ++q) a.part_out[(size_t)q * a.nvb + blockIdx.x] = v[q];
     }
 }
 
-// P = ((P - V*omega)*beta) + R  (bicgstab_cpu.c:210-213; k_vecop_P)
+// Start of an iteration: apply the pending (r, r0), ||r||^2 of the previous
+// one (stop tests, beta), then P = ((P - V*omega)*beta) + R
+// (bicgstab_cpu.c:203-213; k_vecop_P).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_vec_P(VecArgs<T> a)
 {
+    __shared__ double red[kMaxDots * kWaves];
     const VecBlock vb = a.blocks[blockIdx.x];
-    const SolverState *st = a.st + vb.sys;
-    if (st->done) return;
-    const T om = (T)st->omega, be = (T)st->beta;
+    SolverState s = a.st_in[vb.sys];
+    if (!s.done && s.pending) apply_in<2, T>(a, vb, s, FIN_XR, red);
+    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys] = s;
+    if (s.done) return;
+    const T om = (T)s.omega, be = (T)s.beta;
     T *__restrict__ P = a.P;
     const T *__restrict__ V = a.V;
     const T *__restrict__ R = a.R;
@@ -419,17 +398,18 @@ __global__ __launch_bounds__(kThreads) void k_vec_P(VecArgs<T> a)
         });
 }
 
-// S = R - V*alpha, ||S||^2  (bicgstab_cpu.c:222-223; k_vecop_S; stop test of
-// bicgstab_gpu.c:231-236)
+// alpha = rho / (r0, v) from the SpMV partials; S = R - V*alpha, ||S||^2
+// (bicgstab_cpu.c:219-223; k_vecop_S).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_vec_S(VecArgs<T> a)
 {
     __shared__ double red[kMaxDots * kWaves];
-    __shared__ int s_flag;
     const VecBlock vb = a.blocks[blockIdx.x];
-    const SolverState *st = a.st + vb.sys;
-    if (st->done) return;
-    const T al = (T)st->alpha;
+    SolverState s = a.st_in[vb.sys];
+    if (!s.done) apply_in<1, T>(a, vb, s, FIN_ALPHA, red);
+    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys] = s;
+    if (s.done) return;
+    const T al = (T)s.alpha;
     T *__restrict__ S = a.S;
     const T *__restrict__ V = a.V;
     const T *__restrict__ R = a.R;
@@ -437,43 +417,45 @@ __global__ __launch_bounds__(kThreads) void k_vec_S(VecArgs<T> a)
     visit<T>(
         vb.e0, vb.e1,
         [&](int i) {
-            const T s = R[i] - V[i] * al;
-            S[i] = s;
-            d[0].add(s, s);
+            const T x = R[i] - V[i] * al;
+            S[i] = x;
+            d[0].add(x, x);
         },
         [&](int i) {
             const Pk<T> v = ldp(V + i), r = ldp(R + i);
-            Pk<T> s;
+            Pk<T> x;
 #pragma unroll
             for (int u = 0; u < Pk<T>::N; ++u) {
-                s.v[u] = r.v[u] - v.v[u] * al;
-                d[0].add(s.v[u], s.v[u]);
+                x.v[u] = r.v[u] - v.v[u] * al;
+                d[0].add(x.v[u], x.v[u]);
             }
-            stp(S + i, s);
+            stp(S + i, x);
         });
-    vec_reduce<1, T>(a, vb, d, FIN_S, red, &s_flag);
+    write_parts<1, T>(a, d, red);
 }
 
-// X = (X + P*alpha) + S*omega ; R = S - T*omega ; (R, R0), ||R||^2
-// (bicgstab_cpu.c:232-242 and the next iteration's rho, :203; k_vecop_XR).
-// After the ||s|| test fired: X = X + P*alpha only (bicgstab_gpu.c:219-236).
+// omega = (t, s) / (t, t) from the SpMV partials; X = (X + P*alpha) + S*omega;
+// R = S - T*omega; partials (R, R0), ||R||^2 for the next iteration
+// (bicgstab_cpu.c:229-242, :203; k_vecop_XR).  After the ||s|| test fired:
+// X = X + P*alpha only (bicgstab_gpu.c:219-236).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_vec_XR(VecArgs<T> a)
 {
     __shared__ double red[kMaxDots * kWaves];
-    __shared__ int s_flag;
     const VecBlock vb = a.blocks[blockIdx.x];
-    const SolverState *st = a.st + vb.sys;
-    if (st->done) return;
-    const T al = (T)st->alpha, om = (T)st->omega;
+    SolverState s = a.st_in[vb.sys];
+    if (!s.done && !s.s_conv) apply_in<2, T>(a, vb, s, FIN_OMEGA, red);
+    if (!s.done) s.pending = 1;
+    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys] = s;
+    if (s.done) return;
+    const T al = (T)s.alpha, om = (T)s.omega;
     T *__restrict__ X = a.X;
     T *__restrict__ R = a.R;
     const T *__restrict__ P = a.P;
     const T *__restrict__ S = a.S;
     const T *__restrict__ Tv = a.T_;
     const T *__restrict__ R0 = a.R0;
-    Acc<T> d[2];
-    if (st->s_conv) {
+    if (s.s_conv) {
         visit<T>(
             vb.e0, vb.e1, [&](int i) { X[i] = X[i] + P[i] * al; },
             [&](int i) {
@@ -483,43 +465,85 @@ __global__ __launch_bounds__(kThreads) void k_vec_XR(VecArgs<T> a)
                 for (int u = 0; u < Pk<T>::N; ++u) x.v[u] = x.v[u] + p.v[u] * al;
                 stp(X + i, x);
             });
-    } else {
-        visit<T>(
-            vb.e0, vb.e1,
-            [&](int i) {
-                X[i] = (X[i] + P[i] * al) + S[i] * om;
-                const T r = S[i] - Tv[i] * om;
-                R[i] = r;
-                d[0].add(r, R0[i]);
-                d[1].add(r, r);
-            },
-            [&](int i) {
-                Pk<T> x = ldp(X + i);
-                const Pk<T> p = ldp(P + i), s = ldp(S + i), t = ldp(Tv + i), r0 = ldp(R0 + i);
-                Pk<T> r;
-#pragma unroll
-                for (int u = 0; u < Pk<T>::N; ++u) {
-                    x.v[u] = (x.v[u] + p.v[u] * al) + s.v[u] * om;
-                    r.v[u] = s.v[u] - t.v[u] * om;
-                    d[0].add(r.v[u], r0.v[u]);
-                    d[1].add(r.v[u], r.v[u]);
-                }
-                stp(X + i, x);
-                stp(R + i, r);
-            });
+        return;
     }
-    vec_reduce<2, T>(a, vb, d, FIN_XR, red, &s_flag);
+    Acc<T> d[2];
+    visit<T>(
+        vb.e0, vb.e1,
+        [&](int i) {
+            X[i] = (X[i] + P[i] * al) + S[i] * om;
+            const T r = S[i] - Tv[i] * om;
+            R[i] = r;
+            d[0].add(r, R0[i]);
+            d[1].add(r, r);
+        },
+        [&](int i) {
+            Pk<T> x = ldp(X + i);
+            const Pk<T> p = ldp(P + i), sv = ldp(S + i), t = ldp(Tv + i), r0 = ldp(R0 + i);
+            Pk<T> r;
+#pragma unroll
+            for (int u = 0; u < Pk<T>::N; ++u) {
+                x.v[u] = (x.v[u] + p.v[u] * al) + sv.v[u] * om;
+                r.v[u] = sv.v[u] - t.v[u] * om;
+                d[0].add(r.v[u], r0.v[u]);
+                d[1].add(r.v[u], r.v[u]);
+            }
+            stp(X + i, x);
+            stp(R + i, r);
+        });
+    write_parts<2, T>(a, d, red);
 }
 
-// R0 = R ; P = V = 0  (bicgstab_cpu.c:180-188)
+// Apply a pending FIN_XR in place, one workgroup per system (end of a chunk
+// of iterations, so the host sees the state of the last iteration).
+__global__ __launch_bounds__(kThreads) void k_apply_xr(SolverState *st, const SysRange *sys, const double *part,
+                                                      int nvb)
+{
+    __shared__ double red[kMaxDots * kWaves];
+    const int q = blockIdx.x;
+    SolverState s = st[q];
+    if (s.done || !s.pending) return;
+    const SysRange sr = sys[q];
+    double v[2];
+    reduce_parts<2>(part, nvb, sr.v0, sr.v1, v, red);
+    apply_fin(s, FIN_XR, v[0], v[1]);
+    __syncthreads();
+    if (threadIdx.x == 0) st[q] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_finalize(SolverState *st, const SysRange *sys, const double *part,
+                                                      int nparts, int in_spmv, int fin)
+{
+    __shared__ double red[kMaxDots * kWaves];
+    const int q = blockIdx.x;
+    SolverState s = st[q];
+    const SysRange sr = sys[q];
+    double v[1];
+    if (in_spmv) reduce_parts<1>(part, nparts, sr.g0, sr.g1, v, red);
+    else reduce_parts<1>(part, nparts, sr.v0, sr.v1, v, red);
+    apply_fin(s, fin, v[0], 0.0);
+    __syncthreads();
+    if (threadIdx.x == 0) st[q] = s;
+}
+
+// R0 = R ; P = V = 0 (bicgstab_cpu.c:180-188); partials (R, R0) = ||R||^2
+// for the first iteration's rho and the threshold tol * ||r0||, pending in
+// st_out.
 __global__ __launch_bounds__(kThreads) void k_init_vectors(VecArgs<double> a)
 {
+    __shared__ double red[kMaxDots * kWaves];
     const VecBlock vb = a.blocks[blockIdx.x];
+    Acc<double> d[2];
     for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) {
-        const_cast<double *>(a.R0)[i] = a.R[i];
+        const double r = a.R[i];
+        const_cast<double *>(a.R0)[i] = r;
         a.P[i] = 0.0;
         a.V[i] = 0.0;
+        d[0].add(r, r);
     }
+    d[1] = d[0];
+    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys].pending = 1;
+    write_parts<2, double>(a, d, red);
 }
 
 // Mixed sweep start (FP32 inner solve of A d = r / ||r||).
@@ -527,11 +551,10 @@ __global__ __launch_bounds__(kThreads) void k_ir_start(VecArgs<float> a, const d
                                                       const SolverState *outer)
 {
     __shared__ double red[kMaxDots * kWaves];
-    __shared__ int s_flag;
     const VecBlock vb = a.blocks[blockIdx.x];
     const double nr = sqrt(outer[vb.sys].resid2);
     const double inv = nr > 0.0 ? 1.0 / nr : 0.0;
-    Acc<float> d[1];
+    Acc<float> d[2];
     for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) {
         const float r = (float)(r64[i] * inv);
         a.R[i] = r;
@@ -541,32 +564,33 @@ __global__ __launch_bounds__(kThreads) void k_ir_start(VecArgs<float> a, const d
         a.V[i] = 0.f;
         d[0].add(r, r);
     }
-    if (threadIdx.x == 0 && (int)blockIdx.x == a.sys[vb.sys].v0) a.st[vb.sys].scale = nr;
-    vec_reduce<1, float>(a, vb, d, FIN_INIT, red, &s_flag);
+    d[1] = d[0];
+    if (first_of_sys(a, vb) && threadIdx.x == 0) {
+        a.st_out[vb.sys].scale = nr;
+        a.st_out[vb.sys].pending = 1;
+    }
+    write_parts<2, float>(a, d, red);
 }
 
 __global__ __launch_bounds__(kThreads) void k_ir_update(VecArgs<float> a, double *x64)
 {
     const VecBlock vb = a.blocks[blockIdx.x];
-    const double sc = a.st[vb.sys].scale;
+    const double sc = a.st_in[vb.sys].scale;
     for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) x64[i] = x64[i] + (double)a.X[i] * sc;
 }
 
-__global__ __launch_bounds__(kThreads) void k_dot(VecArgs<double> a, const double *x, const double *y,
-                                                 int fin)
+__global__ __launch_bounds__(kThreads) void k_dot(VecArgs<double> a, const double *x, const double *y)
 {
     __shared__ double red[kMaxDots * kWaves];
-    __shared__ int s_flag;
     const VecBlock vb = a.blocks[blockIdx.x];
     Acc<double> d[1];
     for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) d[0].add(x[i], y[i]);
-    vec_reduce<1, double>(a, vb, d, fin, red, &s_flag);
+    write_parts<1, double>(a, d, red);
 }
 
-__global__ void k_state_init(SolverState *st, int nsys, double tol, int maxit, int fixed,
-                             unsigned *ndone)
+__global__ void k_state_init(SolverState *st, int nstates, double tol, int maxit, int fixed)
 {
-    for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
+    for (int q = threadIdx.x; q < nstates; q += blockDim.x) {
         SolverState z{};
         z.rho = 1.0;
         z.alpha = 1.0;
@@ -574,150 +598,32 @@ __global__ void k_state_init(SolverState *st, int nsys, double tol, int maxit, i
         z.tol = tol;
         z.maxit = maxit;
         z.fixed = fixed;
-        z.done = maxit <= 0 ? 1 : 0;
-        st[s] = z;
+        z.iter = -1;
+        st[q] = z;
     }
-    if (threadIdx.x == 0 && ndone) *ndone = 0;
-}
-
-// ---- BLAS-1 and helpers ---------------------------------------------------
-
-__global__ void k_axpy(int n, double a, const double *x, double *y)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        y[i] = y[i] + a * x[i];
-}
-__global__ void k_scal(int n, double a, double *x)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        x[i] = x[i] * a;
-}
-__global__ void k_fill_zero(unsigned *p, size_t nwords, unsigned char *tail, int ntail)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords;
-         i += (size_t)gridDim.x * blockDim.x)
-        p[i] = 0u;
-    if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
-}
-__global__ void k_copy(size_t n, const double *src, double *dst)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
-}
-__global__ void k_to_f32(int n, const double *src, float *dst)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        dst[i] = (float)src[i];
-}
-__global__ void k_gather(int n, const double *in, const int *idx, double *out)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        out[i] = in[idx[i]];
-}
-__global__ void k_scatter(int n, const double *in, const int *idx, double *out)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        out[idx[i]] = in[i];
-}
-
-// g[j] = -lambda[row(j)] * x[col[j]] over the CSR pattern; one wave per row
-// batch, rows of up to 64 entries per pass.
-__global__ __launch_bounds__(kThreads) void k_grad_pattern(int m, const int *rowptr, const int *col,
-                                                          const double *lambda, const double *x,
-                                                          double *g)
-{
-    const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * kThreads + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * kThreads) >> 6;
-    for (int r = wave; r < m; r += nwaves) {
-        const double lr = -lambda[r];
-        for (int j = rowptr[r] + lane; j < rowptr[r + 1]; j += 64) g[j] = lr * x[col[j]];
-    }
-}
-
-// Y[r, c] = sum_j val[j] * X[col[j], c], row-major with ld = k.  `tile`
-// lanes share one row (one RHS column each), in CSR order per column, so
-// every column equals the single-RHS order-exact sum.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_spmm(int m, const int *__restrict__ rowptr,
-                                                  const int *__restrict__ col,
-                                                  const T *__restrict__ val, int k,
-                                                  const T *__restrict__ X, T *__restrict__ Y, int tile)
-{
-    const int lane = threadIdx.x & (tile - 1);
-    const int rows_per_block = kThreads / tile;
-    const int r = blockIdx.x * rows_per_block + threadIdx.x / tile;
-    const int c = blockIdx.y * tile + lane;
-    if (r >= m || c >= k) return;
-    T acc = T(0);
-    const int e = rowptr[r + 1];
-    for (int j = rowptr[r]; j < e; ++j) acc += val[j] * X[(size_t)col[j] * k + c];
-    Y[(size_t)r * k + c] = acc;
-}
-
-// ---- reference-shaped kernels (hhp_dp_kernels.cu:16-74) -------------------
-
-__global__ void k_dp_vecop_P(double *P, const double *V, const double *R, const double *omega,
-                             const double *beta, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) P[i] = (P[i] - (*omega) * V[i]) * (*beta) + R[i];
-}
-__global__ void k_dp_vecop_S(double *S, const double *R, const double *V, const double *neg_alpha,
-                             int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) S[i] = R[i] + (*neg_alpha) * V[i];
-}
-__global__ void k_dp_vecop_XR(double *X, double *R, const double *P, const double *S, const double *T,
-                              const double *alpha, const double *omega, const double *neg_omega, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        X[i] += (*alpha) * P[i] + (*omega) * S[i];
-        R[i] = S[i] + (*neg_omega) * T[i];
-    }
-}
-__global__ void k_dp_scalar(int which, double *a, double *b, const double *c, const double *d,
-                            const double *e)
-{
-    if (which == 0) {            // beta = (rho_new/rho_old)*(alpha/omega); rho_old = rho_new
-        *a = (*c / *b) * (*d / *e);
-        *b = *c;
-    } else if (which == 1) {     // alpha = rho/rv; neg = -alpha
-        const double v = *c / *d;
-        *a = v;
-        *b = -v;
-    } else if (which == 2) {     // omega = ts/tt; neg = -omega
-        const double v = *c / *d;
-        *a = v;
-        *b = -v;
-    } else {                     // total = a + b
-        *a = *c + *d;
-    }
-}
-
-inline int grid_for(size_t n, int per_block = kThreads, int cap = 8192)
-{
-    size_t g = (n + per_block - 1) / per_block;
-    if (g < 1) g = 1;
-    if (g > (size_t)cap) g = cap;
-    return (int)g;
 }
 
 template <typename T>
 hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, hipStream_t s)
 {
-    if (a.nblk <= 0) return hipSuccess;
+    if (a.ngroups <= 0) return hipSuccess;
     const size_t shmem = (size_t)block_nnz * sizeof(T);
     switch (epi) {
-    case EPI_AXPBY: k_spmv<T, EPI_AXPBY><<<a.nblk, kThreads, shmem, s>>>(a); break;
-    case EPI_DOT1: k_spmv<T, EPI_DOT1><<<a.nblk, kThreads, shmem, s>>>(a); break;
-    case EPI_DOT2: k_spmv<T, EPI_DOT2><<<a.nblk, kThreads, shmem, s>>>(a); break;
-    case EPI_RESID: k_spmv<T, EPI_RESID><<<a.nblk, kThreads, shmem, s>>>(a); break;
+    case EPI_AXPBY: k_spmv<T, EPI_AXPBY><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    case EPI_DOT1: k_spmv<T, EPI_DOT1><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    case EPI_DOT2: k_spmv<T, EPI_DOT2><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    case EPI_RESID: k_spmv<T, EPI_RESID><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    case EPI_RESID2: k_spmv<T, EPI_RESID2><<<a.ngroups, kThreads, shmem, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t chunk_dispatch(const ChunkArgs<T> &a, hipStream_t s)
+{
+    if (a.nchunks <= 0) return hipSuccess;
+    k_chunks<T><<<a.nchunks, kThreads, 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -731,6 +637,8 @@ hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, hipStre
 {
     return spmv_dispatch<float>(a, epi, block_nnz, s);
 }
+hipError_t launch_chunks(const ChunkArgs<double> &a, int, hipStream_t s) { return chunk_dispatch<double>(a, s); }
+hipError_t launch_chunks(const ChunkArgs<float> &a, int, hipStream_t s) { return chunk_dispatch<float>(a, s); }
 
 #define HBS_VEC_LAUNCH(NAME, KER, T)                                   \
     hipError_t NAME(const VecArgs<T> &a, hipStream_t s)                \
@@ -747,6 +655,22 @@ HBS_VEC_LAUNCH(launch_vec_S, k_vec_S, float)
 HBS_VEC_LAUNCH(launch_vec_XR, k_vec_XR, float)
 #undef HBS_VEC_LAUNCH
 
+hipError_t launch_apply_xr(SolverState *st, const SysRange *sys, int nsys, const double *part, int nvb,
+                           hipStream_t s)
+{
+    if (nsys <= 0) return hipSuccess;
+    k_apply_xr<<<nsys, kThreads, 0, s>>>(st, sys, part, nvb);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(SolverState *st, const SysRange *sys, int nsys, const double *part, int nparts,
+                           int in_spmv, int fin, hipStream_t s)
+{
+    if (nsys <= 0) return hipSuccess;
+    k_finalize<<<nsys, kThreads, 0, s>>>(st, sys, part, nparts, in_spmv, fin);
+    return hipGetLastError();
+}
+
 hipError_t launch_init_vectors(const VecArgs<double> &a, hipStream_t s)
 {
     if (a.nvb <= 0) return hipSuccess;
@@ -754,8 +678,7 @@ hipError_t launch_init_vectors(const VecArgs<double> &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_ir_start(const VecArgs<float> &a, const double *r64, const SolverState *outer,
-                           hipStream_t s)
+hipError_t launch_ir_start(const VecArgs<float> &a, const double *r64, const SolverState *outer, hipStream_t s)
 {
     if (a.nvb <= 0) return hipSuccess;
     k_ir_start<<<a.nvb, kThreads, 0, s>>>(a, r64, outer);
@@ -769,119 +692,16 @@ hipError_t launch_ir_update(const VecArgs<float> &a, double *x64, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_dot(const VecArgs<double> &a, const double *x, const double *y, int fin,
-                      hipStream_t s)
+hipError_t launch_dot(const VecArgs<double> &a, const double *x, const double *y, hipStream_t s)
 {
     if (a.nvb <= 0) return hipSuccess;
-    k_dot<<<a.nvb, kThreads, 0, s>>>(a, x, y, fin);
+    k_dot<<<a.nvb, kThreads, 0, s>>>(a, x, y);
     return hipGetLastError();
 }
 
-hipError_t launch_state_init(SolverState *st, int nsys, double tol, int maxit, int fixed,
-                             unsigned *ndone, hipStream_t s)
+hipError_t launch_state_init(SolverState *st, int nstates, double tol, int maxit, int fixed, hipStream_t s)
 {
-    k_state_init<<<1, kThreads, 0, s>>>(st, nsys, tol, maxit, fixed, ndone);
-    return hipGetLastError();
-}
-
-hipError_t launch_axpy_f64(int n, double a, const double *x, double *y, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    k_axpy<<<grid_for(n), kThreads, 0, s>>>(n, a, x, y);
-    return hipGetLastError();
-}
-hipError_t launch_scal_f64(int n, double a, double *x, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    k_scal<<<grid_for(n), kThreads, 0, s>>>(n, a, x);
-    return hipGetLastError();
-}
-hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s)
-{
-    if (bytes == 0) return hipSuccess;
-    const size_t nwords = bytes / 4;
-    const int ntail = (int)(bytes % 4);
-    k_fill_zero<<<grid_for(nwords > 0 ? nwords : 1), kThreads, 0, s>>>(
-        (unsigned *)p, nwords, (unsigned char *)p + nwords * 4, ntail);
-    return hipGetLastError();
-}
-hipError_t launch_copy_f64(size_t n, const double *src, double *dst, hipStream_t s)
-{
-    if (n == 0) return hipSuccess;
-    k_copy<<<grid_for(n), kThreads, 0, s>>>(n, src, dst);
-    return hipGetLastError();
-}
-hipError_t launch_to_f32(int n, const double *src, float *dst, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    k_to_f32<<<grid_for(n), kThreads, 0, s>>>(n, src, dst);
-    return hipGetLastError();
-}
-hipError_t launch_gather_f64(int n, const double *in, const int *idx, double *out, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    k_gather<<<grid_for(n), kThreads, 0, s>>>(n, in, idx, out);
-    return hipGetLastError();
-}
-hipError_t launch_scatter_f64(int n, const double *in, const int *idx, double *out, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    k_scatter<<<grid_for(n), kThreads, 0, s>>>(n, in, idx, out);
-    return hipGetLastError();
-}
-hipError_t launch_grad_pattern(int m, const int *rowptr, const int *col, const double *lambda,
-                               const double *x, double *g, hipStream_t s)
-{
-    if (m <= 0) return hipSuccess;
-    k_grad_pattern<<<grid_for((size_t)m * 64), kThreads, 0, s>>>(m, rowptr, col, lambda, x, g);
-    return hipGetLastError();
-}
-hipError_t launch_spmm_f64(int m, const int *rowptr, const int *col, const double *val, int k,
-                           const double *X, double *Y, int tile, hipStream_t s)
-{
-    if (m <= 0 || k <= 0) return hipSuccess;
-    const int rpb = kThreads / tile;
-    dim3 grid((m + rpb - 1) / rpb, (k + tile - 1) / tile);
-    k_spmm<double><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, X, Y, tile);
-    return hipGetLastError();
-}
-hipError_t launch_spmm_f32(int m, const int *rowptr, const int *col, const float *val, int k,
-                           const float *X, float *Y, int tile, hipStream_t s)
-{
-    if (m <= 0 || k <= 0) return hipSuccess;
-    const int rpb = kThreads / tile;
-    dim3 grid((m + rpb - 1) / rpb, (k + tile - 1) / tile);
-    k_spmm<float><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, X, Y, tile);
-    return hipGetLastError();
-}
-
-hipError_t launch_dp_vecop_P(double *P, const double *V, const double *R, const double *omega,
-                             const double *beta, int n, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    k_dp_vecop_P<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(P, V, R, omega, beta, n);
-    return hipGetLastError();
-}
-hipError_t launch_dp_vecop_S(double *S, const double *R, const double *V, const double *neg_alpha,
-                             int n, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    k_dp_vecop_S<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(S, R, V, neg_alpha, n);
-    return hipGetLastError();
-}
-hipError_t launch_dp_vecop_XR(double *X, double *R, const double *P, const double *S,
-                              const double *T, const double *alpha, const double *omega,
-                              const double *neg_omega, int n, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    k_dp_vecop_XR<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(X, R, P, S, T, alpha, omega,
-                                                                     neg_omega, n);
-    return hipGetLastError();
-}
-hipError_t launch_dp_scalar(int which, double *a, double *b, const double *c, const double *d,
-                            const double *e, hipStream_t s)
-{
-    k_dp_scalar<<<1, 1, 0, s>>>(which, a, b, c, d, e);
+    k_state_init<<<1, kThreads, 0, s>>>(st, nstates, tol, maxit, fixed);
     return hipGetLastError();
 }
 

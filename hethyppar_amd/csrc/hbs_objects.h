@@ -1,0 +1,144 @@
+// hbs_objects.h -- the library's handle types and host-side helpers shared by
+// hbs_api.cpp, hbs_solver.cpp and hbs_shard.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hbs_err.h"
+#include "hbs_internal.h"
+
+namespace hbs {
+
+#define HIPCHK(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return ::hbs::set_error(HBS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                    __FILE__, __LINE__);                                                \
+    } while (0)
+
+#define CHK(expr)                      \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != HBS_OK) return rc_; \
+    } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(HBS_ERR_ALLOC, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    return HBS_OK;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <typename T>
+int upload(T *dev, const std::vector<T> &host)
+{
+    if (host.empty()) return HBS_OK;
+    HIPCHK(hipMemcpy(dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+    return HBS_OK;
+}
+
+struct DevGuard {
+    explicit DevGuard(int d) { (void)hipSetDevice(d); }
+};
+
+// Vector workgroups of n elements (one system) on the device.
+struct DevVecPlan {
+    int n = -1;
+    int nvb = 0;
+    VecBlock *blocks = nullptr;
+    SysRange *sys = nullptr;    // {0, 0, 0, nvb}
+    double *part = nullptr;     // [nvb * kMaxDots]
+    int build(int n);
+    void release();
+};
+
+}  // namespace hbs
+
+struct hbs_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hbs::SolverState *st = nullptr;   // device, BLAS-1 results
+    hbs::SolverState *hst = nullptr;  // pinned host
+    hbs::DevVecPlan vp;               // cached plan for BLAS-1 dots
+};
+
+struct hbs_csr_s {
+    hbs_ctx ctx = nullptr;
+    int m = 0, n = 0, nnz = 0, flags = 0;
+    int *rowptr = nullptr, *col = nullptr;
+    double *val = nullptr;
+    float *val32 = nullptr;
+    hbs::SpmvPlan plan;
+    hbs::SysRange *sys = nullptr;     // {0, ngroups, 0, 0}
+    double *chunk_sums = nullptr;     // [nchunks]
+    double *part = nullptr;           // [ngroups * kMaxDots]
+    hbs_csr_s *tr = nullptr;
+    std::map<std::pair<int, int>, int> spmm_tile;   // (k, dtype) -> tile
+    hipEvent_t *evs = nullptr;
+    int nevs = 0;
+};
+
+namespace hbs {
+
+// SpMV launch arguments of matrix a with values val (FP64 or FP32 copy).
+template <typename T>
+SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
+{
+    SpmvArgs<T> s{};
+    s.blocks = a->plan.blocks;
+    s.aux = a->plan.aux;
+    s.groups = a->plan.groups;
+    s.ngroups = a->plan.ngroups;
+    s.sys = a->sys;
+    s.rowptr = a->rowptr;
+    s.col = a->col;
+    s.val = val;
+    s.chunk_sums = a->chunk_sums;
+    s.part_out = a->part;
+    s.alpha = T(1);
+    s.beta = T(0);
+    s.fin_in = FIN_NONE;
+    return s;
+}
+
+// Queue one SpMV: the long-row chunk sums (if any) then the row blocks.
+template <typename T>
+hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a, int epi, const SolverState *chunk_st, hipStream_t q)
+{
+    if (a->plan.nchunks > 0) {
+        ChunkArgs<T> c{};
+        c.chunks = a->plan.chunks;
+        c.nchunks = a->plan.nchunks;
+        c.col = s.col;
+        c.val = s.val;
+        c.x = s.x;
+        c.chunk_sums = const_cast<double *>(s.chunk_sums);
+        c.st = chunk_st;
+        hipError_t e = launch_chunks(c, a->plan.block_nnz, q);
+        if (e != hipSuccess) return e;
+    }
+    return launch_spmv(s, epi, a->plan.block_nnz, q);
+}
+
+}  // namespace hbs

@@ -1,4 +1,4 @@
-// hbs_plan.cpp -- row-block and vector-block planning for the HIP kernels.
+// hbs_plan.cpp -- row-block, workgroup and vector-block planning.
 #include <algorithm>
 
 #include "hbs_internal.h"
@@ -18,38 +18,35 @@ int pow2_floor(int v)
 
 // Greedy row blocking: a block takes consecutive rows while their entries fit
 // in block_nnz (the LDS product buffer).  A row longer than block_nnz becomes
-// a run of chunk blocks.  Blocks whose average row length is at most
-// serial_max_avg use the order-exact serial row sum; others get
-// pow2_floor(256 / rows) lanes per row (2..64).
+// one kind-1 block whose value is the ordered sum of its chunk sums, each
+// chunk (<= block_nnz entries) summed by its own workgroup of k_chunks.
+// Blocks whose average row length is at most serial_max_avg use the
+// order-exact serial row sum; others pow2_floor(256 / rows) lanes per row.
 void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max_avg, int sys,
-               std::vector<RowBlock> &blocks, std::vector<BlockAux> &aux, int *serial_rows,
-               int *max_row)
+               HostPlan &plan)
 {
     const int end = row0 + m;
     int r = row0;
     while (r < end) {
         const int len = rowptr[r + 1] - rowptr[r];
-        if (max_row && len > *max_row) *max_row = len;
+        plan.max_row = std::max(plan.max_row, len);
         if (len > block_nnz) {
             const int nch = (len + block_nnz - 1) / block_nnz;
-            const int first = (int)blocks.size();
+            const int first = (int)plan.chunks.size();
             for (int c = 0; c < nch; ++c) {
                 const int j0 = rowptr[r] + c * block_nnz;
-                const int j1 = std::min(j0 + block_nnz, rowptr[r + 1]);
-                blocks.push_back(RowBlock{r, r + 1, j0, j1});
-                aux.push_back(BlockAux{c, nch, first, 1 | (sys << 8)});
+                plan.chunks.push_back(Chunk{j0, std::min(j0 + block_nnz, rowptr[r + 1]), r, sys});
             }
+            plan.blocks.push_back(RowBlock{r, r + 1, rowptr[r], rowptr[r + 1]});
+            plan.aux.push_back(BlockAux{1, first, nch, 1});
             ++r;
             continue;
         }
         const int r0 = r;
         const int j0 = rowptr[r];
         while (r < end && rowptr[r + 1] - j0 <= block_nnz && r - r0 < kMaxRowsPerBlock) {
+            plan.max_row = std::max(plan.max_row, rowptr[r + 1] - rowptr[r]);
             ++r;
-            if (r < end && max_row) {
-                const int l = rowptr[r + 1] - rowptr[r];
-                if (l > *max_row) *max_row = l;
-            }
         }
         const int nr = r - r0;
         const int nz = rowptr[r] - j0;
@@ -58,9 +55,23 @@ void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max
             lanes = std::min(64, pow2_floor(kThreads / nr));
             if (lanes < 2) lanes = 1;
         }
-        if (lanes == 1 && serial_rows) *serial_rows += nr;
-        blocks.push_back(RowBlock{r0, r, j0, rowptr[r]});
-        aux.push_back(BlockAux{-1, 0, 0, lanes | (sys << 8)});
+        if (lanes == 1) plan.serial_rows += nr;
+        plan.blocks.push_back(RowBlock{r0, r, j0, rowptr[r]});
+        plan.aux.push_back(BlockAux{0, 0, 0, lanes});
+    }
+}
+
+// Contiguous, equal-count split of row blocks [rb0, rb1) over ngroups
+// workgroups (row blocks carry about block_nnz entries each).
+void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan)
+{
+    const int nb = rb1 - rb0;
+    if (nb <= 0) return;
+    ngroups = std::max(1, std::min(ngroups, nb));
+    for (int p = 0; p < ngroups; ++p) {
+        const int a = rb0 + (int)((long long)nb * p / ngroups);
+        const int b = rb0 + (int)((long long)nb * (p + 1) / ngroups);
+        plan.groups.push_back(Group{a, b, sys, 0});
     }
 }
 
@@ -68,6 +79,16 @@ void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock>
 {
     for (int e = e0; e < e0 + n; e += elems_per_block)
         blocks.push_back(VecBlock{sys, e, std::min(e + elems_per_block, e0 + n), 0});
+}
+
+// Elements per vector workgroup: a multiple of 64 (whole 16-byte packs for
+// FP64 and FP32), at least 512, and few enough workgroups (<= kMaxGroups)
+// that a consumer sums their partials cheaply.
+int vec_elems_per_block(int n)
+{
+    int e = (n + kMaxGroups - 1) / kMaxGroups;
+    e = (e + 63) / 64 * 64;
+    return std::max(512, e);
 }
 
 }  // namespace hbs

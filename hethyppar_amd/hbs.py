@@ -128,6 +128,10 @@ _PROTOS = {
     "hbs_bench_spmv_f32": (_i, [_vp, _vp, _vp, _i, _dp]),
     "hbs_bench_copy": (_i, [_vp, _vp, _vp, C.c_size_t, _i, _dp]),
     "hbs_set_spmv_plan": (_i, [_i, _i]),
+    "hbs_set_spmv_groups": (_i, [_i]),
+    "hbs_csr_plan_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
+    "hbs_shard_pack_host": (_i, [_vp, _vp, _dp]),
+    "hbs_shard_spmv2": (_i, [_vp, _vp, _dp, _vp, _d, _d, _vp]),
 }
 
 _lib = None
@@ -448,7 +452,11 @@ class DeviceCSR:
     def info(self):
         v = [C.c_int() for _ in range(5)]
         check(lib().hbs_csr_info(self.h, *[C.byref(x) for x in v]))
-        return dict(zip(["m", "n", "nnz", "nblocks", "serial_rows"], [x.value for x in v]))
+        d = dict(zip(["m", "n", "nnz", "nblocks", "serial_rows"], [x.value for x in v]))
+        w = [C.c_int() for _ in range(4)]
+        check(lib().hbs_csr_plan_info(self.h, *[C.byref(x) for x in w]))
+        d.update(zip(["ngroups", "nchunks", "block_nnz", "max_row"], [x.value for x in w]))
+        return d
 
     def spmv(self, x, y, alpha=1.0, beta=0.0):
         if x.dtype == np.float32:
@@ -585,5 +593,6 @@ class Shard:
             pass
 
 
-def set_spmv_plan(block_nnz=2048, serial_max_avg=64):
+def set_spmv_plan(block_nnz=2048, serial_max_avg=64, max_groups=1024):
     check(lib().hbs_set_spmv_plan(block_nnz, serial_max_avg))
+    check(lib().hbs_set_spmv_groups(max_groups))

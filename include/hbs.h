@@ -101,7 +101,7 @@ typedef struct {
     double tol;          /* stop when ||s|| or ||r|| < tol*||r0||            */
     int maxit;           /* iteration cap                                    */
     int fixed_iters;     /* 1: run exactly maxit iterations, no stop test    */
-    int check_every;     /* iterations per convergence poll (0 = default 8)  */
+    int check_every;     /* iterations per convergence poll (0 = default 4)  */
     int use_graph;       /* 1: replay hipGraph-captured iteration chunks     */
     int inner_maxit;     /* mixed: FP32 inner iteration cap per sweep        */
     double inner_tol;    /* mixed: FP32 inner relative tolerance per sweep   */
@@ -272,6 +272,12 @@ int hbs_bench_copy(hbs_ctx ctx, const double *src_dev, double *dst_dev, size_t n
 /* nnz per row block (<= 4096) and the average row length above which a
  * block uses several lanes per row instead of the order-exact serial sum. */
 int hbs_set_spmv_plan(int block_nnz, int serial_max_avg);
+/* Workgroups of the SpMV kernel (<= 1024); each takes a contiguous run of
+ * row blocks, and its dot partial is one of those a consumer sums. */
+int hbs_set_spmv_groups(int max_groups);
+/* Plan of a device CSR: SpMV workgroups, long-row chunks, nnz per row
+ * block, longest row. */
+int hbs_csr_plan_info(hbs_csr a, int *ngroups, int *nchunks, int *block_nnz, int *max_row);
 
 #ifdef __cplusplus
 }
