@@ -139,6 +139,7 @@ struct VecArgs {
     const VecBlock *blocks;
     const SysRange *sys;
     int nvb;
+    int upb;                    // 16-byte packs per thread (1, 2, 4, 8)
     T *X, *R, *P, *V, *S, *T_;
     const T *R0;
     const SolverState *st_in;
@@ -166,6 +167,7 @@ void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan);
 // Vector blocks of elements [e0, e0 + n) of one system.
 void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock> &blocks);
 int vec_elems_per_block(int n);
+int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 
 // ---- launchers (hbs_kernels.hip) ----
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, hipStream_t s);

@@ -14,13 +14,15 @@
 //               products BiCGStab needs after each SpMV.
 //  * k_vec_*    the three BiCGStab vector updates (reference k_vecop_P/S/XR,
 //               hhp_dp_kernels.cu:51-74), each fused with the reductions of
-//               bicgstab_cpu.c:203,219,229,242 that follow it.
+//               bicgstab_cpu.c:203,219,229,242 that follow it.  Each thread
+//               loads its 16-byte packs first, so the loads overlap the
+//               prologue that derives the scalars.
 //  * reductions atomic-free and deterministic: a producer writes one partial
-//               per workgroup (<= 1024); every workgroup of the consumer
-//               kernel sums them in the same fixed order, so all of them
-//               derive bit-identical scalars (alpha, omega, beta, stop
-//               decisions) in their prologue.  The first workgroup of a
-//               system stores the updated scalars in the other state slot.
+//               per workgroup; every workgroup of the consumer kernel sums
+//               them in the same fixed order, so all of them derive
+//               bit-identical scalars (alpha, omega, beta, stop decisions) in
+//               their prologue.  The first workgroup of a system stores the
+//               updated scalars in the other state slot.
 #include <hip/hip_runtime.h>
 
 #include "hbs_internal.h"
@@ -188,7 +190,7 @@ struct EpiDots {
 
 // Row epilogue: write y[r] and feed the fused dots.
 template <typename T, int EPI>
-__device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, Acc<T> *d)
+__device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, Acc<T> &d0, Acc<T> &d1)
 {
     if constexpr (EPI == EPI_AXPBY) {
         T out = a.alpha * acc;
@@ -196,16 +198,16 @@ __device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, Acc<
         a.y[r] = out;
     } else if constexpr (EPI == EPI_DOT1) {
         a.y[r] = acc;
-        d[0].add(a.w[r], acc);
+        d0.add(a.w[r], acc);
     } else if constexpr (EPI == EPI_DOT2) {
         a.y[r] = acc;
-        d[0].add(acc, a.w[r]);
-        d[1].add(acc, acc);
+        d0.add(acc, a.w[r]);
+        d1.add(acc, acc);
     } else {
         const T out = a.w[r] - acc;
         a.y[r] = out;
-        d[0].add(out, out);
-        if constexpr (EPI == EPI_RESID2) d[1].add(out, out);
+        d0.add(out, out);
+        if constexpr (EPI == EPI_RESID2) d1.add(out, out);
     }
 }
 
@@ -215,28 +217,33 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
     extern __shared__ __align__(16) unsigned char smem[];
     T *lds = reinterpret_cast<T *>(smem);
     __shared__ double red[kMaxDots * kWaves];
+    __shared__ SolverState sh;
     constexpr int ND = EpiDots<EPI>::n;
 
     const int p = xcd_remap(blockIdx.x, a.ngroups);
     const Group g = a.groups[p];
     if (a.st_in) {
-        SolverState s = a.st_in[g.sys];
-        if (a.check_done && s.done) {
-            if (a.st_out && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_out[g.sys] = s;
+        if (threadIdx.x == 0) sh = a.st_in[g.sys];
+        __syncthreads();
+        if (a.check_done && sh.done) {
+            if (a.st_out && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_out[g.sys] = sh;
             return;
         }
         if (a.fin_in == FIN_S) {
             const SysRange sr = a.sys[g.sys];
             double v[1];
             reduce_parts<1>(a.part_in, a.nparts_in, sr.v0, sr.v1, v, red);
-            apply_fin(s, FIN_S, v[0], 0.0);
-            if (p == sr.g0 && threadIdx.x == 0) a.st_out[g.sys] = s;
-            if (s.s_conv) return;
+            if (threadIdx.x == 0) {
+                apply_fin(sh, FIN_S, v[0], 0.0);
+                if (p == sr.g0) a.st_out[g.sys] = sh;
+            }
+            __syncthreads();
+            if (sh.s_conv) return;
         }
     }
     if (a.st_mark && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_mark[g.sys].pending = 1;
 
-    Acc<T> d[kMaxDots];
+    Acc<T> d0, d1;
     const T *__restrict__ x = a.x;
     for (int b = g.rb0; b < g.rb1; ++b) {
         const RowBlock rb = a.blocks[b];
@@ -246,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
             if (threadIdx.x == 0) {
                 T total = T(0);
                 for (int c = 0; c < ax.nchunks; ++c) total += (T)a.chunk_sums[ax.first + c];
-                row_out<T, EPI>(a, rb.r0, total, d);
+                row_out<T, EPI>(a, rb.r0, total, d0, d1);
             }
             continue;
         }
@@ -266,7 +273,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
                 const int s = a.rowptr[r] - rb.j0, e = a.rowptr[r + 1] - rb.j0;
                 T acc = T(0);
                 for (int q = s; q < e; ++q) acc += lds[q];
-                row_out<T, EPI>(a, r, acc, d);
+                row_out<T, EPI>(a, r, acc, d0, d1);
             }
         } else {
             const int gi = threadIdx.x / lanes, l = threadIdx.x & (lanes - 1);
@@ -280,7 +287,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
                     for (int q = s + l; q < e; q += lanes) acc += lds[q];
                 }
                 for (int o = lanes >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-                if (t < nr && l == 0) row_out<T, EPI>(a, r, acc, d);
+                if (t < nr && l == 0) row_out<T, EPI>(a, r, acc, d0, d1);
             }
         }
         __syncthreads();   // LDS is reused by the next row block
@@ -288,8 +295,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
 
     if constexpr (ND > 0) {
         double v[ND];
-#pragma unroll
-        for (int q = 0; q < ND; ++q) v[q] = d[q].get();
+        v[0] = d0.get();
+        if constexpr (ND > 1) v[1] = d1.get();
         block_sum<ND>(v, red);
         if (threadIdx.x == 0) {
 #pragma unroll
@@ -318,27 +325,37 @@ struct alignas(16) Pk {
     static constexpr int N = 16 / sizeof(T);
     T v[N];
 };
+
+// Load / store the 16-byte pack at element idx, masked against e1.
 template <typename T>
-__device__ __forceinline__ Pk<T> ldp(const T *p)
+__device__ __forceinline__ Pk<T> ldm(const T *p, int idx, int e1)
 {
-    return *reinterpret_cast<const Pk<T> *>(p);
+    Pk<T> r;
+    if (idx + Pk<T>::N <= e1) {
+        r = *reinterpret_cast<const Pk<T> *>(p + idx);
+    } else {
+#pragma unroll
+        for (int k = 0; k < Pk<T>::N; ++k) r.v[k] = idx + k < e1 ? p[idx + k] : T(0);
+    }
+    return r;
 }
 template <typename T>
-__device__ __forceinline__ void stp(T *p, const Pk<T> &x)
+__device__ __forceinline__ void stm(T *p, int idx, int e1, const Pk<T> &x)
 {
-    *reinterpret_cast<Pk<T> *>(p) = x;
+    if (idx + Pk<T>::N <= e1) {
+        *reinterpret_cast<Pk<T> *>(p + idx) = x;
+    } else {
+#pragma unroll
+        for (int k = 0; k < Pk<T>::N; ++k)
+            if (idx + k < e1) p[idx + k] = x.v[k];
+    }
 }
 
-// Visit [e0, e1): scalar head up to a 16-byte boundary, 16-byte packs, tail.
-template <typename T, typename F, typename FV>
-__device__ __forceinline__ void visit(int e0, int e1, F f, FV fv)
+// Element index of pack u of this thread (e0 is a multiple of the pack width).
+template <typename T>
+__device__ __forceinline__ int pk_idx(int e0, int u)
 {
-    constexpr int N = Pk<T>::N;
-    const int a0 = min(e1, (e0 + N - 1) / N * N);
-    const int a1 = max(a0, e1 / N * N);
-    if ((int)threadIdx.x < a0 - e0) f(e0 + threadIdx.x);
-    for (int i = a0 + (int)threadIdx.x * N; i < a1; i += kThreads * N) fv(i);
-    if ((int)threadIdx.x < e1 - a1) f(a1 + threadIdx.x);
+    return e0 + (u * kThreads + (int)threadIdx.x) * Pk<T>::N;
 }
 
 template <typename T>
@@ -347,23 +364,26 @@ __device__ __forceinline__ bool first_of_sys(const VecArgs<T> &a, const VecBlock
     return (int)blockIdx.x == a.sys[vb.sys].v0;
 }
 
+// Sum the producer's partials of this system (all threads) and apply fin to
+// the shared state (thread 0).
 template <int ND, typename T>
-__device__ __forceinline__ void apply_in(const VecArgs<T> &a, const VecBlock &vb, SolverState &s, int fin,
+__device__ __forceinline__ void apply_in(const VecArgs<T> &a, const VecBlock &vb, SolverState &sh, int fin,
                                          double *red)
 {
     const SysRange sr = a.sys[vb.sys];
     double v[ND];
     if (a.in_spmv) reduce_parts<ND>(a.part_in, a.nparts_in, sr.g0, sr.g1, v, red);
     else reduce_parts<ND>(a.part_in, a.nparts_in, sr.v0, sr.v1, v, red);
-    apply_fin(s, fin, v[0], ND > 1 ? v[ND - 1] : 0.0);
+    if (threadIdx.x == 0) apply_fin(sh, fin, v[0], ND > 1 ? v[ND - 1] : 0.0);
+    __syncthreads();
 }
 
 template <int ND, typename T>
-__device__ __forceinline__ void write_parts(const VecArgs<T> &a, Acc<T> *d, double *red)
+__device__ __forceinline__ void write_parts(const VecArgs<T> &a, Acc<T> &d0, Acc<T> &d1, double *red)
 {
     double v[ND];
-#pragma unroll
-    for (int q = 0; q < ND; ++q) v[q] = d[q].get();
+    v[0] = d0.get();
+    if constexpr (ND > 1) v[1] = d1.get();
     block_sum<ND>(v, red);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -374,124 +394,122 @@ __device__ __forceinline__ void write_parts(const VecArgs<T> &a, Acc<T> *d, doub
 // Start of an iteration: apply the pending (r, r0), ||r||^2 of the previous
 // one (stop tests, beta), then P = ((P - V*omega)*beta) + R
 // (bicgstab_cpu.c:203-213; k_vecop_P).
-template <typename T>
+template <typename T, int U>
 __global__ __launch_bounds__(kThreads) void k_vec_P(VecArgs<T> a)
 {
     __shared__ double red[kMaxDots * kWaves];
+    __shared__ SolverState sh;
     const VecBlock vb = a.blocks[blockIdx.x];
-    SolverState s = a.st_in[vb.sys];
-    if (!s.done && s.pending) apply_in<2, T>(a, vb, s, FIN_XR, red);
-    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys] = s;
-    if (s.done) return;
-    const T om = (T)s.omega, be = (T)s.beta;
-    T *__restrict__ P = a.P;
-    const T *__restrict__ V = a.V;
-    const T *__restrict__ R = a.R;
-    visit<T>(
-        vb.e0, vb.e1, [&](int i) { P[i] = ((P[i] - V[i] * om) * be) + R[i]; },
-        [&](int i) {
-            Pk<T> p = ldp(P + i);
-            const Pk<T> v = ldp(V + i), r = ldp(R + i);
+    if (threadIdx.x == 0) sh = a.st_in[vb.sys];
+    Pk<T> p[U], v[U], r[U];
 #pragma unroll
-            for (int u = 0; u < Pk<T>::N; ++u) p.v[u] = ((p.v[u] - v.v[u] * om) * be) + r.v[u];
-            stp(P + i, p);
-        });
+    for (int u = 0; u < U; ++u) {
+        const int i = pk_idx<T>(vb.e0, u);
+        p[u] = ldm(a.P, i, vb.e1);
+        v[u] = ldm(a.V, i, vb.e1);
+        r[u] = ldm(a.R, i, vb.e1);
+    }
+    __syncthreads();
+    if (!sh.done && sh.pending) apply_in<2, T>(a, vb, sh, FIN_XR, red);
+    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys] = sh;
+    if (sh.done) return;
+    const T om = (T)sh.omega, be = (T)sh.beta;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int k = 0; k < Pk<T>::N; ++k) p[u].v[k] = ((p[u].v[k] - v[u].v[k] * om) * be) + r[u].v[k];
+        stm(a.P, pk_idx<T>(vb.e0, u), vb.e1, p[u]);
+    }
 }
 
 // alpha = rho / (r0, v) from the SpMV partials; S = R - V*alpha, ||S||^2
 // (bicgstab_cpu.c:219-223; k_vecop_S).
-template <typename T>
+template <typename T, int U>
 __global__ __launch_bounds__(kThreads) void k_vec_S(VecArgs<T> a)
 {
     __shared__ double red[kMaxDots * kWaves];
+    __shared__ SolverState sh;
     const VecBlock vb = a.blocks[blockIdx.x];
-    SolverState s = a.st_in[vb.sys];
-    if (!s.done) apply_in<1, T>(a, vb, s, FIN_ALPHA, red);
-    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys] = s;
-    if (s.done) return;
-    const T al = (T)s.alpha;
-    T *__restrict__ S = a.S;
-    const T *__restrict__ V = a.V;
-    const T *__restrict__ R = a.R;
-    Acc<T> d[1];
-    visit<T>(
-        vb.e0, vb.e1,
-        [&](int i) {
-            const T x = R[i] - V[i] * al;
-            S[i] = x;
-            d[0].add(x, x);
-        },
-        [&](int i) {
-            const Pk<T> v = ldp(V + i), r = ldp(R + i);
-            Pk<T> x;
+    if (threadIdx.x == 0) sh = a.st_in[vb.sys];
+    Pk<T> v[U], r[U];
 #pragma unroll
-            for (int u = 0; u < Pk<T>::N; ++u) {
-                x.v[u] = r.v[u] - v.v[u] * al;
-                d[0].add(x.v[u], x.v[u]);
-            }
-            stp(S + i, x);
-        });
-    write_parts<1, T>(a, d, red);
+    for (int u = 0; u < U; ++u) {
+        const int i = pk_idx<T>(vb.e0, u);
+        v[u] = ldm(a.V, i, vb.e1);
+        r[u] = ldm(a.R, i, vb.e1);
+    }
+    __syncthreads();
+    if (!sh.done) apply_in<1, T>(a, vb, sh, FIN_ALPHA, red);
+    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys] = sh;
+    if (sh.done) return;
+    const T al = (T)sh.alpha;
+    Acc<T> d0, d1;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        Pk<T> s;
+#pragma unroll
+        for (int k = 0; k < Pk<T>::N; ++k) {
+            s.v[k] = r[u].v[k] - v[u].v[k] * al;
+            d0.add(s.v[k], s.v[k]);
+        }
+        stm(a.S, pk_idx<T>(vb.e0, u), vb.e1, s);
+    }
+    write_parts<1, T>(a, d0, d1, red);
 }
 
 // omega = (t, s) / (t, t) from the SpMV partials; X = (X + P*alpha) + S*omega;
 // R = S - T*omega; partials (R, R0), ||R||^2 for the next iteration
 // (bicgstab_cpu.c:229-242, :203; k_vecop_XR).  After the ||s|| test fired:
 // X = X + P*alpha only (bicgstab_gpu.c:219-236).
-template <typename T>
+template <typename T, int U>
 __global__ __launch_bounds__(kThreads) void k_vec_XR(VecArgs<T> a)
 {
     __shared__ double red[kMaxDots * kWaves];
+    __shared__ SolverState sh;
     const VecBlock vb = a.blocks[blockIdx.x];
-    SolverState s = a.st_in[vb.sys];
-    if (!s.done && !s.s_conv) apply_in<2, T>(a, vb, s, FIN_OMEGA, red);
-    if (!s.done) s.pending = 1;
-    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys] = s;
-    if (s.done) return;
-    const T al = (T)s.alpha, om = (T)s.omega;
-    T *__restrict__ X = a.X;
-    T *__restrict__ R = a.R;
-    const T *__restrict__ P = a.P;
-    const T *__restrict__ S = a.S;
-    const T *__restrict__ Tv = a.T_;
-    const T *__restrict__ R0 = a.R0;
-    if (s.s_conv) {
-        visit<T>(
-            vb.e0, vb.e1, [&](int i) { X[i] = X[i] + P[i] * al; },
-            [&](int i) {
-                Pk<T> x = ldp(X + i);
-                const Pk<T> p = ldp(P + i);
+    if (threadIdx.x == 0) sh = a.st_in[vb.sys];
+    Pk<T> x[U], p[U], s[U], t[U], r0[U];
 #pragma unroll
-                for (int u = 0; u < Pk<T>::N; ++u) x.v[u] = x.v[u] + p.v[u] * al;
-                stp(X + i, x);
-            });
+    for (int u = 0; u < U; ++u) {
+        const int i = pk_idx<T>(vb.e0, u);
+        x[u] = ldm(a.X, i, vb.e1);
+        p[u] = ldm(a.P, i, vb.e1);
+        s[u] = ldm(a.S, i, vb.e1);
+        t[u] = ldm(a.T_, i, vb.e1);
+        r0[u] = ldm(a.R0, i, vb.e1);
+    }
+    __syncthreads();
+    if (!sh.done && !sh.s_conv) apply_in<2, T>(a, vb, sh, FIN_OMEGA, red);
+    if (threadIdx.x == 0 && !sh.done) sh.pending = 1;
+    __syncthreads();
+    if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys] = sh;
+    if (sh.done) return;
+    const T al = (T)sh.alpha, om = (T)sh.omega;
+    if (sh.s_conv) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int k = 0; k < Pk<T>::N; ++k) x[u].v[k] = x[u].v[k] + p[u].v[k] * al;
+            stm(a.X, pk_idx<T>(vb.e0, u), vb.e1, x[u]);
+        }
         return;
     }
-    Acc<T> d[2];
-    visit<T>(
-        vb.e0, vb.e1,
-        [&](int i) {
-            X[i] = (X[i] + P[i] * al) + S[i] * om;
-            const T r = S[i] - Tv[i] * om;
-            R[i] = r;
-            d[0].add(r, R0[i]);
-            d[1].add(r, r);
-        },
-        [&](int i) {
-            Pk<T> x = ldp(X + i);
-            const Pk<T> p = ldp(P + i), sv = ldp(S + i), t = ldp(Tv + i), r0 = ldp(R0 + i);
-            Pk<T> r;
+    Acc<T> d0, d1;
 #pragma unroll
-            for (int u = 0; u < Pk<T>::N; ++u) {
-                x.v[u] = (x.v[u] + p.v[u] * al) + sv.v[u] * om;
-                r.v[u] = sv.v[u] - t.v[u] * om;
-                d[0].add(r.v[u], r0.v[u]);
-                d[1].add(r.v[u], r.v[u]);
-            }
-            stp(X + i, x);
-            stp(R + i, r);
-        });
-    write_parts<2, T>(a, d, red);
+    for (int u = 0; u < U; ++u) {
+        Pk<T> r;
+#pragma unroll
+        for (int k = 0; k < Pk<T>::N; ++k) {
+            x[u].v[k] = (x[u].v[k] + p[u].v[k] * al) + s[u].v[k] * om;
+            r.v[k] = s[u].v[k] - t[u].v[k] * om;
+            d0.add(r.v[k], r0[u].v[k]);
+            d1.add(r.v[k], r.v[k]);
+        }
+        const int i = pk_idx<T>(vb.e0, u);
+        stm(a.X, i, vb.e1, x[u]);
+        stm(a.R, i, vb.e1, r);
+    }
+    write_parts<2, T>(a, d0, d1, red);
 }
 
 // Apply a pending FIN_XR in place, one workgroup per system (end of a chunk
@@ -500,30 +518,35 @@ __global__ __launch_bounds__(kThreads) void k_apply_xr(SolverState *st, const Sy
                                                       int nvb)
 {
     __shared__ double red[kMaxDots * kWaves];
+    __shared__ SolverState sh;
     const int q = blockIdx.x;
-    SolverState s = st[q];
-    if (s.done || !s.pending) return;
+    if (threadIdx.x == 0) sh = st[q];
+    __syncthreads();
+    if (sh.done || !sh.pending) return;
     const SysRange sr = sys[q];
     double v[2];
     reduce_parts<2>(part, nvb, sr.v0, sr.v1, v, red);
-    apply_fin(s, FIN_XR, v[0], v[1]);
-    __syncthreads();
-    if (threadIdx.x == 0) st[q] = s;
+    if (threadIdx.x == 0) {
+        apply_fin(sh, FIN_XR, v[0], v[1]);
+        st[q] = sh;
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void k_finalize(SolverState *st, const SysRange *sys, const double *part,
                                                       int nparts, int in_spmv, int fin)
 {
     __shared__ double red[kMaxDots * kWaves];
+    __shared__ SolverState sh;
     const int q = blockIdx.x;
-    SolverState s = st[q];
+    if (threadIdx.x == 0) sh = st[q];
     const SysRange sr = sys[q];
     double v[1];
     if (in_spmv) reduce_parts<1>(part, nparts, sr.g0, sr.g1, v, red);
     else reduce_parts<1>(part, nparts, sr.v0, sr.v1, v, red);
-    apply_fin(s, fin, v[0], 0.0);
-    __syncthreads();
-    if (threadIdx.x == 0) st[q] = s;
+    if (threadIdx.x == 0) {
+        apply_fin(sh, fin, v[0], 0.0);
+        st[q] = sh;
+    }
 }
 
 // R0 = R ; P = V = 0 (bicgstab_cpu.c:180-188); partials (R, R0) = ||R||^2
@@ -533,17 +556,17 @@ __global__ __launch_bounds__(kThreads) void k_init_vectors(VecArgs<double> a)
 {
     __shared__ double red[kMaxDots * kWaves];
     const VecBlock vb = a.blocks[blockIdx.x];
-    Acc<double> d[2];
+    Acc<double> d0, d1;
     for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) {
         const double r = a.R[i];
         const_cast<double *>(a.R0)[i] = r;
         a.P[i] = 0.0;
         a.V[i] = 0.0;
-        d[0].add(r, r);
+        d0.add(r, r);
     }
-    d[1] = d[0];
+    d1 = d0;
     if (first_of_sys(a, vb) && threadIdx.x == 0) a.st_out[vb.sys].pending = 1;
-    write_parts<2, double>(a, d, red);
+    write_parts<2, double>(a, d0, d1, red);
 }
 
 // Mixed sweep start (FP32 inner solve of A d = r / ||r||).
@@ -554,7 +577,7 @@ __global__ __launch_bounds__(kThreads) void k_ir_start(VecArgs<float> a, const d
     const VecBlock vb = a.blocks[blockIdx.x];
     const double nr = sqrt(outer[vb.sys].resid2);
     const double inv = nr > 0.0 ? 1.0 / nr : 0.0;
-    Acc<float> d[2];
+    Acc<float> d0, d1;
     for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) {
         const float r = (float)(r64[i] * inv);
         a.R[i] = r;
@@ -562,14 +585,14 @@ __global__ __launch_bounds__(kThreads) void k_ir_start(VecArgs<float> a, const d
         a.X[i] = 0.f;
         a.P[i] = 0.f;
         a.V[i] = 0.f;
-        d[0].add(r, r);
+        d0.add(r, r);
     }
-    d[1] = d[0];
+    d1 = d0;
     if (first_of_sys(a, vb) && threadIdx.x == 0) {
         a.st_out[vb.sys].scale = nr;
         a.st_out[vb.sys].pending = 1;
     }
-    write_parts<2, float>(a, d, red);
+    write_parts<2, float>(a, d0, d1, red);
 }
 
 __global__ __launch_bounds__(kThreads) void k_ir_update(VecArgs<float> a, double *x64)
@@ -583,9 +606,9 @@ __global__ __launch_bounds__(kThreads) void k_dot(VecArgs<double> a, const doubl
 {
     __shared__ double red[kMaxDots * kWaves];
     const VecBlock vb = a.blocks[blockIdx.x];
-    Acc<double> d[1];
-    for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) d[0].add(x[i], y[i]);
-    write_parts<1, double>(a, d, red);
+    Acc<double> d0, d1;
+    for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) d0.add(x[i], y[i]);
+    write_parts<1, double>(a, d0, d1, red);
 }
 
 __global__ void k_state_init(SolverState *st, int nstates, double tol, int maxit, int fixed)
@@ -627,6 +650,26 @@ hipError_t chunk_dispatch(const ChunkArgs<T> &a, hipStream_t s)
     return hipGetLastError();
 }
 
+// a.upb = 16-byte packs per thread = ceil(elements per block / (256 * pack)).
+#define HBS_VEC_KERNEL(NAME)                                                      \
+    template <typename T>                                                          \
+    hipError_t NAME##_dispatch(const VecArgs<T> &a, hipStream_t s)                 \
+    {                                                                              \
+        if (a.nvb <= 0) return hipSuccess;                                         \
+        switch (a.upb) {                                                           \
+        case 1: NAME<T, 1><<<a.nvb, kThreads, 0, s>>>(a); break;                   \
+        case 2: NAME<T, 2><<<a.nvb, kThreads, 0, s>>>(a); break;                   \
+        case 4: NAME<T, 4><<<a.nvb, kThreads, 0, s>>>(a); break;                   \
+        case 8: NAME<T, 8><<<a.nvb, kThreads, 0, s>>>(a); break;                   \
+        default: return hipErrorInvalidValue;                                      \
+        }                                                                          \
+        return hipGetLastError();                                                  \
+    }
+HBS_VEC_KERNEL(k_vec_P)
+HBS_VEC_KERNEL(k_vec_S)
+HBS_VEC_KERNEL(k_vec_XR)
+#undef HBS_VEC_KERNEL
+
 }  // namespace
 
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, hipStream_t s)
@@ -640,20 +683,12 @@ hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, hipStre
 hipError_t launch_chunks(const ChunkArgs<double> &a, int, hipStream_t s) { return chunk_dispatch<double>(a, s); }
 hipError_t launch_chunks(const ChunkArgs<float> &a, int, hipStream_t s) { return chunk_dispatch<float>(a, s); }
 
-#define HBS_VEC_LAUNCH(NAME, KER, T)                                   \
-    hipError_t NAME(const VecArgs<T> &a, hipStream_t s)                \
-    {                                                                  \
-        if (a.nvb <= 0) return hipSuccess;                             \
-        KER<T><<<a.nvb, kThreads, 0, s>>>(a);                          \
-        return hipGetLastError();                                      \
-    }
-HBS_VEC_LAUNCH(launch_vec_P, k_vec_P, double)
-HBS_VEC_LAUNCH(launch_vec_S, k_vec_S, double)
-HBS_VEC_LAUNCH(launch_vec_XR, k_vec_XR, double)
-HBS_VEC_LAUNCH(launch_vec_P, k_vec_P, float)
-HBS_VEC_LAUNCH(launch_vec_S, k_vec_S, float)
-HBS_VEC_LAUNCH(launch_vec_XR, k_vec_XR, float)
-#undef HBS_VEC_LAUNCH
+hipError_t launch_vec_P(const VecArgs<double> &a, hipStream_t s) { return k_vec_P_dispatch<double>(a, s); }
+hipError_t launch_vec_S(const VecArgs<double> &a, hipStream_t s) { return k_vec_S_dispatch<double>(a, s); }
+hipError_t launch_vec_XR(const VecArgs<double> &a, hipStream_t s) { return k_vec_XR_dispatch<double>(a, s); }
+hipError_t launch_vec_P(const VecArgs<float> &a, hipStream_t s) { return k_vec_P_dispatch<float>(a, s); }
+hipError_t launch_vec_S(const VecArgs<float> &a, hipStream_t s) { return k_vec_S_dispatch<float>(a, s); }
+hipError_t launch_vec_XR(const VecArgs<float> &a, hipStream_t s) { return k_vec_XR_dispatch<float>(a, s); }
 
 hipError_t launch_apply_xr(SolverState *st, const SysRange *sys, int nsys, const double *part, int nvb,
                            hipStream_t s)

@@ -81,14 +81,21 @@ void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock>
         blocks.push_back(VecBlock{sys, e, std::min(e + elems_per_block, e0 + n), 0});
 }
 
-// Elements per vector workgroup: a multiple of 64 (whole 16-byte packs for
-// FP64 and FP32), at least 512, and few enough workgroups (<= kMaxGroups)
-// that a consumer sums their partials cheaply.
+// Elements per vector workgroup: 512 * u with u in {1, 2, 4, 8}, the
+// smallest that keeps the workgroups (and the partials a consumer sums) near
+// 2048 or fewer.  An FP64 thread then holds u 16-byte packs, an FP32 thread
+// max(1, u/2).
 int vec_elems_per_block(int n)
 {
-    int e = (n + kMaxGroups - 1) / kMaxGroups;
-    e = (e + 63) / 64 * 64;
-    return std::max(512, e);
+    int u = 1;
+    while (u < 8 && (long long)n > 512LL * u * 2048) u *= 2;
+    return 512 * u;
+}
+
+int vec_packs_per_thread(int elems_per_block, int elem_bytes)
+{
+    const int per_pack = 16 / elem_bytes;
+    return std::max(1, (elems_per_block + kThreads * per_pack - 1) / (kThreads * per_pack));
 }
 
 }  // namespace hbs

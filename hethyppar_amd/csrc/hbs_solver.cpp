@@ -144,6 +144,7 @@ VecArgs<T> vec_args(hbs_solver s, const Work<T> &w)
     a.blocks = s->vblocks;
     a.sys = s->sys;
     a.nvb = s->nvb;
+    a.upb = vec_packs_per_thread(vec_elems_per_block(s->n), (int)sizeof(T));
     a.X = w.X;
     a.R = w.R;
     a.P = w.P;
