@@ -21,6 +21,7 @@ thread_local std::string g_err;
 int g_block_nnz = 2048;
 int g_serial_max_avg = 64;
 int g_max_groups = kMaxGroups;
+int g_stream_nt = 0;
 }  // namespace
 
 #define fail ::hbs::set_error
@@ -268,8 +269,14 @@ extern "C" int hbs_set_spmv_plan(int block_nnz, int serial_max_avg)
 
 extern "C" int hbs_set_spmv_groups(int max_groups)
 {
-    if (max_groups < 1 || max_groups > kMaxGroups) return fail(HBS_ERR_ARG, "max_groups must be in [1, %d]", kMaxGroups);
+    if (max_groups < 1 || max_groups > 65536) return fail(HBS_ERR_ARG, "max_groups must be in [1, 65536]");
     g_max_groups = max_groups;
+    return HBS_OK;
+}
+
+extern "C" int hbs_set_spmv_stream_nt(int on)
+{
+    g_stream_nt = on ? 1 : 0;
     return HBS_OK;
 }
 
@@ -344,6 +351,7 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     plan_rows(0, m, rowptr, g_block_nnz, g_serial_max_avg, 0, hp);
     plan_groups(0, (int)hp.blocks.size(), g_max_groups, 0, hp);
     SpmvPlan &p = a->plan;
+    a->stream_nt = g_stream_nt;
     p.block_nnz = g_block_nnz;
     p.nblk = (int)hp.blocks.size();
     p.ngroups = (int)hp.groups.size();

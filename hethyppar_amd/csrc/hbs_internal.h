@@ -16,7 +16,7 @@ constexpr int kThreads = 256;          // workgroup size of every kernel (4 wave
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxBlockNnz = 4096;     // upper bound of nnz per SpMV row block
 constexpr int kMaxDots = 2;            // reductions fused into one pass
-constexpr int kMaxGroups = 1024;       // workgroups of a reducing kernel = partials a consumer sums
+constexpr int kMaxGroups = 4096;       // SpMV workgroups by default = partials a consumer sums
 
 // One SpMV row block: rows [r0, r1) with entries [j0, j1).
 struct alignas(16) RowBlock {
@@ -119,6 +119,7 @@ struct SpmvArgs {
     SolverState *st_out;
     int fin_in;                 // FIN_NONE or FIN_S
     int check_done;             // return when the system is done (or ||s|| fired)
+    int stream_nt;              // load col/val non-temporally
     SolverState *st_mark;       // start of a solve: set pending in this slot
 };
 

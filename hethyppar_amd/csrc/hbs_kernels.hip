@@ -261,8 +261,15 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
         const int nz = rb.j1 - rb.j0;
         const int *__restrict__ colp = a.col + rb.j0;
         const T *__restrict__ valp = a.val + rb.j0;
+        if (a.stream_nt) {
+            // col/val are read once: non-temporal, so they do not push x out of L2
 #pragma unroll 4
-        for (int k = threadIdx.x; k < nz; k += kThreads) lds[k] = valp[k] * x[colp[k]];
+            for (int k = threadIdx.x; k < nz; k += kThreads)
+                lds[k] = __builtin_nontemporal_load(valp + k) * x[__builtin_nontemporal_load(colp + k)];
+        } else {
+#pragma unroll 4
+            for (int k = threadIdx.x; k < nz; k += kThreads) lds[k] = valp[k] * x[colp[k]];
+        }
         __syncthreads();
         const int nr = rb.r1 - rb.r0;
         const int lanes = ax.lanes;

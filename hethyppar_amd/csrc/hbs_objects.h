@@ -95,6 +95,7 @@ struct hbs_csr_s {
     double *part = nullptr;           // [ngroups * kMaxDots]
     hbs_csr_s *tr = nullptr;
     std::map<std::pair<int, int>, int> spmm_tile;   // (k, dtype) -> tile
+    int stream_nt = 0;                // non-temporal col/val loads
     hipEvent_t *evs = nullptr;
     int nevs = 0;
 };
@@ -116,6 +117,7 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
     s.val = val;
     s.chunk_sums = a->chunk_sums;
     s.part_out = a->part;
+    s.stream_nt = a->stream_nt;
     s.alpha = T(1);
     s.beta = T(0);
     s.fin_in = FIN_NONE;

@@ -275,6 +275,9 @@ int hbs_set_spmv_plan(int block_nnz, int serial_max_avg);
 /* Workgroups of the SpMV kernel (<= 1024); each takes a contiguous run of
  * row blocks, and its dot partial is one of those a consumer sums. */
 int hbs_set_spmv_groups(int max_groups);
+/* 1: load the matrix streams (col, val) non-temporally (default 0: measured
+ * slower on MI355X for every workload tried). */
+int hbs_set_spmv_stream_nt(int on);
 /* Plan of a device CSR: SpMV workgroups, long-row chunks, nnz per row
  * block, longest row. */
 int hbs_csr_plan_info(hbs_csr a, int *ngroups, int *nchunks, int *block_nnz, int *max_row);
