@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="only the SpMV launches and the copy probe (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -93,6 +95,9 @@ def main():
 
     ctx = hbs.Context(0)
     A = hbs.DeviceCSR(ctx, rp, c, v)
+    if args.probe:
+        probe(hbs, ctx, A, xt, n, nnz)
+        return
     S = hbs.Solver(A)
     db = ctx.to_device(b)
     dx = ctx.empty(n)
@@ -172,6 +177,24 @@ def main():
         log("bench: CPU baseline (oracle, 1 thread)")
         result["cpu_baseline"] = cpu_baseline(rp, c, v, b)
     print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+def probe(hbs, ctx, A, xt, n, nnz):
+    """The launches the counter profiles read: 1 + 10 FP64 SpMV launches of the
+    bench matrix, then 1 + 3 copies of 2^25 doubles (256 MiB read and 256 MiB
+    written per launch), the scale of FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+    dx, dy = ctx.to_device(xt), ctx.empty(n)
+    ms = A.bench_spmv(dx, dy, 10)
+    cn = 1 << 25
+    s, d = ctx.empty(cn), ctx.empty(cn)
+    s.zero()
+    cms = C.c_double()
+    hbs.check(hbs.lib().hbs_bench_copy(ctx.h, s.ptr, d.ptr, cn, 3, C.byref(cms)))
+    print(json.dumps({"probe": "spmv+copy", "n": n, "nnz": nnz, "spmv_ms": ms,
+                      "spmv_bytes": spmv_bytes(n, n, nnz), "copy_doubles": cn,
+                      "copy_ms": cms.value}), flush=True)
     ctx.close()
 
 
