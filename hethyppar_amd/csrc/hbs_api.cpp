@@ -296,6 +296,7 @@ extern "C" int hbs_csr_destroy(hbs_csr a)
     dfree(a->sys);
     dfree(a->chunk_sums);
     dfree(a->part);
+    dfree(a->tr_perm);
     if (a->evs) {
         for (int i = 0; i < a->nevs; ++i) (void)hipEventDestroy(a->evs[i]);
         delete[] a->evs;
@@ -319,8 +320,11 @@ int validate_csr(int m, int n, int nnz, const int *rowptr, const int *col, const
     return HBS_OK;
 }
 
+// Systems: sys_rows[nsys + 1] row offsets of independent diagonal blocks
+// (nullptr: one system of all m rows).  Each system gets its own row blocks
+// and a share of the SpMV workgroups proportional to its entries.
 int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int *col, const double *val, int flags,
-               hbs_csr *out)
+               int nsys, const int *sys_rows, hbs_csr *out)
 {
     hbs_csr a = new hbs_csr_s();
     a->ctx = ctx;
@@ -348,9 +352,27 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
             return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
     }
     HostPlan hp;
-    plan_rows(0, m, rowptr, g_block_nnz, g_serial_max_avg, 0, hp);
-    plan_groups(0, (int)hp.blocks.size(), g_max_groups, 0, hp);
+    std::vector<SysRange> ranges;
+    if (!sys_rows) {
+        a->sys_rows = {0, m};
+        nsys = 1;
+    } else {
+        a->sys_rows.assign(sys_rows, sys_rows + nsys + 1);
+    }
+    a->nsys = nsys;
+    for (int q = 0; q < nsys; ++q) {
+        const int r0 = a->sys_rows[(size_t)q], r1 = a->sys_rows[(size_t)q + 1];
+        const int rb0 = (int)hp.blocks.size(), g0 = (int)hp.groups.size();
+        plan_rows(r0, r1 - r0, rowptr, g_block_nnz, g_serial_max_avg, q, hp);
+        const long long nz = (long long)rowptr[r1] - rowptr[r0];
+        const int want = nsys == 1 ? g_max_groups
+                                   : (int)std::max(1LL, nnz > 0 ? (long long)g_max_groups * nz / nnz : 1LL);
+        plan_groups(rb0, (int)hp.blocks.size(), want, q, hp);
+        ranges.push_back(SysRange{g0, (int)hp.groups.size(), 0, 0});
+    }
+    a->sys_host = ranges;
     SpmvPlan &p = a->plan;
+    p.nsys = nsys;
     a->stream_nt = g_stream_nt;
     p.block_nnz = g_block_nnz;
     p.nblk = (int)hp.blocks.size();
@@ -360,12 +382,53 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     p.max_row = hp.max_row;
     if ((rc = dalloc(&p.blocks, hp.blocks.size())) || (rc = dalloc(&p.aux, hp.aux.size())) ||
         (rc = dalloc(&p.chunks, hp.chunks.size())) || (rc = dalloc(&p.groups, hp.groups.size())) ||
-        (rc = dalloc(&a->sys, 1)) || (rc = dalloc(&a->chunk_sums, hp.chunks.size())) ||
+        (rc = dalloc(&a->sys, (size_t)nsys)) || (rc = dalloc(&a->chunk_sums, hp.chunks.size())) ||
         (rc = dalloc(&a->part, (size_t)std::max(p.ngroups, 1) * kMaxDots)))
         return bail(rc);
     if ((rc = upload(p.blocks, hp.blocks)) || (rc = upload(p.aux, hp.aux)) || (rc = upload(p.chunks, hp.chunks)) ||
-        (rc = upload(p.groups, hp.groups)) || (rc = upload(a->sys, std::vector<SysRange>{{0, p.ngroups, 0, 0}})))
+        (rc = upload(p.groups, hp.groups)) || (rc = upload(a->sys, ranges)))
         return bail(rc);
+    *out = a;
+    return HBS_OK;
+}
+
+}  // namespace
+
+namespace {
+
+int csr_create(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int *col, const double *val, int flags,
+               int nsys, const int *sys_rows, hbs_csr *out)
+{
+    if (!ctx || !out) return fail(HBS_ERR_ARG, "null argument");
+    *out = nullptr;
+    CHK(validate_csr(m, n, nnz, rowptr, col, val));
+    DevGuard g(ctx->device);
+    hbs_csr a = nullptr;
+    CHK(csr_upload(ctx, m, n, nnz, rowptr, col, val, flags, nsys, sys_rows, &a));
+    if (flags & HBS_CSR_TRANSPOSE) {
+        // A^T as CSR: counting sort by column, rows visited in ascending order
+        std::vector<int> tptr((size_t)n + 1, 0), tcol((size_t)nnz);
+        std::vector<double> tval((size_t)nnz);
+        for (int j = 0; j < nnz; ++j) tptr[(size_t)col[j] + 1]++;
+        for (int c = 0; c < n; ++c) tptr[(size_t)c + 1] += tptr[(size_t)c];
+        std::vector<int> fill(tptr.begin(), tptr.end() - 1), perm((size_t)nnz);
+        for (int r = 0; r < m; ++r)
+            for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+                const int d = fill[(size_t)col[j]]++;
+                tcol[(size_t)d] = r;
+                tval[(size_t)d] = val[j];
+                perm[(size_t)j] = d;
+            }
+        int rc = csr_upload(ctx, n, m, nnz, tptr.data(), tcol.data(), tval.data(), 0, nsys, sys_rows, &a->tr);
+        if (rc == HBS_OK) rc = dalloc(&a->tr_perm, (size_t)nnz);
+        if (rc == HBS_OK) rc = upload(a->tr_perm, perm);
+        if (rc != HBS_OK) {
+            std::string e = g_err;
+            hbs_csr_destroy(a);
+            g_err = e;
+            return rc;
+        }
+    }
     *out = a;
     return HBS_OK;
 }
@@ -375,34 +438,44 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
 extern "C" int hbs_csr_create(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int *col,
                               const double *val, int flags, hbs_csr *out)
 {
-    if (!ctx || !out) return fail(HBS_ERR_ARG, "null argument");
+    return csr_create(ctx, m, n, nnz, rowptr, col, val, flags, 0, nullptr, out);
+}
+
+extern "C" int hbs_csr_create_systems(hbs_ctx ctx, int nsys, const int *sys_rows, int nnz, const int *rowptr,
+                                      const int *col, const double *val, int flags, hbs_csr *out)
+{
+    if (!out) return fail(HBS_ERR_ARG, "null argument");
     *out = nullptr;
-    CHK(validate_csr(m, n, nnz, rowptr, col, val));
-    DevGuard g(ctx->device);
-    hbs_csr a = nullptr;
-    CHK(csr_upload(ctx, m, n, nnz, rowptr, col, val, flags, &a));
-    if (flags & HBS_CSR_TRANSPOSE) {
-        // A^T as CSR: counting sort by column, rows visited in ascending order
-        std::vector<int> tptr((size_t)n + 1, 0), tcol((size_t)nnz);
-        std::vector<double> tval((size_t)nnz);
-        for (int j = 0; j < nnz; ++j) tptr[(size_t)col[j] + 1]++;
-        for (int c = 0; c < n; ++c) tptr[(size_t)c + 1] += tptr[(size_t)c];
-        std::vector<int> fill(tptr.begin(), tptr.end() - 1);
-        for (int r = 0; r < m; ++r)
-            for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-                const int d = fill[(size_t)col[j]]++;
-                tcol[(size_t)d] = r;
-                tval[(size_t)d] = val[j];
-            }
-        int rc = csr_upload(ctx, n, m, nnz, tptr.data(), tcol.data(), tval.data(), 0, &a->tr);
-        if (rc != HBS_OK) {
-            std::string e = g_err;
-            hbs_csr_destroy(a);
-            g_err = e;
-            return rc;
-        }
-    }
-    *out = a;
+    if (nsys < 1 || !sys_rows) return fail(HBS_ERR_ARG, "need nsys >= 1 and sys_rows[nsys + 1]");
+    if (sys_rows[0] != 0) return fail(HBS_ERR_ARG, "sys_rows[0] must be 0");
+    for (int q = 0; q < nsys; ++q)
+        if (sys_rows[q + 1] <= sys_rows[q]) return fail(HBS_ERR_ARG, "system %d has no rows", q);
+    const int m = sys_rows[nsys];
+    CHK(validate_csr(m, m, nnz, rowptr, col, val));
+    for (int q = 0; q < nsys; ++q)
+        for (int r = sys_rows[q]; r < sys_rows[q + 1]; ++r)
+            for (int j = rowptr[r]; j < rowptr[r + 1]; ++j)
+                if (col[j] < sys_rows[q] || col[j] >= sys_rows[q + 1])
+                    return fail(HBS_ERR_ARG, "entry %d of row %d leaves system %d (column %d)", j, r, q, col[j]);
+    return csr_create(ctx, m, m, nnz, rowptr, col, val, flags, nsys, sys_rows, out);
+}
+
+extern "C" int hbs_csr_set_values(hbs_csr a, const double *val_dev)
+{
+    if (!a || (a->nnz > 0 && !val_dev)) return fail(HBS_ERR_ARG, "bad argument");
+    DevGuard g(a->ctx->device);
+    hipStream_t q = a->ctx->stream;
+    if (a->nnz == 0) return HBS_OK;
+    HIPCHK(hipMemcpyAsync(a->val, val_dev, sizeof(double) * (size_t)a->nnz, hipMemcpyDeviceToDevice, q));
+    if (a->val32) HIPCHK(launch_to_f32(a->nnz, a->val, a->val32, q));
+    if (a->tr) HIPCHK(launch_scatter_f64(a->nnz, a->val, a->tr_perm, a->tr->val, q));
+    return HBS_OK;
+}
+
+extern "C" int hbs_csr_systems(hbs_csr a, int *nsys)
+{
+    if (!a || !nsys) return fail(HBS_ERR_ARG, "null argument");
+    *nsys = a->nsys;
     return HBS_OK;
 }
 

@@ -1,0 +1,277 @@
+// hbs_block.cpp -- BiCGStab on k right-hand sides (hbs_bicgstab_block_f64):
+// one independent BiCGStab per column, columns in groups of up to 64 that
+// share every SpMM pass over A.  Per iteration of a group: k_bvec P, SpMM
+// with (r0, v), alpha, k_bvec S with ||s||^2, the ||s|| test, SpMM with
+// (t, s), (t, t), omega, k_bvec XR with (r, r0), ||r||^2, the ||r|| test --
+// the scalar steps each one workgroup per column.  Iteration chunks are
+// captured into hipGraphs; the host polls the column states once per chunk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "hbs_objects.h"
+
+using namespace hbs;
+
+#define fail ::hbs::set_error
+
+namespace hbs {
+
+struct BlockWork {
+    int n = 0, k = 0;
+    double *X = nullptr, *R = nullptr, *R0 = nullptr, *P = nullptr, *V = nullptr, *S = nullptr, *T = nullptr;
+    SolverState *st = nullptr;    // device [kBlockCols]
+    SolverState *hst = nullptr;   // pinned host [3][kBlockCols]
+    double *part = nullptr;       // [kMaxDots][kBlockCols][ngroups], consumed before the next producer
+    int ngroups = 0, rows_per_group = 0;
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;   // (first column, columns, chunk)
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+void block_work_free(BlockWork *w)
+{
+    if (!w) return;
+    for (double **p : {&w->X, &w->R, &w->R0, &w->P, &w->V, &w->S, &w->T, &w->part}) dfree(*p);
+    dfree(w->st);
+    if (w->hst) (void)hipHostFree(w->hst);
+    for (auto &g : w->graphs) (void)hipGraphExecDestroy(g.second);
+    for (hipEvent_t e : w->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete w;
+}
+
+}  // namespace hbs
+
+namespace {
+
+int pow2_ceil(int v)
+{
+    int p = 1;
+    while (p < v) p *= 2;
+    return p;
+}
+
+int block_work(hbs_solver s, int k, BlockWork **out)
+{
+    BlockWork *w = s->blk;
+    if (w && w->k == k) {
+        *out = w;
+        return HBS_OK;
+    }
+    if (w) {
+        block_work_free(w);
+        s->blk = nullptr;
+    }
+    w = new BlockWork();
+    w->n = s->n;
+    w->k = k;
+    // workgroups of contiguous rows, each pass of 4 waves covers 4 * (64 / kp) rows
+    const int kp = pow2_ceil(std::min(k, kBlockCols));
+    const long long pass = (long long)kWaves * (64 / kp);
+    const long long want = std::min<long long>(2048, (w->n + pass - 1) / pass);
+    w->rows_per_group = (int)((w->n + want - 1) / want);
+    w->ngroups = (w->n + w->rows_per_group - 1) / w->rows_per_group;
+    const size_t nk = (size_t)w->n * k;
+    int rc = HBS_OK;
+    for (double **p : {&w->X, &w->R, &w->R0, &w->P, &w->V, &w->S, &w->T})
+        if (rc == HBS_OK) rc = dalloc(p, nk);
+    if (rc == HBS_OK) rc = dalloc(&w->part, (size_t)kMaxDots * kBlockCols * w->ngroups);
+    if (rc == HBS_OK) rc = dalloc(&w->st, kBlockCols);
+    if (rc == HBS_OK && hipHostMalloc((void **)&w->hst, 3 * sizeof(SolverState) * kBlockCols) != hipSuccess)
+        rc = fail(HBS_ERR_ALLOC, "pinned host allocation failed");
+    for (hipEvent_t &e : w->ev)
+        if (rc == HBS_OK && hipEventCreate(&e) != hipSuccess) rc = fail(HBS_ERR_HIP, "event creation failed");
+    if (rc != HBS_OK) {
+        block_work_free(w);
+        return rc;
+    }
+    s->blk = w;
+    *out = w;
+    return HBS_OK;
+}
+
+BlockArgs group_args(hbs_solver s, BlockWork *w, int c0, int kc)
+{
+    BlockArgs a{};
+    a.n = w->n;
+    a.ld = w->k;
+    a.kc = kc;
+    a.kp = pow2_ceil(kc);
+    a.ngroups = w->ngroups;
+    a.rows_per_group = w->rows_per_group;
+    a.rowptr = s->A->rowptr;
+    a.col = s->A->col;
+    a.val = s->A->val;
+    a.X = w->X + c0;
+    a.R = w->R + c0;
+    a.R0 = w->R0 + c0;
+    a.P = w->P + c0;
+    a.V = w->V + c0;
+    a.S = w->S + c0;
+    a.T = w->T + c0;
+    a.st = w->st;
+    a.skip_done = 1;
+    a.part = w->part;
+    return a;
+}
+
+hipError_t block_iteration(const BlockArgs &g, BlockWork *w, hipStream_t q)
+{
+    hipError_t e;
+    BlockArgs a = g;
+    if ((e = launch_block_vec(a, BVEC_P, q)) != hipSuccess) return e;
+    a.x = g.P;
+    a.y = g.V;
+    a.w = g.R0;
+    if ((e = launch_block_spmm(a, BEPI_DOT1, q)) != hipSuccess) return e;
+    if ((e = launch_block_fin(w->st, g.kc, w->part, w->ngroups, FIN_ALPHA, q)) != hipSuccess) return e;
+    if ((e = launch_block_vec(g, BVEC_S, q)) != hipSuccess) return e;
+    if ((e = launch_block_fin(w->st, g.kc, w->part, w->ngroups, FIN_S, q)) != hipSuccess) return e;
+    a.x = g.S;
+    a.y = g.T;
+    a.w = g.S;
+    if ((e = launch_block_spmm(a, BEPI_DOT2, q)) != hipSuccess) return e;
+    if ((e = launch_block_fin(w->st, g.kc, w->part, w->ngroups, FIN_OMEGA, q)) != hipSuccess) return e;
+    if ((e = launch_block_vec(g, BVEC_XR, q)) != hipSuccess) return e;
+    return launch_block_fin(w->st, g.kc, w->part, w->ngroups, FIN_XR, q);
+}
+
+int block_chunk(const BlockArgs &g, BlockWork *w, int c0, int chunk, bool use_graph, hipStream_t q)
+{
+    if (!use_graph) {
+        for (int i = 0; i < chunk; ++i) HIPCHK(block_iteration(g, w, q));
+        return HBS_OK;
+    }
+    const auto key = std::make_tuple(c0, g.kc, chunk);
+    auto it = w->graphs.find(key);
+    if (it == w->graphs.end()) {
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
+        hipError_t le = hipSuccess;
+        for (int i = 0; i < chunk && le == hipSuccess; ++i) le = block_iteration(g, w, q);
+        hipError_t ce = hipStreamEndCapture(q, &gr);
+        if (le != hipSuccess) return fail(HBS_ERR_HIP, "launch during capture failed: %s", hipGetErrorString(le));
+        if (ce != hipSuccess) return fail(HBS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+        hipGraphExec_t ex = nullptr;
+        hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ie != hipSuccess) return fail(HBS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+        it = w->graphs.emplace(key, ex).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, q));
+    return HBS_OK;
+}
+
+// Chunks of iterations until every column of the group is done; the states
+// of chunk c are read after chunk c + 1 is queued.
+int block_iterate(const BlockArgs &g, BlockWork *w, int c0, int maxit, int check_every, bool use_graph,
+                  hipStream_t q)
+{
+    const int k = check_every > 0 ? check_every : 4;
+    const size_t bytes = sizeof(SolverState) * (size_t)g.kc;
+    long issued = 0;
+    int c = 0, pending = -1;
+    for (;;) {
+        CHK(block_chunk(g, w, c0, k, use_graph, q));
+        issued += k;
+        HIPCHK(hipMemcpyAsync(w->hst + (size_t)(c & 1) * kBlockCols, w->st, bytes, hipMemcpyDeviceToHost, q));
+        HIPCHK(hipEventRecord(w->ev[c & 1], q));
+        if (pending >= 0) {
+            HIPCHK(hipEventSynchronize(w->ev[pending & 1]));
+            const SolverState *h = w->hst + (size_t)(pending & 1) * kBlockCols;
+            bool all = true;
+            for (int i = 0; i < g.kc && all; ++i) all = h[i].done != 0;
+            if (all) break;
+        }
+        pending = c++;
+        if (issued >= maxit) {
+            HIPCHK(hipEventSynchronize(w->ev[pending & 1]));
+            break;
+        }
+    }
+    return HBS_OK;
+}
+
+}  // namespace
+
+extern "C" int hbs_bicgstab_block_f64(hbs_solver s, int k, const double *B, double *X, const hbs_solve_opts *opts,
+                                      hbs_solve_info *infos)
+{
+    if (!s || !infos) return fail(HBS_ERR_ARG, "null argument");
+    if (k < 1) return fail(HBS_ERR_ARG, "need k >= 1 right-hand sides (got %d)", k);
+    if (!B || !X) return fail(HBS_ERR_ARG, "B and X must be device blocks");
+    if ((size_t)s->n * (size_t)k > (size_t)INT32_MAX * 8) return fail(HBS_ERR_ARG, "n * k too large");
+    hbs_solve_opts o;
+    hbs_solve_opts_default(&o);
+    if (opts) o = *opts;
+    if (o.maxit < 0 || !(o.tol >= 0.0)) return fail(HBS_ERR_ARG, "bad tolerance or iteration cap");
+    DevGuard dg(s->A->ctx->device);
+    hipStream_t q = s->A->ctx->stream;
+    BlockWork *w = nullptr;
+    CHK(block_work(s, k, &w));
+    const size_t nk = (size_t)w->n * k;
+    hipEvent_t e0 = s->e0, e1 = s->e1;
+    HIPCHK(hipEventRecord(e0, q));
+    HIPCHK(hipMemcpyAsync(w->X, X, sizeof(double) * nk, hipMemcpyDeviceToDevice, q));
+    std::vector<SolverState> hs((size_t)k);
+    for (int c0 = 0; c0 < k; c0 += kBlockCols) {
+        const int kc = std::min(kBlockCols, k - c0);
+        const BlockArgs g = group_args(s, w, c0, kc);
+        HIPCHK(launch_state_init(w->st, kc, o.tol, o.maxit, o.fixed_iters ? 1 : 0, q));
+        // R = B - A X ; R0 = R ; P = V = 0 ; iteration 0's rho and threshold
+        BlockArgs a = g;
+        a.skip_done = 0;
+        a.x = g.X;
+        a.y = g.R;
+        a.w = B + c0;
+        HIPCHK(launch_block_spmm(a, BEPI_RESID, q));
+        HIPCHK(launch_block_vec(a, BVEC_INIT, q));
+        HIPCHK(launch_block_fin(w->st, kc, w->part, w->ngroups, FIN_XR, q));
+        CHK(block_iterate(g, w, c0, std::max(o.maxit, 1), o.check_every, o.use_graph != 0, q));
+        // sqrt(||B - AX||^2 / ||B||^2) per column
+        a.x = g.X;
+        a.y = g.V;
+        a.w = B + c0;
+        HIPCHK(launch_block_spmm(a, BEPI_RESID, q));
+        HIPCHK(launch_block_fin(w->st, kc, w->part, w->ngroups, FIN_RESID, q));
+        a.x = B + c0;
+        HIPCHK(launch_block_vec(a, BVEC_DOT, q));
+        HIPCHK(launch_block_fin(w->st, kc, w->part, w->ngroups, FIN_STORE, q));
+        SolverState *h = w->hst + 2 * (size_t)kBlockCols;
+        HIPCHK(hipMemcpyAsync(h, w->st, sizeof(SolverState) * kc, hipMemcpyDeviceToHost, q));
+        HIPCHK(hipStreamSynchronize(q));
+        std::memcpy(&hs[(size_t)c0], h, sizeof(SolverState) * kc);
+    }
+    HIPCHK(hipMemcpyAsync(X, w->X, sizeof(double) * nk, hipMemcpyDeviceToDevice, q));
+    HIPCHK(hipEventRecord(e1, q));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    int worst = HBS_OK, wc = 0;
+    for (int c = 0; c < k; ++c) {
+        const SolverState &st = hs[(size_t)c];
+        hbs_solve_info &in = infos[c];
+        in.iters = std::max(st.iter, 0);
+        in.stop = st.stop;
+        in.sweeps = 0;
+        in.status = st.stop == 4 ? HBS_ERR_BREAKDOWN : ((!o.fixed_iters && st.stop == 0) ? HBS_ERR_NOCONV : HBS_OK);
+        in.relres = st.out > 0 ? std::sqrt(st.resid2 / st.out) : std::sqrt(st.resid2);
+        in.rel_r0 = st.rr0 > 0 ? std::sqrt(st.resid2 / st.rr0) : 0.0;
+        in.time_ms = ms;
+        if (in.status != worst && (in.status == HBS_ERR_BREAKDOWN || (in.status == HBS_ERR_NOCONV && worst == HBS_OK))) {
+            worst = in.status;
+            wc = c;
+        }
+    }
+    if (worst == HBS_ERR_BREAKDOWN)
+        return fail(worst, "block BiCGStab: breakdown in column %d after %d iterations", wc, infos[wc].iters);
+    if (worst == HBS_ERR_NOCONV)
+        return fail(worst, "block BiCGStab: column %d reached %d iterations, ||r||/||r0|| = %.3e", wc,
+                    infos[wc].iters, infos[wc].rel_r0);
+    return HBS_OK;
+}

@@ -1,0 +1,221 @@
+// hbs_block.hip -- BiCGStab on k right-hand sides at once: one independent
+// BiCGStab per column, every column sharing each pass over A.
+//
+// The n x k blocks are row-major, so the kc <= 64 columns of a group sit
+// side by side in a wave: lane l works on column l % kp of row l / kp of the
+// wave's pass.  The SpMM reads each row's col/val once for all columns and
+// gathers whole x rows (kc contiguous values); every column's sum runs over
+// the row in CSR order, the same order as the single-vector serial sum.
+// Each column keeps its own SolverState; the scalars are reduced by a small
+// kernel with one workgroup per column (k_bfin), in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "hbs_device.h"
+#include "hbs_internal.h"
+
+namespace hbs {
+namespace {
+
+struct LaneMap {
+    int c;        // column of the group
+    int ro;       // row offset inside the wave's pass
+    int rstep;    // rows per workgroup pass
+    bool col_ok;  // c < kc
+};
+
+__device__ __forceinline__ LaneMap lane_map(const BlockArgs &a)
+{
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int rpw = 64 / a.kp;
+    LaneMap m;
+    m.c = l & (a.kp - 1);
+    m.ro = w * rpw + l / a.kp;
+    m.rstep = kWaves * rpw;
+    m.col_ok = m.c < a.kc;
+    return m;
+}
+
+// Column sums of ND per-lane values -> part[d][c][p], fixed order: xor
+// butterfly over the lanes of one column inside a wave, then waves 0..3.
+template <int ND>
+__device__ __forceinline__ void column_partials(const BlockArgs &a, double (&v)[ND], int p)
+{
+    __shared__ double red[ND * kWaves * 64];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+        for (int o = 32; o >= a.kp; o >>= 1) v[d] += __shfl_xor(v[d], o, 64);
+    if (l < a.kp) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) red[(d * kWaves + w) * 64 + l] = v[d];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < a.kc) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double t = red[(d * kWaves) * 64 + threadIdx.x];
+#pragma unroll
+            for (int k = 1; k < kWaves; ++k) t += red[(d * kWaves + k) * 64 + threadIdx.x];
+            a.part[((size_t)d * kBlockCols + threadIdx.x) * a.ngroups + p] = t;
+        }
+    }
+}
+
+// Whether this lane's column takes part in the pass.
+__device__ __forceinline__ bool column_active(const BlockArgs &a, const LaneMap &m, bool need_not_sconv)
+{
+    if (!m.col_ok) return false;
+    if (!a.skip_done) return true;
+    const SolverState &s = a.st[m.c];
+    return !s.done && !(need_not_sconv && s.s_conv);
+}
+
+template <int EPI>
+__global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a)
+{
+    constexpr int ND = EPI == BEPI_DOT1 ? 1 : 2;
+    const int p = xcd_remap(blockIdx.x, a.ngroups);
+    const LaneMap m = lane_map(a);
+    const bool act = column_active(a, m, EPI == BEPI_DOT2);
+    double v[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) v[d] = 0.0;
+    if (__syncthreads_or(act)) {
+        const int r0 = p * a.rows_per_group;
+        const int r1 = min(a.n, r0 + a.rows_per_group);
+        const double *__restrict__ x = a.x + m.c;
+        for (int r = r0 + m.ro; r < r1; r += m.rstep) {
+            if (!act) continue;
+            const int j1 = a.rowptr[r + 1];
+            double acc = 0.0;
+            for (int j = a.rowptr[r]; j < j1; ++j) acc += a.val[j] * x[(size_t)a.col[j] * a.ld];
+            const size_t i = (size_t)r * a.ld + m.c;
+            if constexpr (EPI == BEPI_DOT1) {
+                a.y[i] = acc;
+                v[0] += a.w[i] * acc;
+            } else if constexpr (EPI == BEPI_DOT2) {
+                a.y[i] = acc;
+                v[0] += acc * a.w[i];
+                v[1] += acc * acc;
+            } else {
+                const double o = a.w[i] - acc;
+                a.y[i] = o;
+                v[0] += o * o;
+                v[1] += o * o;
+            }
+        }
+    }
+    column_partials<ND>(a, v, p);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_bvec(BlockArgs a)
+{
+    constexpr int ND = (KIND == BVEC_XR) ? 2 : ((KIND == BVEC_S || KIND == BVEC_DOT) ? 1 : 0);
+    const int p = xcd_remap(blockIdx.x, a.ngroups);
+    const LaneMap m = lane_map(a);
+    const bool act = column_active(a, m, false);
+    double v[ND > 0 ? ND : 1] = {};
+    if (__syncthreads_or(act)) {
+        double al = 0.0, om = 0.0, be = 0.0;
+        int sconv = 0;
+        if (m.col_ok && (KIND == BVEC_P || KIND == BVEC_S || KIND == BVEC_XR)) {
+            const SolverState &s = a.st[m.c];
+            al = s.alpha;
+            om = s.omega;
+            be = s.beta;
+            sconv = s.s_conv;
+        }
+        const int r0 = p * a.rows_per_group;
+        const int r1 = min(a.n, r0 + a.rows_per_group);
+        for (int r = r0 + m.ro; r < r1; r += m.rstep) {
+            if (!act) continue;
+            const size_t i = (size_t)r * a.ld + m.c;
+            if constexpr (KIND == BVEC_P) {
+                a.P[i] = ((a.P[i] - a.V[i] * om) * be) + a.R[i];
+            } else if constexpr (KIND == BVEC_S) {
+                const double s = a.R[i] - a.V[i] * al;
+                a.S[i] = s;
+                v[0] += s * s;
+            } else if constexpr (KIND == BVEC_XR) {
+                if (sconv) {
+                    a.X[i] = a.X[i] + a.P[i] * al;
+                } else {
+                    const double s = a.S[i];
+                    a.X[i] = (a.X[i] + a.P[i] * al) + s * om;
+                    const double rn = s - a.T[i] * om;
+                    a.R[i] = rn;
+                    v[0] += rn * a.R0[i];
+                    v[ND > 1 ? 1 : 0] += rn * rn;
+                }
+            } else if constexpr (KIND == BVEC_INIT) {
+                const_cast<double *>(a.R0)[i] = a.R[i];
+                a.P[i] = 0.0;
+                a.V[i] = 0.0;
+            } else {
+                const double b = a.x[i];
+                v[0] += b * b;
+            }
+        }
+    }
+    if constexpr (ND > 0) column_partials<ND>(a, v, p);
+}
+
+// One workgroup per column: ordered sum of its partials, then the scalar
+// update of bicgstab_cpu.c / the stop tests (apply_fin).
+__global__ __launch_bounds__(kThreads) void k_bfin(SolverState *st, const double *part, int ngroups, int fin)
+{
+    __shared__ double red[kMaxDots * kWaves];
+    const int c = blockIdx.x;
+    const int nd = (fin == FIN_OMEGA || fin == FIN_XR) ? 2 : 1;
+    double v[2] = {0.0, 0.0};
+    for (int d = 0; d < nd; ++d)
+        for (int i = threadIdx.x; i < ngroups; i += kThreads) v[d] += part[((size_t)d * kBlockCols + c) * ngroups + i];
+    block_sum<2>(v, red);
+    if (threadIdx.x == 0) {
+        SolverState s = st[c];
+        if (fin == FIN_RESID || fin == FIN_STORE) {
+            apply_fin(s, fin, v[0], 0.0);
+        } else if (!s.done && !(fin == FIN_OMEGA && s.s_conv)) {
+            apply_fin(s, fin, v[0], v[1]);
+        }
+        st[c] = s;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_block_spmm(const BlockArgs &a, int epi, hipStream_t s)
+{
+    if (a.ngroups <= 0) return hipSuccess;
+    switch (epi) {
+    case BEPI_DOT1: k_bspmm<BEPI_DOT1><<<a.ngroups, kThreads, 0, s>>>(a); break;
+    case BEPI_DOT2: k_bspmm<BEPI_DOT2><<<a.ngroups, kThreads, 0, s>>>(a); break;
+    case BEPI_RESID: k_bspmm<BEPI_RESID><<<a.ngroups, kThreads, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_block_vec(const BlockArgs &a, int kind, hipStream_t s)
+{
+    if (a.ngroups <= 0) return hipSuccess;
+    switch (kind) {
+    case BVEC_P: k_bvec<BVEC_P><<<a.ngroups, kThreads, 0, s>>>(a); break;
+    case BVEC_S: k_bvec<BVEC_S><<<a.ngroups, kThreads, 0, s>>>(a); break;
+    case BVEC_XR: k_bvec<BVEC_XR><<<a.ngroups, kThreads, 0, s>>>(a); break;
+    case BVEC_INIT: k_bvec<BVEC_INIT><<<a.ngroups, kThreads, 0, s>>>(a); break;
+    case BVEC_DOT: k_bvec<BVEC_DOT><<<a.ngroups, kThreads, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_block_fin(SolverState *st, int kc, const double *part, int ngroups, int fin, hipStream_t s)
+{
+    if (kc <= 0) return hipSuccess;
+    k_bfin<<<kc, kThreads, 0, s>>>(st, part, ngroups, fin);
+    return hipGetLastError();
+}
+
+}  // namespace hbs

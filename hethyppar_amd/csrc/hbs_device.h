@@ -1,0 +1,167 @@
+// hbs_device.h -- device helpers shared by the kernel files: XCD-aware
+// workgroup placement, deterministic wave / workgroup / grid sums, the FP32
+// compensated dot accumulator, and the BiCGStab scalar updates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hbs_internal.h"
+
+namespace hbs {
+namespace {
+
+constexpr int kXcds = 8;   // MI355X: 8 XCDs, workgroups dispatched round robin
+
+// Give each XCD a contiguous range of logical workgroups so neighbouring row
+// blocks (which gather neighbouring x entries) share one XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int n)
+{
+    const int q = n / kXcds, r = n % kXcds;
+    const int x = bid % kXcds, k = bid / kXcds;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// Per-thread dot accumulators.  FP64: sequential sum of rounded products.
+// FP32: Dot2 of Ogita, Rump and Oishi (2005) -- TwoProduct by FMA and Knuth's
+// TwoSum, error terms kept in a second float -- handed on as FP64 to the
+// workgroup and grid sums.
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double a, double b) { s += a * b; }
+    __device__ __forceinline__ double get() const { return s; }
+};
+template <>
+struct Acc<float> {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float a, float b)
+    {
+        const float p = a * b;
+        const float e = __builtin_fmaf(a, b, -p);
+        const float t = s + p;
+        const float z = t - s;
+        const float q = (s - (t - z)) + (p - z);
+        s = t;
+        c += q + e;
+    }
+    __device__ __forceinline__ double get() const { return (double)s + (double)c; }
+};
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Workgroup sum of ND values; every thread gets the totals.  Fixed order:
+// xor butterfly inside each wave, then waves 0..3 in sequence.
+template <int ND>
+__device__ __forceinline__ void block_sum(double (&v)[ND], double *red)
+{
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) v[d] = wave_sum(v[d]);
+    if (l == 0) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) red[d * kWaves + w] = v[d];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double t = red[d * kWaves];
+#pragma unroll
+        for (int k = 1; k < kWaves; ++k) t += red[d * kWaves + k];
+        v[d] = t;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum1(T v, T *red)
+{
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    v = wave_sum(v);
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    T t = red[0];
+#pragma unroll
+    for (int k = 1; k < kWaves; ++k) t += red[k];
+    __syncthreads();
+    return t;
+}
+
+// Sum of partials [p0, p1) (stride nparts between the ND rows), fixed order.
+template <int ND>
+__device__ __forceinline__ void reduce_parts(const double *part, int nparts, int p0, int p1, double (&v)[ND],
+                                             double *red)
+{
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double acc = 0.0;
+        for (int i = p0 + (int)threadIdx.x; i < p1; i += kThreads) acc += part[(size_t)d * nparts + i];
+        v[d] = acc;
+    }
+    block_sum<ND>(v, red);
+}
+
+__device__ __forceinline__ void set_done(SolverState &s, int stop)
+{
+    s.done = 1;
+    s.stop = stop;
+}
+
+// Scalar updates (identical in every workgroup).  Order of operations of
+// bicgstab_cpu.c:203-229; stop tests of bicgstab_gpu.c:148-150,231-236,289-293.
+__device__ void apply_fin(SolverState &s, int fin, double d0, double d1)
+{
+    switch (fin) {
+    case FIN_ALPHA:
+        s.rv = d0;
+        s.alpha = s.rho / d0;
+        if (d0 == 0.0 || !isfinite(s.alpha)) set_done(s, 4);
+        break;
+    case FIN_OMEGA:
+        s.ts = d0;
+        s.tt = d1;
+        s.omega = d0 / d1;
+        if (d1 == 0.0 || !isfinite(s.omega)) set_done(s, 4);
+        break;
+    case FIN_XR:
+        // end of iteration `iter` (or, with iter == -1, the initial residual)
+        s.pending = 0;
+        s.iter += 1;
+        if (s.iter == 0) {
+            s.rr0 = d1;
+            s.thr = s.tol * sqrt(d1);
+        }
+        if (s.s_conv) { set_done(s, 1); break; }
+        s.rho_next = d0;
+        s.rr = d1;
+        if (s.iter > 0 && !s.fixed && sqrt(d1) < s.thr) { set_done(s, 2); break; }
+        if (d1 == 0.0) { set_done(s, 2); break; }
+        if (s.iter >= s.maxit) { set_done(s, 0); break; }
+        if (d0 == 0.0 || !isfinite(d0)) { set_done(s, 4); break; }
+        s.beta = (d0 / s.rho) * (s.alpha / s.omega);
+        s.rho = d0;
+        break;
+    case FIN_S:
+        s.ss = d0;
+        if (!s.fixed && sqrt(d0) < s.thr) s.s_conv = 1;
+        break;
+    case FIN_RESID:
+        s.resid2 = d0;
+        break;
+    case FIN_STORE:
+        s.out = d0;
+        break;
+    default:
+        break;
+    }
+}
+
+}  // namespace
+}  // namespace hbs

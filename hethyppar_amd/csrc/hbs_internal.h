@@ -36,9 +36,10 @@ struct alignas(16) Chunk {
 struct alignas(16) Group {
     int rb0, rb1, sys, pad;
 };
-// Vector workgroup: elements [e0, e1) of system sys.
+// Vector workgroup: elements [e0, e1) of system sys; its 16-byte packs start
+// at base (e0 rounded down to a multiple of 4 elements).
 struct alignas(16) VecBlock {
-    int sys, e0, e1, pad;
+    int sys, e0, e1, base;
 };
 // Per system: SpMV workgroups [g0, g1) and vector workgroups [v0, v1).
 struct alignas(16) SysRange {
@@ -150,6 +151,44 @@ struct VecArgs {
     int in_spmv;                // 1: part_in from an SpMV (ranges g0..g1), 0: from a vector kernel
     double *part_out;           // [nvb * kMaxDots]
 };
+
+// ---- k right-hand sides (hbs_block.hip) ----
+// A group of kc <= 64 columns of row-major n x ld blocks (pointers already
+// offset to the group's first column).  A wave holds 64 / kp rows x kp
+// columns (kp = kc rounded up to a power of two): lane l works on column
+// l % kp of row offset l / kp, so one row's columns are contiguous in HBM.
+constexpr int kBlockCols = 64;
+struct BlockArgs {
+    int n, ld, kc, kp;
+    int ngroups;                  // workgroups = partials per column
+    int rows_per_group;
+    const int *rowptr, *col;
+    const double *val;
+    const double *x;              // SpMM input
+    double *y;                    // SpMM output
+    const double *w;              // SpMM epilogue operand
+    double *X, *R, *P, *V, *S, *T;
+    const double *R0;
+    const SolverState *st;        // [kc], read-only in the streaming kernels
+    int skip_done;                // 1: columns whose state is done are skipped
+    double *part;                 // [kMaxDots][kBlockCols][ngroups]
+};
+enum BlockEpi : int {
+    BEPI_DOT1 = 0,    // y = A x ; d0 = (w, y)
+    BEPI_DOT2 = 1,    // y = A x ; d0 = (y, w), d1 = (y, y)   (skips columns with s_conv)
+    BEPI_RESID = 2,   // y = w - A x ; d0 = d1 = (y, y)
+};
+enum BlockVec : int {
+    BVEC_P = 0,       // P = ((P - V omega) beta) + R
+    BVEC_S = 1,       // S = R - V alpha ; d0 = (S, S)
+    BVEC_XR = 2,      // X = (X + P alpha) + S omega ; R = S - T omega ; d0 = (R, R0), d1 = (R, R)
+    BVEC_INIT = 3,    // R0 = R ; P = V = 0
+    BVEC_DOT = 4,     // d0 = (x, x) of a.x
+};
+hipError_t launch_block_spmm(const BlockArgs &a, int epi, hipStream_t s);
+hipError_t launch_block_vec(const BlockArgs &a, int kind, hipStream_t s);
+// Reduce the partials of every column into st[c] with fin (one workgroup per column).
+hipError_t launch_block_fin(SolverState *st, int kc, const double *part, int ngroups, int fin, hipStream_t s);
 
 // ---- host-side planning (hbs_plan.cpp) ----
 struct HostPlan {

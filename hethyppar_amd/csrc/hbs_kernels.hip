@@ -25,163 +25,11 @@
 //               updated scalars in the other state slot.
 #include <hip/hip_runtime.h>
 
+#include "hbs_device.h"
 #include "hbs_internal.h"
 
 namespace hbs {
 namespace {
-
-constexpr int kXcds = 8;   // MI355X: 8 XCDs, workgroups dispatched round robin
-
-// Give each XCD a contiguous range of logical workgroups so neighbouring row
-// blocks (which gather neighbouring x entries) share one XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int n)
-{
-    const int q = n / kXcds, r = n % kXcds;
-    const int x = bid % kXcds, k = bid / kXcds;
-    return x * q + (x < r ? x : r) + k;
-}
-
-// Per-thread dot accumulators.  FP64: sequential sum of rounded products.
-// FP32: Dot2 of Ogita, Rump and Oishi (2005) -- TwoProduct by FMA and Knuth's
-// TwoSum, error terms kept in a second float -- handed on as FP64 to the
-// workgroup and grid sums.
-template <typename T>
-struct Acc;
-template <>
-struct Acc<double> {
-    double s = 0.0;
-    __device__ __forceinline__ void add(double a, double b) { s += a * b; }
-    __device__ __forceinline__ double get() const { return s; }
-};
-template <>
-struct Acc<float> {
-    float s = 0.f, c = 0.f;
-    __device__ __forceinline__ void add(float a, float b)
-    {
-        const float p = a * b;
-        const float e = __builtin_fmaf(a, b, -p);
-        const float t = s + p;
-        const float z = t - s;
-        const float q = (s - (t - z)) + (p - z);
-        s = t;
-        c += q + e;
-    }
-    __device__ __forceinline__ double get() const { return (double)s + (double)c; }
-};
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Workgroup sum of ND values; every thread gets the totals.  Fixed order:
-// xor butterfly inside each wave, then waves 0..3 in sequence.
-template <int ND>
-__device__ __forceinline__ void block_sum(double (&v)[ND], double *red)
-{
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 0; d < ND; ++d) v[d] = wave_sum(v[d]);
-    if (l == 0) {
-#pragma unroll
-        for (int d = 0; d < ND; ++d) red[d * kWaves + w] = v[d];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        double t = red[d * kWaves];
-#pragma unroll
-        for (int k = 1; k < kWaves; ++k) t += red[d * kWaves + k];
-        v[d] = t;
-    }
-    __syncthreads();
-}
-
-template <typename T>
-__device__ __forceinline__ T block_sum1(T v, T *red)
-{
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    v = wave_sum(v);
-    if (l == 0) red[w] = v;
-    __syncthreads();
-    T t = red[0];
-#pragma unroll
-    for (int k = 1; k < kWaves; ++k) t += red[k];
-    __syncthreads();
-    return t;
-}
-
-// Sum of partials [p0, p1) (stride nparts between the ND rows), fixed order.
-template <int ND>
-__device__ __forceinline__ void reduce_parts(const double *part, int nparts, int p0, int p1, double (&v)[ND],
-                                             double *red)
-{
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        double acc = 0.0;
-        for (int i = p0 + (int)threadIdx.x; i < p1; i += kThreads) acc += part[(size_t)d * nparts + i];
-        v[d] = acc;
-    }
-    block_sum<ND>(v, red);
-}
-
-__device__ __forceinline__ void set_done(SolverState &s, int stop)
-{
-    s.done = 1;
-    s.stop = stop;
-}
-
-// Scalar updates (identical in every workgroup).  Order of operations of
-// bicgstab_cpu.c:203-229; stop tests of bicgstab_gpu.c:148-150,231-236,289-293.
-__device__ void apply_fin(SolverState &s, int fin, double d0, double d1)
-{
-    switch (fin) {
-    case FIN_ALPHA:
-        s.rv = d0;
-        s.alpha = s.rho / d0;
-        if (d0 == 0.0 || !isfinite(s.alpha)) set_done(s, 4);
-        break;
-    case FIN_OMEGA:
-        s.ts = d0;
-        s.tt = d1;
-        s.omega = d0 / d1;
-        if (d1 == 0.0 || !isfinite(s.omega)) set_done(s, 4);
-        break;
-    case FIN_XR:
-        // end of iteration `iter` (or, with iter == -1, the initial residual)
-        s.pending = 0;
-        s.iter += 1;
-        if (s.iter == 0) {
-            s.rr0 = d1;
-            s.thr = s.tol * sqrt(d1);
-        }
-        if (s.s_conv) { set_done(s, 1); break; }
-        s.rho_next = d0;
-        s.rr = d1;
-        if (s.iter > 0 && !s.fixed && sqrt(d1) < s.thr) { set_done(s, 2); break; }
-        if (d1 == 0.0) { set_done(s, 2); break; }
-        if (s.iter >= s.maxit) { set_done(s, 0); break; }
-        if (d0 == 0.0 || !isfinite(d0)) { set_done(s, 4); break; }
-        s.beta = (d0 / s.rho) * (s.alpha / s.omega);
-        s.rho = d0;
-        break;
-    case FIN_S:
-        s.ss = d0;
-        if (!s.fixed && sqrt(d0) < s.thr) s.s_conv = 1;
-        break;
-    case FIN_RESID:
-        s.resid2 = d0;
-        break;
-    case FIN_STORE:
-        s.out = d0;
-        break;
-    default:
-        break;
-    }
-}
 
 template <int EPI>
 struct EpiDots {
@@ -333,36 +181,37 @@ struct alignas(16) Pk {
     T v[N];
 };
 
-// Load / store the 16-byte pack at element idx, masked against e1.
+// Load / store the 16-byte pack at element idx, masked against [e0, e1)
+// (elements outside load as 0 and are not stored).
 template <typename T>
-__device__ __forceinline__ Pk<T> ldm(const T *p, int idx, int e1)
+__device__ __forceinline__ Pk<T> ldm(const T *p, int idx, const VecBlock &vb)
 {
     Pk<T> r;
-    if (idx + Pk<T>::N <= e1) {
+    if (idx >= vb.e0 && idx + Pk<T>::N <= vb.e1) {
         r = *reinterpret_cast<const Pk<T> *>(p + idx);
     } else {
 #pragma unroll
-        for (int k = 0; k < Pk<T>::N; ++k) r.v[k] = idx + k < e1 ? p[idx + k] : T(0);
+        for (int k = 0; k < Pk<T>::N; ++k) r.v[k] = (idx + k >= vb.e0 && idx + k < vb.e1) ? p[idx + k] : T(0);
     }
     return r;
 }
 template <typename T>
-__device__ __forceinline__ void stm(T *p, int idx, int e1, const Pk<T> &x)
+__device__ __forceinline__ void stm(T *p, int idx, const VecBlock &vb, const Pk<T> &x)
 {
-    if (idx + Pk<T>::N <= e1) {
+    if (idx >= vb.e0 && idx + Pk<T>::N <= vb.e1) {
         *reinterpret_cast<Pk<T> *>(p + idx) = x;
     } else {
 #pragma unroll
         for (int k = 0; k < Pk<T>::N; ++k)
-            if (idx + k < e1) p[idx + k] = x.v[k];
+            if (idx + k >= vb.e0 && idx + k < vb.e1) p[idx + k] = x.v[k];
     }
 }
 
-// Element index of pack u of this thread (e0 is a multiple of the pack width).
+// Element index of pack u of this thread (base is 16-byte aligned).
 template <typename T>
-__device__ __forceinline__ int pk_idx(int e0, int u)
+__device__ __forceinline__ int pk_idx(const VecBlock &vb, int u)
 {
-    return e0 + (u * kThreads + (int)threadIdx.x) * Pk<T>::N;
+    return vb.base + (u * kThreads + (int)threadIdx.x) * Pk<T>::N;
 }
 
 template <typename T>
@@ -411,10 +260,10 @@ __global__ __launch_bounds__(kThreads) void k_vec_P(VecArgs<T> a)
     Pk<T> p[U], v[U], r[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int i = pk_idx<T>(vb.e0, u);
-        p[u] = ldm(a.P, i, vb.e1);
-        v[u] = ldm(a.V, i, vb.e1);
-        r[u] = ldm(a.R, i, vb.e1);
+        const int i = pk_idx<T>(vb, u);
+        p[u] = ldm(a.P, i, vb);
+        v[u] = ldm(a.V, i, vb);
+        r[u] = ldm(a.R, i, vb);
     }
     __syncthreads();
     if (!sh.done && sh.pending) apply_in<2, T>(a, vb, sh, FIN_XR, red);
@@ -425,7 +274,7 @@ __global__ __launch_bounds__(kThreads) void k_vec_P(VecArgs<T> a)
     for (int u = 0; u < U; ++u) {
 #pragma unroll
         for (int k = 0; k < Pk<T>::N; ++k) p[u].v[k] = ((p[u].v[k] - v[u].v[k] * om) * be) + r[u].v[k];
-        stm(a.P, pk_idx<T>(vb.e0, u), vb.e1, p[u]);
+        stm(a.P, pk_idx<T>(vb, u), vb, p[u]);
     }
 }
 
@@ -441,9 +290,9 @@ __global__ __launch_bounds__(kThreads) void k_vec_S(VecArgs<T> a)
     Pk<T> v[U], r[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int i = pk_idx<T>(vb.e0, u);
-        v[u] = ldm(a.V, i, vb.e1);
-        r[u] = ldm(a.R, i, vb.e1);
+        const int i = pk_idx<T>(vb, u);
+        v[u] = ldm(a.V, i, vb);
+        r[u] = ldm(a.R, i, vb);
     }
     __syncthreads();
     if (!sh.done) apply_in<1, T>(a, vb, sh, FIN_ALPHA, red);
@@ -459,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void k_vec_S(VecArgs<T> a)
             s.v[k] = r[u].v[k] - v[u].v[k] * al;
             d0.add(s.v[k], s.v[k]);
         }
-        stm(a.S, pk_idx<T>(vb.e0, u), vb.e1, s);
+        stm(a.S, pk_idx<T>(vb, u), vb, s);
     }
     write_parts<1, T>(a, d0, d1, red);
 }
@@ -478,12 +327,12 @@ __global__ __launch_bounds__(kThreads) void k_vec_XR(VecArgs<T> a)
     Pk<T> x[U], p[U], s[U], t[U], r0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int i = pk_idx<T>(vb.e0, u);
-        x[u] = ldm(a.X, i, vb.e1);
-        p[u] = ldm(a.P, i, vb.e1);
-        s[u] = ldm(a.S, i, vb.e1);
-        t[u] = ldm(a.T_, i, vb.e1);
-        r0[u] = ldm(a.R0, i, vb.e1);
+        const int i = pk_idx<T>(vb, u);
+        x[u] = ldm(a.X, i, vb);
+        p[u] = ldm(a.P, i, vb);
+        s[u] = ldm(a.S, i, vb);
+        t[u] = ldm(a.T_, i, vb);
+        r0[u] = ldm(a.R0, i, vb);
     }
     __syncthreads();
     if (!sh.done && !sh.s_conv) apply_in<2, T>(a, vb, sh, FIN_OMEGA, red);
@@ -497,7 +346,7 @@ __global__ __launch_bounds__(kThreads) void k_vec_XR(VecArgs<T> a)
         for (int u = 0; u < U; ++u) {
 #pragma unroll
             for (int k = 0; k < Pk<T>::N; ++k) x[u].v[k] = x[u].v[k] + p[u].v[k] * al;
-            stm(a.X, pk_idx<T>(vb.e0, u), vb.e1, x[u]);
+            stm(a.X, pk_idx<T>(vb, u), vb, x[u]);
         }
         return;
     }
@@ -512,9 +361,9 @@ __global__ __launch_bounds__(kThreads) void k_vec_XR(VecArgs<T> a)
             d0.add(r.v[k], r0[u].v[k]);
             d1.add(r.v[k], r.v[k]);
         }
-        const int i = pk_idx<T>(vb.e0, u);
-        stm(a.X, i, vb.e1, x[u]);
-        stm(a.R, i, vb.e1, r);
+        const int i = pk_idx<T>(vb, u);
+        stm(a.X, i, vb, x[u]);
+        stm(a.R, i, vb, r);
     }
     write_parts<2, T>(a, d0, d1, red);
 }

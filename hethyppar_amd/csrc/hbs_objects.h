@@ -90,14 +90,53 @@ struct hbs_csr_s {
     double *val = nullptr;
     float *val32 = nullptr;
     hbs::SpmvPlan plan;
-    hbs::SysRange *sys = nullptr;     // {0, ngroups, 0, 0}
+    hbs::SysRange *sys = nullptr;     // [nsys] {g0, g1, 0, 0}
     double *chunk_sums = nullptr;     // [nchunks]
     double *part = nullptr;           // [ngroups * kMaxDots]
     hbs_csr_s *tr = nullptr;
+    int *tr_perm = nullptr;           // [nnz] position of entry j of A in A^T
     std::map<std::pair<int, int>, int> spmm_tile;   // (k, dtype) -> tile
     int stream_nt = 0;                // non-temporal col/val loads
+    int nsys = 1;                     // independent diagonal blocks (systems)
+    std::vector<int> sys_rows;        // [nsys + 1] row offsets of the systems
+    std::vector<hbs::SysRange> sys_host;   // host copy of sys
     hipEvent_t *evs = nullptr;
     int nevs = 0;
+};
+
+namespace hbs {
+constexpr int kSlotA = 0, kSlotB = 1, kSlotOuter = 2, kNumSlots = 4;
+struct BlockWork;                    // k right-hand-side workspace (hbs_block.cpp)
+void block_work_free(BlockWork *w);
+}  // namespace hbs
+
+struct hbs_solver_s {
+    hbs_csr A = nullptr;
+    int n = 0;
+    int nsys = 1;                 // independent systems of A (diagonal blocks)
+    double *X = nullptr, *R = nullptr, *R0 = nullptr, *P = nullptr, *V = nullptr, *S = nullptr, *T = nullptr;
+    float *X32 = nullptr, *R32 = nullptr, *R032 = nullptr, *P32 = nullptr, *V32 = nullptr, *S32 = nullptr,
+          *T32 = nullptr;
+    hbs::SolverState *st = nullptr;    // device [kNumSlots][nsys]
+    hbs::SolverState *hst = nullptr;   // pinned host [3][nsys]: two poll buffers, one read buffer
+    hbs::VecBlock *vblocks = nullptr;
+    hbs::SysRange *sys = nullptr;      // [nsys] {g0, g1, v0, v1}
+    int nvb = 0;
+    // partials, one buffer per producer so a consumer never sees them overwritten
+    double *partA = nullptr;      // SpMV1: (r0, v)
+    double *partB = nullptr;      // k_vec_S: ||s||^2
+    double *partC = nullptr;      // SpMV2: (t, s), (t, t)
+    double *partD = nullptr;      // k_vec_XR / start: (r, r0), ||r||^2
+    double *partE = nullptr;      // residual SpMV: ||b - Ax||^2
+    double *partF = nullptr;      // dots: (b, b)
+    double *chunk_sums = nullptr; // long-row chunk sums
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // [0] FP64, [1] FP32
+    int graph_chunk[2] = {0, 0};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hbs_solver_s *adj = nullptr;
+    hbs::BlockWork *blk = nullptr;
+    hbs::SolverState *slot(int k) const { return st + (size_t)k * nsys; }
 };
 
 namespace hbs {

@@ -75,10 +75,19 @@ void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan)
     }
 }
 
+// A block's packs start at base = e0 rounded down to 4 elements (16-byte
+// aligned for FP32 and FP64) and cover [base, base + elems_per_block); the
+// kernels mask what lies outside [e0, e1).  elems_per_block is a multiple of
+// 4, so only a system's first block can start below its e0.
 void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock> &blocks)
 {
-    for (int e = e0; e < e0 + n; e += elems_per_block)
-        blocks.push_back(VecBlock{sys, e, std::min(e + elems_per_block, e0 + n), 0});
+    const int end = e0 + n;
+    for (int e = e0; e < end;) {
+        const int base = e & ~3;
+        const int e1 = std::min(base + elems_per_block, end);
+        blocks.push_back(VecBlock{sys, e, e1, base});
+        e = e1;
+    }
 }
 
 // Elements per vector workgroup: 512 * u with u in {1, 2, 4, 8}, the

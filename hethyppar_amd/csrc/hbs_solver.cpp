@@ -21,36 +21,6 @@ using namespace hbs;
 
 #define fail ::hbs::set_error
 
-namespace {
-constexpr int kSlotA = 0, kSlotB = 1, kSlotOuter = 2, kNumSlots = 4;
-}
-
-struct hbs_solver_s {
-    hbs_csr A = nullptr;
-    int n = 0;
-    double *X = nullptr, *R = nullptr, *R0 = nullptr, *P = nullptr, *V = nullptr, *S = nullptr, *T = nullptr;
-    float *X32 = nullptr, *R32 = nullptr, *R032 = nullptr, *P32 = nullptr, *V32 = nullptr, *S32 = nullptr,
-          *T32 = nullptr;
-    SolverState *st = nullptr;    // device [kNumSlots]
-    SolverState *hst = nullptr;   // pinned host [3]
-    VecBlock *vblocks = nullptr;
-    SysRange *sys = nullptr;      // {0, ngroups, 0, nvb}
-    int nvb = 0;
-    // partials, one buffer per producer so a consumer never sees them overwritten
-    double *partA = nullptr;      // SpMV1: (r0, v)
-    double *partB = nullptr;      // k_vec_S: ||s||^2
-    double *partC = nullptr;      // SpMV2: (t, s), (t, t)
-    double *partD = nullptr;      // k_vec_XR / start: (r, r0), ||r||^2
-    double *partE = nullptr;      // residual SpMV: ||b - Ax||^2
-    double *partF = nullptr;      // dots: (b, b)
-    double *chunk_sums = nullptr; // long-row chunk sums
-    hipGraphExec_t graph[2] = {nullptr, nullptr};   // [0] FP64, [1] FP32
-    int graph_chunk[2] = {0, 0};
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hbs_solver_s *adj = nullptr;
-};
-
 extern "C" void hbs_solve_opts_default(hbs_solve_opts *o)
 {
     if (!o) return;
@@ -82,6 +52,7 @@ extern "C" int hbs_solver_destroy(hbs_solver s)
     for (hipEvent_t e : {s->ev[0], s->ev[1], s->e0, s->e1})
         if (e) (void)hipEventDestroy(e);
     if (s->adj) hbs_solver_destroy(s->adj);
+    if (s->blk) block_work_free(s->blk);
     delete s;
     return HBS_OK;
 }
@@ -103,18 +74,25 @@ extern "C" int hbs_solver_create(hbs_csr a, hbs_solver *out)
         return rc;
     };
     int rc;
+    s->nsys = a->nsys;
     for (double **p : {&s->X, &s->R, &s->R0, &s->P, &s->V, &s->S, &s->T})
         if ((rc = dalloc(p, s->n))) return bail(rc);
-    if ((rc = dalloc(&s->st, kNumSlots))) return bail(rc);
-    if (hipHostMalloc((void **)&s->hst, 3 * sizeof(SolverState)) != hipSuccess)
+    if ((rc = dalloc(&s->st, (size_t)kNumSlots * s->nsys))) return bail(rc);
+    if (hipHostMalloc((void **)&s->hst, 3 * sizeof(SolverState) * (size_t)s->nsys) != hipSuccess)
         return bail(fail(HBS_ERR_ALLOC, "pinned host allocation failed"));
+    // vector workgroups system by system; one block size for the launch
     std::vector<VecBlock> vb;
-    plan_vec(0, s->n, vec_elems_per_block(s->n), 0, vb);
+    std::vector<SysRange> ranges = a->sys_host;
+    const int epb = vec_elems_per_block(s->n);
+    for (int q = 0; q < s->nsys; ++q) {
+        ranges[(size_t)q].v0 = (int)vb.size();
+        plan_vec(a->sys_rows[(size_t)q], a->sys_rows[(size_t)q + 1] - a->sys_rows[(size_t)q], epb, q, vb);
+        ranges[(size_t)q].v1 = (int)vb.size();
+    }
     s->nvb = (int)vb.size();
     const int ng = a->plan.ngroups;
-    if ((rc = dalloc(&s->vblocks, vb.size())) || (rc = dalloc(&s->sys, 1))) return bail(rc);
-    if ((rc = upload(s->vblocks, vb)) || (rc = upload(s->sys, std::vector<SysRange>{{0, ng, 0, s->nvb}})))
-        return bail(rc);
+    if ((rc = dalloc(&s->vblocks, vb.size())) || (rc = dalloc(&s->sys, (size_t)s->nsys))) return bail(rc);
+    if ((rc = upload(s->vblocks, vb)) || (rc = upload(s->sys, ranges))) return bail(rc);
     const size_t ps = (size_t)std::max(ng, 1) * kMaxDots, pv = (size_t)std::max(s->nvb, 1) * kMaxDots;
     if ((rc = dalloc(&s->partA, ps)) || (rc = dalloc(&s->partB, pv)) || (rc = dalloc(&s->partC, ps)) ||
         (rc = dalloc(&s->partD, pv)) || (rc = dalloc(&s->partE, ps)) || (rc = dalloc(&s->partF, pv)) ||
@@ -169,7 +147,7 @@ template <typename T>
 hipError_t launch_iteration(hbs_solver s, const Work<T> &w)
 {
     hipStream_t q = s->A->ctx->stream;
-    SolverState *A = s->st + kSlotA, *B = s->st + kSlotB;
+    SolverState *A = s->slot(kSlotA), *B = s->slot(kSlotB);
     const int ng = s->A->plan.ngroups;
     hipError_t e;
     VecArgs<T> va = vec_args<T>(s, w);
@@ -225,7 +203,7 @@ hipError_t launch_chunk(hbs_solver s, const Work<T> &w, int k)
     for (int i = 0; i < k && e == hipSuccess; ++i) e = launch_iteration<T>(s, w);
     if (e != hipSuccess) return e;
     // the state the host polls: FIN_XR of the chunk's last iteration applied
-    return launch_apply_xr(s->st + kSlotA, s->sys, 1, s->partD, s->nvb, s->A->ctx->stream);
+    return launch_apply_xr(s->slot(kSlotA), s->sys, s->nsys, s->partD, s->nvb, s->A->ctx->stream);
 }
 
 template <typename T>
@@ -260,24 +238,32 @@ int run_chunk(hbs_solver s, const Work<T> &w, int k, bool use_graph)
     return HBS_OK;
 }
 
-// Queue iteration chunks until the device state says done.  The state of
-// chunk c is read after chunk c + 1 is queued, so the GPU never idles on the
-// poll; iterations queued after convergence return at once.
+bool all_done(const SolverState *h, int nsys)
+{
+    for (int i = 0; i < nsys; ++i)
+        if (!h[i].done) return false;
+    return true;
+}
+
+// Queue iteration chunks until the device states say every system is done.
+// The states of chunk c are read after chunk c + 1 is queued, so the GPU
+// never idles on the poll; iterations queued after convergence return at once.
 template <typename T>
 int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use_graph)
 {
     hipStream_t q = s->A->ctx->stream;
     const int k = check_every > 0 ? check_every : 4;
+    const size_t bytes = sizeof(SolverState) * (size_t)s->nsys;
     long issued = 0;
     int c = 0, pending = -1;
     for (;;) {
         CHK(run_chunk<T>(s, w, k, use_graph));
         issued += k;
-        HIPCHK(hipMemcpyAsync(&s->hst[c & 1], s->st + kSlotA, sizeof(SolverState), hipMemcpyDeviceToHost, q));
+        HIPCHK(hipMemcpyAsync(s->hst + (size_t)(c & 1) * s->nsys, s->slot(kSlotA), bytes, hipMemcpyDeviceToHost, q));
         HIPCHK(hipEventRecord(s->ev[c & 1], q));
         if (pending >= 0) {
             HIPCHK(hipEventSynchronize(s->ev[pending & 1]));
-            if (s->hst[pending & 1].done) break;
+            if (all_done(s->hst + (size_t)(pending & 1) * s->nsys, s->nsys)) break;
         }
         pending = c++;
         if (issued >= maxit) {
@@ -288,13 +274,21 @@ int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use
     return HBS_OK;
 }
 
-int read_state(hbs_solver s, int slot, SolverState *out)
+// Copies slot k's states of every system to out[nsys].
+int read_states(hbs_solver s, int k, SolverState *out)
 {
     hipStream_t q = s->A->ctx->stream;
-    HIPCHK(hipMemcpyAsync(&s->hst[2], s->st + slot, sizeof(SolverState), hipMemcpyDeviceToHost, q));
+    SolverState *h = s->hst + 2 * (size_t)s->nsys;
+    HIPCHK(hipMemcpyAsync(h, s->slot(k), sizeof(SolverState) * (size_t)s->nsys, hipMemcpyDeviceToHost, q));
     HIPCHK(hipStreamSynchronize(q));
-    *out = s->hst[2];
+    std::memcpy(out, h, sizeof(SolverState) * (size_t)s->nsys);
     return HBS_OK;
+}
+
+int read_state(hbs_solver s, int k, SolverState *out)
+{
+    if (s->nsys != 1) return fail(HBS_ERR_ARG, "single-system call on a matrix of %d systems", s->nsys);
+    return read_states(s, k, out);
 }
 
 int status_of(const SolverState &st, bool fixed)
@@ -304,8 +298,8 @@ int status_of(const SolverState &st, bool fixed)
     return HBS_OK;
 }
 
-// r = b - A x (FP64) with ||r||^2 into st[slot].resid2.
-hipError_t residual(hbs_solver s, const double *x, const double *b, double *r, int slot)
+// r = b - A x (FP64) with ||r||^2 into resid2 of slot k (every system).
+hipError_t residual(hbs_solver s, const double *x, const double *b, double *r, int k)
 {
     hipStream_t q = s->A->ctx->stream;
     SpmvArgs<double> sa = solver_spmv<double>(s, s->A->val);
@@ -315,40 +309,42 @@ hipError_t residual(hbs_solver s, const double *x, const double *b, double *r, i
     sa.part_out = s->partE;
     hipError_t e = run_spmv(sa, s->A, EPI_RESID, nullptr, q);
     if (e != hipSuccess) return e;
-    return launch_finalize(s->st + slot, s->sys, 1, s->partE, s->A->plan.ngroups, 1, FIN_RESID, q);
+    return launch_finalize(s->slot(k), s->sys, s->nsys, s->partE, s->A->plan.ngroups, 1, FIN_RESID, q);
 }
 
-// (b, b) into st[slot].out.
-hipError_t bnorm2(hbs_solver s, const double *b, int slot)
+// (b, b) into out of slot k (every system).
+hipError_t bnorm2(hbs_solver s, const double *b, int k)
 {
     hipStream_t q = s->A->ctx->stream;
     VecArgs<double> va = vec_args<double>(s, work64(s));
     va.part_out = s->partF;
     hipError_t e = launch_dot(va, b, b, q);
     if (e != hipSuccess) return e;
-    return launch_finalize(s->st + slot, s->sys, 1, s->partF, s->nvb, 0, FIN_STORE, q);
+    return launch_finalize(s->slot(k), s->sys, s->nsys, s->partF, s->nvb, 0, FIN_STORE, q);
 }
 
-}  // namespace
-
-extern "C" int hbs_bicgstab_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts *opts,
-                                hbs_solve_info *info)
+void fill_info(const SolverState &hs, bool fixed, float ms, hbs_solve_info *info)
 {
-    if (!s) return fail(HBS_ERR_ARG, "null solver");
-    if (!b || !x) return fail(HBS_ERR_ARG, "b and x must be device vectors");
-    hbs_solve_opts o;
-    hbs_solve_opts_default(&o);
-    if (opts) o = *opts;
-    if (o.maxit < 0 || !(o.tol >= 0.0)) return fail(HBS_ERR_ARG, "bad tolerance or iteration cap");
+    info->iters = std::max(hs.iter, 0);
+    info->stop = hs.stop;
+    info->sweeps = 0;
+    info->status = status_of(hs, fixed);
+    info->relres = hs.out > 0 ? std::sqrt(hs.resid2 / hs.out) : std::sqrt(hs.resid2);
+    info->rel_r0 = hs.rr0 > 0 ? std::sqrt(hs.resid2 / hs.rr0) : 0.0;
+    info->time_ms = ms;
+}
+
+// FP64 BiCGStab of every system of A at once; infos[nsys] (always filled).
+int solve_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts &o, hbs_solve_info *infos)
+{
     hbs_csr A = s->A;
-    DevGuard g(A->ctx->device);
     hipStream_t q = A->ctx->stream;
     const int n = s->n;
     Work<double> w = work64(s);
 
     HIPCHK(hipEventRecord(s->e0, q));
     HIPCHK(hipMemcpyAsync(s->X, x, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
-    HIPCHK(launch_state_init(s->st, 2, o.tol, o.maxit, o.fixed_iters ? 1 : 0, q));
+    HIPCHK(launch_state_init(s->st, 2 * s->nsys, o.tol, o.maxit, o.fixed_iters ? 1 : 0, q));
     // R = B - A X ; R0 = R ; P = V = 0 ; (R, R0) = ||R||^2 pending in slot A
     // (bicgstab_cpu.c:176-188)
     {
@@ -359,7 +355,7 @@ extern "C" int hbs_bicgstab_f64(hbs_solver s, const double *b, double *x, const 
         sa.part_out = s->partE;
         HIPCHK(run_spmv(sa, A, EPI_RESID, nullptr, q));
         VecArgs<double> va = vec_args<double>(s, w);
-        va.st_out = s->st + kSlotA;
+        va.st_out = s->slot(kSlotA);
         va.part_out = s->partD;
         HIPCHK(launch_init_vectors(va, q));
     }
@@ -369,26 +365,79 @@ extern "C" int hbs_bicgstab_f64(hbs_solver s, const double *b, double *x, const 
     HIPCHK(bnorm2(s, b, kSlotA));
     HIPCHK(hipMemcpyAsync(x, s->X, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
     HIPCHK(hipEventRecord(s->e1, q));
-    SolverState hs;
-    CHK(read_state(s, kSlotA, &hs));
+    std::vector<SolverState> hs((size_t)s->nsys);
+    CHK(read_states(s, kSlotA, hs.data()));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
-    const int iters = std::max(hs.iter, 0);
-    const int status = status_of(hs, o.fixed_iters != 0);
-    if (info) {
-        info->iters = iters;
-        info->stop = hs.stop;
-        info->sweeps = 0;
-        info->status = status;
-        info->relres = hs.out > 0 ? std::sqrt(hs.resid2 / hs.out) : std::sqrt(hs.resid2);
-        info->rel_r0 = hs.rr0 > 0 ? std::sqrt(hs.resid2 / hs.rr0) : 0.0;
-        info->time_ms = ms;
-    }
-    if (status == HBS_ERR_BREAKDOWN) return fail(status, "BiCGStab breakdown after %d iterations", iters);
-    if (status == HBS_ERR_NOCONV)
-        return fail(status, "BiCGStab reached %d iterations, ||r||/||r0|| = %.3e", iters,
-                    hs.rr0 > 0 ? std::sqrt(hs.resid2 / hs.rr0) : 0.0);
+    for (int i = 0; i < s->nsys; ++i) fill_info(hs[(size_t)i], o.fixed_iters != 0, ms, &infos[i]);
     return HBS_OK;
+}
+
+int check_opts(const hbs_solve_opts *opts, hbs_solve_opts *o)
+{
+    hbs_solve_opts_default(o);
+    if (opts) *o = *opts;
+    if (o->maxit < 0 || !(o->tol >= 0.0)) return fail(HBS_ERR_ARG, "bad tolerance or iteration cap");
+    return HBS_OK;
+}
+
+int report(const hbs_solve_info &in, const char *what, int sys)
+{
+    if (in.status == HBS_ERR_BREAKDOWN)
+        return fail(in.status, "%s: breakdown after %d iterations (system %d)", what, in.iters, sys);
+    if (in.status == HBS_ERR_NOCONV)
+        return fail(in.status, "%s reached %d iterations, ||r||/||r0|| = %.3e (system %d)", what, in.iters,
+                    in.rel_r0, sys);
+    return HBS_OK;
+}
+
+// Worst status first (breakdown, then not converged), else the first system.
+int worst_system(const std::vector<hbs_solve_info> &infos)
+{
+    int w = 0;
+    for (int i = 1; i < (int)infos.size(); ++i) {
+        const int a = infos[(size_t)i].status, b = infos[(size_t)w].status;
+        if (a != b && (a == HBS_ERR_BREAKDOWN || (a == HBS_ERR_NOCONV && b == HBS_OK))) w = i;
+    }
+    return w;
+}
+
+}  // namespace
+
+extern "C" int hbs_bicgstab_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts *opts,
+                                hbs_solve_info *info)
+{
+    if (!s) return fail(HBS_ERR_ARG, "null solver");
+    if (!b || !x) return fail(HBS_ERR_ARG, "b and x must be device vectors");
+    hbs_solve_opts o;
+    CHK(check_opts(opts, &o));
+    DevGuard g(s->A->ctx->device);
+    std::vector<hbs_solve_info> infos((size_t)s->nsys);
+    CHK(solve_f64(s, b, x, o, infos.data()));
+    // several systems: the most iterations, the largest residuals, the worst status
+    hbs_solve_info agg = infos[(size_t)worst_system(infos)];
+    for (const hbs_solve_info &in : infos) {
+        agg.iters = std::max(agg.iters, in.iters);
+        agg.relres = std::max(agg.relres, in.relres);
+        agg.rel_r0 = std::max(agg.rel_r0, in.rel_r0);
+    }
+    if (info) *info = agg;
+    return report(infos[(size_t)worst_system(infos)], "BiCGStab", worst_system(infos));
+}
+
+extern "C" int hbs_bicgstab_systems_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts *opts,
+                                        hbs_solve_info *infos)
+{
+    if (!s || !infos) return fail(HBS_ERR_ARG, "null argument");
+    if (!b || !x) return fail(HBS_ERR_ARG, "b and x must be device vectors");
+    hbs_solve_opts o;
+    CHK(check_opts(opts, &o));
+    DevGuard g(s->A->ctx->device);
+    std::vector<hbs_solve_info> in((size_t)s->nsys);
+    CHK(solve_f64(s, b, x, o, in.data()));
+    std::memcpy(infos, in.data(), sizeof(hbs_solve_info) * in.size());
+    const int w = worst_system(in);
+    return report(in[(size_t)w], "BiCGStab", w);
 }
 
 extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, const hbs_solve_opts *opts,
@@ -398,6 +447,7 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
     if (!b || !x) return fail(HBS_ERR_ARG, "b and x must be device vectors");
     hbs_csr A = s->A;
     if (!A->val32) return fail(HBS_ERR_ARG, "mixed precision needs a matrix created with HBS_CSR_F32");
+    if (s->nsys != 1) return fail(HBS_ERR_ARG, "mixed precision solves one system (matrix has %d)", s->nsys);
     hbs_solve_opts o;
     hbs_solve_opts_default(&o);
     if (opts) o = *opts;
@@ -411,7 +461,7 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
 
     HIPCHK(hipEventRecord(s->e0, q));
     HIPCHK(hipMemcpyAsync(s->X, x, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
-    HIPCHK(launch_state_init(s->st + kSlotOuter, 1, o.tol, 0, 0, q));
+    HIPCHK(launch_state_init(s->slot(kSlotOuter), 1, o.tol, 0, 0, q));
     // FP64 outer residual r = b - A x in R, ||b||^2
     HIPCHK(residual(s, s->X, b, s->R, kSlotOuter));
     HIPCHK(bnorm2(s, b, kSlotOuter));
@@ -431,11 +481,11 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
         const int imax = std::max(1, std::min(o.inner_maxit, o.maxit - total_iters));
         HIPCHK(launch_state_init(s->st, 2, itol, imax, 0, q));
         VecArgs<float> va = vec_args<float>(s, w32);
-        va.st_out = s->st + kSlotA;
+        va.st_out = s->slot(kSlotA);
         va.part_out = s->partD;
-        HIPCHK(launch_ir_start(va, s->R, s->st + kSlotOuter, q));
+        HIPCHK(launch_ir_start(va, s->R, s->slot(kSlotOuter), q));
         CHK(iterate<float>(s, w32, imax, o.check_every, o.use_graph != 0));
-        va.st_in = s->st + kSlotA;
+        va.st_in = s->slot(kSlotA);
         HIPCHK(launch_ir_update(va, s->X, q));
         HIPCHK(residual(s, s->X, b, s->R, kSlotOuter));
         SolverState hi;

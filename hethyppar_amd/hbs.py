@@ -18,6 +18,8 @@ ERRORS = {
     -5: "HBS_ERR_FORMAT", -6: "HBS_ERR_PARTITION", -7: "HBS_ERR_BREAKDOWN",
     -8: "HBS_ERR_NOCONV", -9: "HBS_ERR_NODEVICE",
 }
+HBS_ERR_BREAKDOWN = -7
+HBS_ERR_NOCONV = -8
 HBS_CSR_F32 = 1
 HBS_CSR_TRANSPOSE = 2
 
@@ -84,6 +86,9 @@ _PROTOS = {
     "hbs_d2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
     "hbs_dev_zero": (_i, [_vp, _vp, C.c_size_t]),
     "hbs_csr_create": (_i, [_vp, _i, _i, _i, _ip, _ip, _dp, _i, C.POINTER(_vp)]),
+    "hbs_csr_create_systems": (_i, [_vp, _i, _ip, _i, _ip, _ip, _dp, _i, C.POINTER(_vp)]),
+    "hbs_csr_systems": (_i, [_vp, _ip]),
+    "hbs_csr_set_values": (_i, [_vp, _vp]),
     "hbs_csr_destroy": (_i, [_vp]),
     "hbs_csr_info": (_i, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "hbs_csr_device_arrays": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
@@ -100,6 +105,8 @@ _PROTOS = {
     "hbs_solver_destroy": (_i, [_vp]),
     "hbs_bicgstab_f64": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "hbs_bicgstab_mixed": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "hbs_bicgstab_systems_f64": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "hbs_bicgstab_block_f64": (_i, [_vp, _i, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "hbs_adjoint_f64": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "hbs_host_free": (None, [_vp]),
     "hbs_mm_read": (_i, [C.c_char_p, _ip, _ip, _ip, _ipp, _ipp, _dpp]),
@@ -157,6 +164,23 @@ def check(rc):
     if rc != HBS_OK:
         raise HBSError(rc, lib().hbs_last_error().decode())
     return rc
+
+
+def _ptr(a):
+    """Device address of a DeviceArray, a torch tensor or a plain integer."""
+    if isinstance(a, (DeviceArray, DevPtr)):
+        return a.ptr
+    if hasattr(a, "data_ptr"):
+        return _vp(a.data_ptr())
+    return _vp(int(a))
+
+
+class DevPtr:
+    """A borrowed device address with the .ptr attribute the wrappers use
+    (e.g. the storage of a torch tensor)."""
+
+    def __init__(self, a):
+        self.ptr = _ptr(a)
 
 
 def _ip_of(a):
@@ -264,6 +288,21 @@ def gen_fem(nblockrows, nblk_per_row=12, dom=1.1, seed=6):
 def gen_uniform(n, lo=-1.0, hi=1.0, seed=1):
     out = np.zeros(n, dtype=np.float64)
     check(lib().hbs_gen_uniform(n, lo, hi, seed, _dp_of(out)))
+    return out
+
+
+def gen_uneven_systems(nsys, lo_exp=3.0, hi_exp=5.0, seed=5):
+    """Independent 2-D convection-diffusion systems of uneven size: system q
+    has n_q = 10^U(lo_exp, hi_exp) rounded to a square grid (side >= 2) and
+    convection c_q ~ U[0, 1] (diagonal 4 + c_q, west -1 - c_q, others -1).
+    The uniforms come from the SplitMix64 stream of gen_uniform(seed).
+    Returns [(rowptr, col, val, conv), ...]."""
+    u = gen_uniform(2 * nsys, 0.0, 1.0, seed)
+    out = []
+    for q in range(nsys):
+        side = max(2, int(round((10.0 ** (lo_exp + (hi_exp - lo_exp) * u[2 * q])) ** 0.5)))
+        rp, c, v = gen_convdiff2d(side, side, float(u[2 * q + 1]))
+        out.append((rp, c, v, float(u[2 * q + 1])))
     return out
 
 
@@ -428,8 +467,25 @@ class DeviceArray:
             pass
 
 
+def concat_systems(systems):
+    """Block-diagonal CSR of independent square systems [(rowptr, col, val),
+    ...] with each system's columns shifted to its rows; returns (rowptr,
+    col, val, sys_rows) for DeviceCSR(..., sys_rows=sys_rows)."""
+    rps, cols, vals, sys_rows = [np.zeros(1, np.int64)], [], [], [0]
+    nz = 0
+    for rp, c, v in systems:
+        rp = np.asarray(rp, dtype=np.int64)
+        rps.append(rp[1:] + nz)
+        cols.append(np.asarray(c, dtype=np.int64) + sys_rows[-1])
+        vals.append(np.asarray(v, dtype=np.float64))
+        nz += int(rp[-1])
+        sys_rows.append(sys_rows[-1] + rp.size - 1)
+    return (np.concatenate(rps).astype(np.int32), np.concatenate(cols).astype(np.int32),
+            np.concatenate(vals), np.asarray(sys_rows, dtype=np.int32))
+
+
 class DeviceCSR:
-    def __init__(self, ctx, rowptr, col, val, ncols=None, flags=0):
+    def __init__(self, ctx, rowptr, col, val, ncols=None, flags=0, sys_rows=None):
         self.ctx = ctx
         rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
         col = np.ascontiguousarray(col, dtype=np.int32)
@@ -437,8 +493,16 @@ class DeviceCSR:
         m = rowptr.size - 1
         n = m if ncols is None else ncols
         h = _vp()
-        check(lib().hbs_csr_create(ctx.h, m, n, col.size, _ip_of(rowptr), _ip_of(col), _dp_of(val),
-                                   flags, C.byref(h)))
+        if sys_rows is None:
+            check(lib().hbs_csr_create(ctx.h, m, n, col.size, _ip_of(rowptr), _ip_of(col), _dp_of(val),
+                                       flags, C.byref(h)))
+            self.nsys = 1
+        else:
+            sys_rows = np.ascontiguousarray(sys_rows, dtype=np.int32)
+            check(lib().hbs_csr_create_systems(ctx.h, sys_rows.size - 1, _ip_of(sys_rows), col.size,
+                                               _ip_of(rowptr), _ip_of(col), _dp_of(val), flags,
+                                               C.byref(h)))
+            self.nsys = sys_rows.size - 1
         self.h = h
         self.m, self.n, self.nnz = m, n, int(col.size)
         ctx._adopt(self)
@@ -464,6 +528,10 @@ class DeviceCSR:
             check(lib().hbs_csr_spmv_f32(self.h, alpha, x.ptr, beta, y.ptr))
         else:
             check(lib().hbs_csr_spmv_f64(self.h, alpha, x.ptr, beta, y.ptr))
+
+    def set_values(self, val_dev):
+        """New values (same pattern) from a device array of nnz doubles."""
+        check(lib().hbs_csr_set_values(self.h, _ptr(val_dev)))
 
     def spmv_t(self, x, y):
         check(lib().hbs_csr_spmv_t_f64(self.h, x.ptr, y.ptr))
@@ -522,6 +590,26 @@ class Solver:
 
     def bicgstab_mixed(self, b, x, opts=None, raise_on_error=True, **kw):
         return self._run(lib().hbs_bicgstab_mixed, b, x, opts or default_opts(**kw), raise_on_error)
+
+    def bicgstab_systems(self, b, x, opts=None, raise_on_error=True, **kw):
+        """Every system of a DeviceCSR(..., sys_rows=...) at once; one
+        SolveInfo per system."""
+        infos = (SolveInfo * self.A.nsys)()
+        o = opts or default_opts(**kw)
+        rc = lib().hbs_bicgstab_systems_f64(self.h, b.ptr, x.ptr, C.byref(o), infos)
+        if rc != HBS_OK and raise_on_error:
+            check(rc)
+        return list(infos)
+
+    def bicgstab_block(self, B, X, k, opts=None, raise_on_error=True, **kw):
+        """k right-hand sides, B and X row-major n x k on the device; one
+        independent BiCGStab per column sharing every SpMM pass over A."""
+        infos = (SolveInfo * k)()
+        o = opts or default_opts(**kw)
+        rc = lib().hbs_bicgstab_block_f64(self.h, k, B.ptr, X.ptr, C.byref(o), infos)
+        if rc != HBS_OK and raise_on_error:
+            check(rc)
+        return list(infos)
 
     def adjoint(self, g, x, lam, grad_val, opts=None, raise_on_error=True, **kw):
         info = SolveInfo()

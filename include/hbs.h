@@ -71,8 +71,23 @@ typedef struct hbs_csr_s *hbs_csr;
 int hbs_csr_create(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr,
                    const int *col, const double *val, int flags, hbs_csr *out);
 int hbs_csr_destroy(hbs_csr a);
+/* Many independent systems in one matrix: a block-diagonal CSR of nsys
+ * square systems, system q owning rows and columns [sys_rows[q],
+ * sys_rows[q+1]) (sys_rows[0] = 0, m = n = sys_rows[nsys]; an entry outside
+ * its system's block is an error).  Each system gets its own row blocks and
+ * vector workgroups, so hbs_bicgstab_systems_f64 runs all of them in the same
+ * launches, each with its own scalars and stop test.  No reference
+ * counterpart (the reference solves one system per process). */
+int hbs_csr_create_systems(hbs_ctx ctx, int nsys, const int *sys_rows, int nnz,
+                           const int *rowptr, const int *col, const double *val,
+                           int flags, hbs_csr *out);
+int hbs_csr_systems(hbs_csr a, int *nsys);
 /* m, n, nnz, number of row blocks, rows handled by serial (order-exact) blocks */
 int hbs_csr_info(hbs_csr a, int *m, int *n, int *nnz, int *nblocks, int *serial_rows);
+/* Replace the values (same pattern) from a device array val_dev[nnz] in CSR
+ * order; the FP32 copy and A^T follow.  Asynchronous on the context stream.
+ * For callers that differentiate through the values. */
+int hbs_csr_set_values(hbs_csr a, const double *val_dev);
 /* Pointers to the device arrays (rowptr, col, val) for callers that need them. */
 int hbs_csr_device_arrays(hbs_csr a, const int **rowptr, const int **col, const double **val);
 
@@ -126,9 +141,25 @@ int hbs_solver_create(hbs_csr a, hbs_solver *out);
 int hbs_solver_destroy(hbs_solver s);
 /* FP64 BiCGStab, the drop-in for the reference loop (bicgstab_cpu.c:172-255,
  * bicgstab_gpu_dp.c:145-208) with the stop test of bicgstab_gpu.c:148-150,
- * 231-236, 289-293.  b_dev, x_dev: device; x holds x0 on entry. */
+ * 231-236, 289-293.  b_dev, x_dev: device; x holds x0 on entry.  On a
+ * matrix of several systems every system is solved at once and info gets
+ * the most iterations, the largest residuals and the worst status. */
 int hbs_bicgstab_f64(hbs_solver s, const double *b_dev, double *x_dev,
                      const hbs_solve_opts *opts, hbs_solve_info *info);
+/* Every system of a hbs_csr_create_systems matrix in the same launches;
+ * infos[nsys] gets each system's iterations, stop test and residual.  Returns
+ * the worst status (HBS_ERR_BREAKDOWN, then HBS_ERR_NOCONV). */
+int hbs_bicgstab_systems_f64(hbs_solver s, const double *b_dev, double *x_dev,
+                             const hbs_solve_opts *opts, hbs_solve_info *infos);
+/* k right-hand sides: B_dev and X_dev are row-major n x k blocks (ld = k),
+ * X holds X0 on entry.  One BiCGStab per column, each with its own scalars,
+ * iteration count and stop test; columns are processed in groups of up to
+ * 64 whose SpMM passes read A once for the whole group.  Each column's
+ * products are summed in CSR order, as the single-vector solve does.
+ * infos[k]; returns the worst status.  No reference counterpart (the
+ * reference solves one right-hand side per run). */
+int hbs_bicgstab_block_f64(hbs_solver s, int k, const double *B_dev, double *X_dev,
+                           const hbs_solve_opts *opts, hbs_solve_info *infos);
 /* Mixed precision: FP32 BiCGStab sweeps (FP32 values and vectors, int32
  * indices, compensated dots) inside FP64 iterative refinement; FP64 for
  * r = b - A x, the scalar reductions and the convergence norm.  Needs

@@ -205,9 +205,41 @@ void orc_vector_sub_seq(const double *a, const double *b, double *out, int n)
  * break when ||s|| < threshold after the alpha step with x already advanced by
  * alpha*p (:219-236), or when ||r|| < threshold after the omega step
  * (:289-293).  Norms are sqrt of the sequential dot. */
+/* Dot product as sums of consecutive blocks of blk elements (each block
+ * sequential), the block sums added in order; blk <= 0 is the reference's
+ * sequential dot. */
+static double dot_blocked(const double *x, const double *y, int n, int blk)
+{
+    if (blk <= 0) return orc_vector_dot_seq(x, y, n);
+    double total = 0.0;
+    for (int i0 = 0; i0 < n; i0 += blk) {
+        const int len = n - i0 < blk ? n - i0 : blk;
+        total += orc_vector_dot_seq(x + i0, y + i0, len);
+    }
+    return total;
+}
+
+static int bicgstab_impl(const orc_csr *a, const double *b, double *x, double tol,
+                         int maxit, int fixed_iters, int blk, int *iters_out,
+                         double *relres_out, int *stop_out);
+
 int orc_bicgstab(const orc_csr *a, const double *b, double *x, double tol,
                  int maxit, int fixed_iters, int *iters_out, double *relres_out,
                  int *stop_out)
+{
+    return bicgstab_impl(a, b, x, tol, maxit, fixed_iters, 0, iters_out, relres_out, stop_out);
+}
+
+int orc_bicgstab_blocked(const orc_csr *a, const double *b, double *x, double tol,
+                         int maxit, int fixed_iters, int blk, int *iters_out,
+                         double *relres_out, int *stop_out)
+{
+    return bicgstab_impl(a, b, x, tol, maxit, fixed_iters, blk, iters_out, relres_out, stop_out);
+}
+
+static int bicgstab_impl(const orc_csr *a, const double *b, double *x, double tol,
+                         int maxit, int fixed_iters, int blk, int *iters_out,
+                         double *relres_out, int *stop_out)
 {
     const int n = a->m;
     if (n <= 0 || a->n != n) return ORC_ERR_ARG;
@@ -227,11 +259,11 @@ int orc_bicgstab(const orc_csr *a, const double *b, double *x, double tol,
     orc_csr_spmv_seq(a, x, Y);
     orc_vector_sub_seq(R, Y, R, n);
     memcpy(R0, R, sizeof(double) * n);
-    const double thr = tol * sqrt(orc_vector_dot_seq(R0, R0, n));
+    const double thr = tol * sqrt(dot_blocked(R0, R0, n, blk));
 
     int it = 0, stop = 0;
     for (it = 0; it < maxit; it++) {
-        double temp_rho = orc_vector_dot_seq(R, R0, n);
+        double temp_rho = dot_blocked(R, R0, n, blk);
         double beta = (temp_rho / rho) * (alpha / omega);
         rho = temp_rho;
         orc_vector_scale_seq(V, omega, V, n);
@@ -239,10 +271,10 @@ int orc_bicgstab(const orc_csr *a, const double *b, double *x, double tol,
         orc_vector_scale_seq(P, beta, P, n);
         orc_vector_add_seq(P, R, P, n);
         orc_csr_spmv_seq(a, P, V);
-        alpha = rho / orc_vector_dot_seq(R0, V, n);
+        alpha = rho / dot_blocked(R0, V, n, blk);
         orc_vector_scale_seq(V, alpha, S, n);
         orc_vector_sub_seq(R, S, S, n);
-        if (!fixed_iters && sqrt(orc_vector_dot_seq(S, S, n)) < thr) {
+        if (!fixed_iters && sqrt(dot_blocked(S, S, n, blk)) < thr) {
             orc_vector_scale_seq(P, alpha, Y, n);
             orc_vector_add_seq(x, Y, x, n);
             stop = 1;
@@ -250,14 +282,14 @@ int orc_bicgstab(const orc_csr *a, const double *b, double *x, double tol,
             break;
         }
         orc_csr_spmv_seq(a, S, T);
-        omega = orc_vector_dot_seq(T, S, n) / orc_vector_dot_seq(T, T, n);
+        omega = dot_blocked(T, S, n, blk) / dot_blocked(T, T, n, blk);
         orc_vector_scale_seq(P, alpha, Y, n);
         orc_vector_add_seq(x, Y, x, n);
         orc_vector_scale_seq(S, omega, Y, n);
         orc_vector_add_seq(x, Y, x, n);
         orc_vector_scale_seq(T, omega, R, n);
         orc_vector_sub_seq(S, R, R, n);
-        if (!fixed_iters && sqrt(orc_vector_dot_seq(R, R, n)) < thr) {
+        if (!fixed_iters && sqrt(dot_blocked(R, R, n, blk)) < thr) {
             stop = 2;
             it++;
             break;

@@ -24,6 +24,8 @@
  *                          out/X_seq.txt, X1.txt, X_mpi_gpu.txt to 2e-6),
  *                          scipy.sparse.linalg.bicgstab cross-check on 64x64
  *                          convection-diffusion, x_target recovery
+ *   orc_bicgstab_blocked   reduces to orc_bicgstab bit for bit when one block
+ *                          holds the vector (an order-sensitivity probe only)
  *   orc_split_row /
  *   orc_loc_split /
  *   orc_setup_comm         invariants: recv.J permutation grouped by ascending
@@ -101,6 +103,15 @@ void   orc_vector_sub_seq(const double *a, const double *b, double *out, int n);
 int orc_bicgstab(const orc_csr *a, const double *b, double *x, double tol,
                  int maxit, int fixed_iters, int *iters_out, double *relres_out,
                  int *stop_out);
+
+/* Same loop with every dot re-associated: sums of consecutive blocks of blk
+ * elements added in order (blk <= 0: the reference's sequential dot).  Not
+ * a restatement of the reference: tests use it only to measure how far an
+ * iteration count moves when the dot sums are re-associated, as any
+ * parallel reduction does. */
+int orc_bicgstab_blocked(const orc_csr *a, const double *b, double *x, double tol,
+                         int maxit, int fixed_iters, int blk, int *iters_out,
+                         double *relres_out, int *stop_out);
 
 int  orc_csr_to_csc(const orc_csr *csr, orc_csr *csc);
 int  orc_shard_csc_to_csr(const orc_csr *csc, orc_csr *csr);

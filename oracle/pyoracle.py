@@ -49,6 +49,8 @@ def lib():
         L.orc_vector_dot_seq.argtypes = [_dp, _dp, C.c_int]
         L.orc_vector_dot_seq.restype = C.c_double
         L.orc_bicgstab.argtypes = [P(OrcCSR), _dp, _dp, C.c_double, C.c_int, C.c_int, _ip, _dp, _ip]
+        L.orc_bicgstab_blocked.argtypes = [P(OrcCSR), _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int, _ip, _dp,
+                                           _ip]
         L.orc_csr_to_csc.argtypes = [P(OrcCSR), P(OrcCSR)]
         L.orc_shard_csc_to_csr.argtypes = [P(OrcCSR), P(OrcCSR)]
         L.orc_split_row.argtypes = [P(OrcCSR), _ip, C.c_int, C.c_int, P(OrcCSR)]
@@ -131,6 +133,33 @@ def bicgstab(A, b, x0=None, tol=1e-8, maxit=1000, fixed=False):
     if rc != 0:
         raise RuntimeError(f"orc_bicgstab failed: {rc}")
     return x, it.value, rr.value, st.value
+
+
+def bicgstab_blocked(A, b, blk, x0=None, tol=1e-8, maxit=1000, fixed=False):
+    """bicgstab with block-summed dots (blk elements per block); returns
+    (x, iters, relres, stop).  Measures order sensitivity only."""
+    x = np.zeros(A.m) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    it, st = C.c_int(), C.c_int()
+    rr = C.c_double()
+    rc = lib().orc_bicgstab_blocked(C.byref(A.s), _dp_of(b), _dp_of(x), tol, maxit, 1 if fixed else 0, blk,
+                                    C.byref(it), C.byref(rr), C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_bicgstab_blocked failed: {rc}")
+    return x, it.value, rr.value, st.value
+
+
+def iteration_spread(A, b, tol=1e-8, maxit=1000, blocks=(512, 64)):
+    """How far re-associating the dot sums moves the oracle's solve:
+    (sequential count, min count, max count, [solutions: sequential first,
+    then one per block size])."""
+    xs, its, _, _ = bicgstab(A, b, tol=tol, maxit=maxit)
+    counts, sols = [its], [xs]
+    for k in blocks:
+        xk, ik, _, _ = bicgstab_blocked(A, b, k, tol=tol, maxit=maxit)
+        counts.append(ik)
+        sols.append(xk)
+    return its, min(counts), max(counts), sols
 
 
 def csr_to_csc(A):

@@ -1,0 +1,186 @@
+"""Batched BiCGStab on the MI355X: many uneven systems in the same launches
+(hbs_bicgstab_systems_f64) and k right-hand sides sharing each SpMM pass
+(hbs_bicgstab_block_f64).  Neither has a reference counterpart; each system
+or column is held to the single-system bar against the oracle's
+restatement of the reference CPU solver (bicgstab_cpu.c:172-255 with the
+stop test of bicgstab_gpu.c:148-150,231-236,289-293): iterations within
++-2, relative residual <= 1e-8 (1% slack), solution within 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from conftest import golden
+from hethyppar_amd import hbs
+from workloads import sample320
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(x, info, xo, ito, what, A=None, b=None, xt=None):
+    print(f"{what}: gpu iters {info.iters} stop {info.stop} relres {info.relres:.3e} | oracle iters {ito}",
+          flush=True)
+    assert info.status == 0
+    assert info.relres <= 1.01e-8
+    dist = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    if (abs(info.iters - ito) > 2 or dist > 1e-6) and A is not None:
+        # an order-sensitive system: the oracle's own count and stopping
+        # iterate move when its dot sums are re-associated (as every parallel
+        # reduction re-associates them).  The bar: that range of counts +-2,
+        # and a forward error (against the known solution xt) within 4x the
+        # worst of the re-associated oracle runs (all stop at relres 1e-8,
+        # where the forward error is only bounded by cond(A) * 1e-8)
+        _, lo, hi, sols = orc.iteration_spread(A, b, maxit=2000)
+        fe = lambda y: np.linalg.norm(y - xt) / np.linalg.norm(xt)
+        worst = max(fe(y) for y in sols)
+        print(f"{what}: re-associated oracle runs span {lo}..{hi} iterations, forward errors up to "
+              f"{worst:.2e}; gpu {fe(x):.2e} ({dist:.2e} from the oracle)", flush=True)
+        assert lo - 2 <= info.iters <= hi + 2
+        assert fe(x) <= 4 * worst
+    else:
+        assert abs(info.iters - ito) <= 2
+        assert dist <= 1e-6
+
+
+def _systems(nsys, lo, hi, seed):
+    sysl = hbs.gen_uneven_systems(nsys, lo, hi, seed)
+    rp, c, v, rows = hbs.concat_systems([(a, b, d) for a, b, d, _ in sysl])
+    return sysl, rp, c, v, rows
+
+
+def test_uneven_systems_each_match_oracle(ctx):
+    sysl, rp, c, v, rows = _systems(12, 1.5, 3.4, 5)
+    assert np.any(np.diff(rows) % 4 != 0)          # systems start off the 16-byte grid
+    n = int(rows[-1])
+    xt = hbs.gen_uniform(n, -1.0, 1.0, 7)
+    b = hbs.host_spmv(rp, c, v, xt)
+    A = hbs.DeviceCSR(ctx, rp, c, v, sys_rows=rows)
+    S = hbs.Solver(A)
+    db, dx = ctx.to_device(b), ctx.empty(n)
+    dx.zero()
+    infos = S.bicgstab_systems(db, dx, tol=1e-8, maxit=2000)
+    x = dx.download()
+    its = []
+    for q, (srp, sc, sv, conv) in enumerate(sysl):
+        r0, r1 = rows[q], rows[q + 1]
+        Aq = orc.CSR(srp, sc, sv)
+        xo, ito, _, _ = orc.bicgstab(Aq, b[r0:r1], tol=1e-8, maxit=2000)
+        _check(x[r0:r1], infos[q], xo, ito, f"system {q} n={r1 - r0} conv={conv:.2f}", Aq, b[r0:r1],
+               xt[r0:r1])
+        its.append(infos[q].iters)
+    assert len(set(its)) > 1                        # systems stopped at their own iteration
+
+
+def test_uneven_systems_caps_and_zero_rhs(ctx):
+    sysl, rp, c, v, rows = _systems(5, 1.5, 3.0, 11)
+    n = int(rows[-1])
+    xt = hbs.gen_uniform(n, -1.0, 1.0, 3)
+    b = hbs.host_spmv(rp, c, v, xt)
+    b[rows[2]:rows[3]] = 0.0                        # system 2: zero right-hand side
+    A = hbs.DeviceCSR(ctx, rp, c, v, sys_rows=rows)
+    S = hbs.Solver(A)
+    # iteration cap between the smallest and the largest system's count
+    full = []
+    for q, (srp, sc, sv, _) in enumerate(sysl):
+        _, ito, _, _ = orc.bicgstab(orc.CSR(srp, sc, sv), b[rows[q]:rows[q + 1]], tol=1e-8, maxit=2000)
+        full.append(ito)
+    cap = sorted(full)[2] + 3
+    db, dx = ctx.to_device(b), ctx.empty(n)
+    dx.zero()
+    infos = S.bicgstab_systems(db, dx, tol=1e-8, maxit=cap, raise_on_error=False)
+    x = dx.download()
+    for q in range(5):
+        print(f"system {q}: oracle {full[q]} gpu {infos[q].iters} stop {infos[q].stop} status {infos[q].status}",
+              flush=True)
+        if q == 2:
+            assert infos[q].iters == 0 and infos[q].stop == 2 and infos[q].status == 0
+            assert np.all(x[rows[2]:rows[3]] == 0.0)
+        elif full[q] <= cap - 3:
+            assert infos[q].status == 0 and abs(infos[q].iters - full[q]) <= 2
+        elif full[q] > cap + 2:
+            assert infos[q].status == hbs.HBS_ERR_NOCONV and infos[q].iters == cap
+    # the aggregate call reports the worst system
+    dx.zero()
+    agg = S.bicgstab(db, dx, tol=1e-8, maxit=cap, raise_on_error=False)
+    assert agg.status == (hbs.HBS_ERR_NOCONV if max(full) > cap + 2 else 0)
+    assert agg.iters == max(i.iters for i in infos)
+
+
+def test_systems_single_system_same_as_plain(ctx):
+    rp, c, v = hbs.gen_convdiff2d(30, 30, 0.5)
+    b = hbs.host_spmv(rp, c, v, hbs.gen_uniform(900, -1.0, 1.0, 1))
+    A1 = hbs.DeviceCSR(ctx, rp, c, v)
+    A2 = hbs.DeviceCSR(ctx, rp, c, v, sys_rows=[0, 900])
+    xs = []
+    for A in (A1, A2):
+        S = hbs.Solver(A)
+        db, dx = ctx.to_device(b), ctx.empty(900)
+        dx.zero()
+        info = S.bicgstab(db, dx, tol=1e-8)
+        xs.append((dx.download(), info.iters))
+    assert xs[0][1] == xs[1][1] and np.array_equal(xs[0][0], xs[1][0])
+
+
+def test_systems_reject_cross_coupling(ctx):
+    rp = np.array([0, 2, 3], np.int32)
+    c = np.array([0, 1, 1], np.int32)
+    v = np.ones(3)
+    with pytest.raises(hbs.HBSError):
+        hbs.DeviceCSR(ctx, rp, c, v, sys_rows=[0, 1, 2])
+
+
+@pytest.mark.parametrize("k", [1, 8, 70])
+def test_block_columns_match_oracle(ctx, k):
+    rp, c, v = hbs.gen_convdiff2d(24, 20, 0.7)
+    n = rp.size - 1
+    Xt = hbs.gen_uniform(n * k, -1.0, 1.0, 40 + k).reshape(n, k)
+    B = np.stack([hbs.host_spmv(rp, c, v, np.ascontiguousarray(Xt[:, j])) for j in range(k)], axis=1)
+    B[:, k // 2] = 0.0                                     # one zero column
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    S = hbs.Solver(A)
+    dB = ctx.to_device(np.ascontiguousarray(B).ravel())
+    dX = ctx.to_device(np.zeros(n * k))
+    infos = S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=1000)
+    X = dX.download().reshape(n, k)
+    for j in range(k):
+        if j == k // 2:
+            assert infos[j].iters == 0 and infos[j].stop == 2 and np.all(X[:, j] == 0.0)
+            continue
+        bj = np.ascontiguousarray(B[:, j])
+        xo, ito, _, _ = orc.bicgstab(orc.CSR(rp, c, v), bj, tol=1e-8, maxit=1000)
+        _check(X[:, j], infos[j], xo, ito, f"column {j} of {k}", orc.CSR(rp, c, v), bj,
+               np.ascontiguousarray(Xt[:, j]))
+
+
+def test_block_fixed_iterations_golden(ctx):
+    rp, c, v = sample320()
+    b = hbs.vector_read(golden("sample320", "B.txt"), 320)
+    x0 = hbs.vector_read(golden("sample320", "X_init.txt"), 320)
+    k = 3
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    S = hbs.Solver(A)
+    dB = ctx.to_device(np.repeat(b[:, None], k, axis=1).ravel())
+    dX = ctx.to_device(np.repeat(x0[:, None], k, axis=1).ravel())
+    infos = S.bicgstab_block(dB, dX, k, maxit=20, fixed_iters=1)
+    X = dX.download().reshape(320, k)
+    xo, _, _, _ = orc.bicgstab(orc.CSR(rp, c, v), b, x0, maxit=20, fixed=True)
+    gold = hbs.vector_read(golden("sample320", "out_X_seq.txt"), 320)
+    for j in range(k):
+        assert infos[j].iters == 20
+        assert np.max(np.abs(X[:, j] - xo)) < 1e-9
+        assert np.max(np.abs(X[:, j] - gold)) <= 1e-6 + 1e-9
+
+
+def test_block_graph_and_eager_agree(ctx):
+    rp, c, v = hbs.gen_convdiff2d(16, 16, 0.3)
+    n, k = 256, 5
+    B = hbs.gen_uniform(n * k, -1.0, 1.0, 9)
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    S = hbs.Solver(A)
+    out = []
+    for graph, every in ((1, 4), (0, 1), (1, 8)):
+        dB, dX = ctx.to_device(B), ctx.to_device(np.zeros(n * k))
+        infos = S.bicgstab_block(dB, dX, k, tol=1e-10, use_graph=graph, check_every=every)
+        out.append((dX.download(), [i.iters for i in infos]))
+    for x, its in out[1:]:
+        assert its == out[0][1] and np.array_equal(x, out[0][0])
