@@ -303,8 +303,9 @@ int hbs_bench_copy(hbs_ctx ctx, const double *src_dev, double *dst_dev, size_t n
 /* nnz per row block (<= 4096) and the average row length above which a
  * block uses several lanes per row instead of the order-exact serial sum. */
 int hbs_set_spmv_plan(int block_nnz, int serial_max_avg);
-/* Workgroups of the SpMV kernel (<= 1024); each takes a contiguous run of
- * row blocks, and its dot partial is one of those a consumer sums. */
+/* Cap on the SpMV workgroups of matrices created afterwards (1..65536,
+ * default 4096); each takes a contiguous run of row blocks, and its dot
+ * partial is one of those a consumer sums. */
 int hbs_set_spmv_groups(int max_groups);
 /* 1: load the matrix streams (col, val) non-temporally (default 0: measured
  * slower on MI355X for every workload tried). */
