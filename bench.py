@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=445000)
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--check-every", type=int, default=4,
+                    help="iterations per convergence poll of the FP64 solve")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true",
@@ -101,7 +103,7 @@ def main():
     S = hbs.Solver(A)
     db = ctx.to_device(b)
     dx = ctx.empty(n)
-    opts = hbs.default_opts(tol=1e-8, maxit=1000, use_graph=1, check_every=8)
+    opts = hbs.default_opts(tol=1e-8, maxit=1000, use_graph=1, check_every=args.check_every)
 
     def solve():
         dx.zero()
@@ -172,6 +174,8 @@ def main():
 
     if not args.no_extras:
         result["extras"] = extras(hbs, ctx, rp, c, v, n)
+        result["extras"].update(extras_batched(hbs, ctx))
+        result["extras"]["fem_1p45M_adjoint"] = extras_adjoint(hbs, ctx)
 
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, 1 thread)")
@@ -196,6 +200,133 @@ def probe(hbs, ctx, A, xt, n, nnz):
                       "spmv_bytes": spmv_bytes(n, n, nnz), "copy_doubles": cn,
                       "copy_ms": cms.value}), flush=True)
     ctx.close()
+
+
+def timed(ctx, fn, reps=1):
+    """Device-synchronised wall time of fn() in ms (best of reps) and its
+    last result."""
+    best, res = None, None
+    for _ in range(reps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        res = fn()
+        ctx.sync()
+        dt = (time.perf_counter() - t0) * 1e3
+        best = dt if best is None else min(best, dt)
+    return best, res
+
+
+def extras_batched(hbs, ctx):
+    """128^3 7-point convection-diffusion with 64 right-hand sides (one
+    BiCGStab per column over a shared SpMM), and 256 uneven 2-D systems in
+    the same launches."""
+    out = {}
+    log("bench: 128^3 x 64 right-hand sides")
+    rp, c, v = hbs.gen_convdiff3d(128, 128, 128, 0.5)
+    n, k = rp.size - 1, 64
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    S = hbs.Solver(A)
+    dXt = ctx.to_device(hbs.gen_uniform(n * k, -1.0, 1.0, 4))
+    dB, dX = ctx.empty(n * k), ctx.empty(n * k)
+    A.spmm(dXt, dB, k)
+    o = hbs.default_opts(tol=1e-8, maxit=2000, use_graph=1, check_every=8)
+
+    def block():
+        dX.zero()
+        return S.bicgstab_block(dB, dX, k, o)
+    block()
+    ms, infos = timed(ctx, block)
+    its = [i.iters for i in infos]
+    nnz = int(c.size)
+    spmm_b = nnz * 12 + 4 * (n + 1) + k * 8 * 2 * n
+    ctx.timer_start()
+    for _ in range(5):
+        A.spmm(dXt, dB, k)
+    sp_ms = ctx.timer_stop() / 5
+    dXt32, dB32 = ctx.empty(n * k, np.float32), ctx.empty(n * k, np.float32)
+    A.spmm(dXt32, dB32, k)
+    ctx.timer_start()
+    for _ in range(5):
+        A.spmm(dXt32, dB32, k)
+    sp32_ms = ctx.timer_stop() / 5
+    spmm32_b = nnz * 8 + 4 * (n + 1) + k * 4 * 2 * n
+    out["block_128cube_k64"] = {
+        "n": n, "nnz": nnz, "k": k, "ms": round(ms, 2), "iters_min": min(its), "iters_max": max(its),
+        "relres_max": max(i.relres for i in infos), "status_worst": min(i.status for i in infos),
+        "spmm_fp64": {"ms": round(sp_ms, 4), "gbs": round(spmm_b / (sp_ms * 1e-3) / 1e9, 1)},
+        "spmm_fp32": {"ms": round(sp32_ms, 4), "gbs": round(spmm32_b / (sp32_ms * 1e-3) / 1e9, 1)}}
+    for d in (dXt, dB, dX, dXt32, dB32):
+        d.free()
+    S.close()
+    A.close()
+
+    log("bench: 256 uneven systems")
+    sysl = hbs.gen_uneven_systems(256)
+    rp, c, v, rows = hbs.concat_systems([(a, b, d) for a, b, d, _ in sysl])
+    n = int(rows[-1])
+    b = hbs.host_spmv(rp, c, v, hbs.gen_uniform(n, -1.0, 1.0, 5))
+    A = hbs.DeviceCSR(ctx, rp, c, v, sys_rows=rows)
+    S = hbs.Solver(A)
+    db, dx = ctx.to_device(b), ctx.empty(n)
+    o = hbs.default_opts(tol=1e-8, maxit=5000, use_graph=1, check_every=8)
+
+    def systems():
+        dx.zero()
+        return S.bicgstab_systems(db, dx, o, raise_on_error=False)
+    systems()
+    ms, infos = timed(ctx, systems, reps=2)
+    its = sorted(i.iters for i in infos)
+    out["systems_256_uneven"] = {
+        "n_total": n, "nnz": int(c.size), "ms": round(ms, 2), "iters_median": its[len(its) // 2],
+        "iters_max": its[-1], "relres_max": max(i.relres for i in infos),
+        "converged": sum(1 for i in infos if i.status == 0)}
+    S.close()
+    A.close()
+    return out
+
+
+def extras_adjoint(hbs, ctx):
+    """FEM-like 3x3-block system, n = 1,450,002, 52.2M entries: FP64 solve,
+    then the adjoint of L = 0.5 ||x - x_target||^2 (A^T solve + gradient on
+    the pattern)."""
+    log("bench: FEM 1.45M forward + adjoint")
+    rp, c, v = hbs.gen_fem(483334, 12, 1.1, 6)
+    n, nnz = rp.size - 1, int(c.size)
+    b = hbs.host_spmv(rp, c, v, hbs.gen_uniform(n, -1.0, 1.0, 61))
+    xt = hbs.gen_uniform(n, -1.0, 1.0, 62)
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_TRANSPOSE)
+    S = hbs.Solver(A)
+    db, dx, dxt = ctx.to_device(b), ctx.empty(n), ctx.to_device(xt)
+    dg, dl, dgv = ctx.empty(n), ctx.empty(n), ctx.empty(nnz)
+    o = hbs.default_opts(tol=1e-8, maxit=2000, use_graph=1, check_every=4)
+
+    def forward():
+        dx.zero()
+        return S.bicgstab(db, dx, o)
+
+    def adjoint():
+        return S.adjoint(dg, dx, dl, dgv, o)
+    forward()
+    fms, finfo = timed(ctx, forward, reps=2)
+    x = dx.download()
+    dg.upload(x - xt)
+    adjoint()
+    ams, ainfo = timed(ctx, adjoint, reps=2)
+    ys = ctx.empty(n)
+    spmv_ms = A.bench_spmv(dx, ys, 20)
+    ctx.timer_start()
+    for _ in range(20):
+        A.spmv_t(dx, ys)
+    t_ms = ctx.timer_stop() / 20
+    B = spmv_bytes(n, n, nnz)
+    res = {"n": n, "nnz": nnz,
+           "forward": {"ms": round(fms, 3), "iters": finfo.iters, "relres": finfo.relres},
+           "adjoint": {"ms": round(ams, 3), "iters": ainfo.iters, "relres": ainfo.relres},
+           "spmv": {"ms": round(spmv_ms, 4), "gbs": round(B / (spmv_ms * 1e-3) / 1e9, 1)},
+           "spmv_transpose": {"ms": round(t_ms, 4), "gbs": round(B / (t_ms * 1e-3) / 1e9, 1)}}
+    S.close()
+    A.close()
+    return res
 
 
 def extras(hbs, ctx, rp, c, v, n):

@@ -18,7 +18,7 @@ using namespace hbs;
 
 namespace {
 thread_local std::string g_err;
-int g_block_nnz = 2048;
+int g_block_nnz = 1024;   // defaults from whole-solve sweeps on MI355X
 int g_serial_max_avg = 64;
 int g_max_groups = kMaxGroups;
 int g_stream_nt = 0;
@@ -271,6 +271,13 @@ extern "C" int hbs_set_spmv_groups(int max_groups)
 {
     if (max_groups < 1 || max_groups > 65536) return fail(HBS_ERR_ARG, "max_groups must be in [1, 65536]");
     g_max_groups = max_groups;
+    return HBS_OK;
+}
+
+extern "C" int hbs_set_vec_blocks(int target)
+{
+    if (target < 1 || target > 65536) return fail(HBS_ERR_ARG, "target must be in [1, 65536]");
+    g_vec_target = target;
     return HBS_OK;
 }
 

@@ -94,16 +94,33 @@ __device__ __forceinline__ T block_sum1(T v, T *red)
     return t;
 }
 
-// Sum of partials [p0, p1) (stride nparts between the ND rows), fixed order.
+// Sum of partials [p0, p1) (stride nparts between the ND rows), fixed order:
+// thread t adds p0 + t, p0 + t + 256, ... in turn, then block_sum.  The loads
+// of a span of kSpan partials per thread are all issued before the first
+// add, so a consumer's prologue costs one L2 round trip per span rather than
+// one per partial.
+constexpr int kSpan = 16;
 template <int ND>
 __device__ __forceinline__ void reduce_parts(const double *part, int nparts, int p0, int p1, double (&v)[ND],
                                              double *red)
 {
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        double acc = 0.0;
-        for (int i = p0 + (int)threadIdx.x; i < p1; i += kThreads) acc += part[(size_t)d * nparts + i];
-        v[d] = acc;
+    for (int d = 0; d < ND; ++d) v[d] = 0.0;
+    for (int b = p0 + (int)threadIdx.x; b < p1; b += kSpan * kThreads) {
+        double buf[ND][kSpan];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+#pragma unroll
+            for (int k = 0; k < kSpan; ++k) {
+                const int i = b + k * kThreads;
+                buf[d][k] = i < p1 ? part[(size_t)d * nparts + i] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+#pragma unroll
+            for (int k = 0; k < kSpan; ++k) v[d] += buf[d][k];
+        }
     }
     block_sum<ND>(v, red);
 }

@@ -16,7 +16,7 @@ constexpr int kThreads = 256;          // workgroup size of every kernel (4 wave
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxBlockNnz = 4096;     // upper bound of nnz per SpMV row block
 constexpr int kMaxDots = 2;            // reductions fused into one pass
-constexpr int kMaxGroups = 4096;       // SpMV workgroups by default = partials a consumer sums
+constexpr int kMaxGroups = 2048;       // SpMV workgroups by default = partials a consumer sums
 
 // One SpMV row block: rows [r0, r1) with entries [j0, j1).
 struct alignas(16) RowBlock {
@@ -207,6 +207,7 @@ void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan);
 // Vector blocks of elements [e0, e0 + n) of one system.
 void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock> &blocks);
 int vec_elems_per_block(int n);
+extern int g_vec_target;      // vector workgroups a plan aims at (hbs_set_vec_blocks)
 int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 
 // ---- launchers (hbs_kernels.hip) ----

@@ -59,7 +59,117 @@ __device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, Acc<
     }
 }
 
-template <typename T, int EPI>
+// The entries of one row block a thread holds in registers (entries
+// t, t + 256, ..., t + 256 * (NPT - 1) of the block), plus the row pointers
+// of its first row, all loaded before they are needed.
+template <typename T, int NPT>
+struct BlockRegs {
+    T prod[NPT];
+    int rs, re;
+};
+
+template <typename T, int NPT>
+__device__ __forceinline__ void load_block(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax,
+                                           BlockRegs<T, NPT> &br)
+{
+    if (ax.kind == 1) return;
+    const int nz = rb.j1 - rb.j0;
+    const int *__restrict__ colp = a.col + rb.j0;
+    const T *__restrict__ valp = a.val + rb.j0;
+    int c[NPT];
+    T v[NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int k = (int)threadIdx.x + i * kThreads;
+        if (a.stream_nt) {
+            c[i] = k < nz ? __builtin_nontemporal_load(colp + k) : 0;
+            v[i] = k < nz ? __builtin_nontemporal_load(valp + k) : T(0);
+        } else {
+            c[i] = k < nz ? colp[k] : 0;
+            v[i] = k < nz ? valp[k] : T(0);
+        }
+    }
+    const int first = ax.lanes <= 1 ? (int)threadIdx.x : (int)threadIdx.x / ax.lanes;
+    if (first < rb.r1 - rb.r0) {
+        br.rs = a.rowptr[rb.r0 + first];
+        br.re = a.rowptr[rb.r0 + first + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int k = (int)threadIdx.x + i * kThreads;
+        br.prod[i] = k < nz ? v[i] * a.x[c[i]] : T(0);
+    }
+}
+
+// Sums the rows of a loaded block and feeds the epilogue.  Serial blocks add
+// each row's products in CSR order from 0 (the reference's order); others
+// use `lanes` lanes per row and an xor butterfly.
+template <typename T, int EPI, int NPT>
+__device__ __forceinline__ void sum_block(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax,
+                                          const BlockRegs<T, NPT> &br, T *lds, Acc<T> &d0, Acc<T> &d1)
+{
+    if (ax.kind == 1) {
+        // long row: ordered sum of its chunk sums
+        if (threadIdx.x == 0) {
+            T total = T(0);
+            for (int c = 0; c < ax.nchunks; ++c) total += (T)a.chunk_sums[ax.first + c];
+            row_out<T, EPI>(a, rb.r0, total, d0, d1);
+        }
+        return;
+    }
+    const int nz = rb.j1 - rb.j0;
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int k = (int)threadIdx.x + i * kThreads;
+        if (k < nz) lds[k] = br.prod[i];
+    }
+    __syncthreads();
+    const int nr = rb.r1 - rb.r0;
+    const int lanes = ax.lanes;
+    if (lanes <= 1) {
+        for (int t = threadIdx.x; t < nr; t += kThreads) {
+            const int r = rb.r0 + t;
+            int s, e;
+            if (t == (int)threadIdx.x) {
+                s = br.rs - rb.j0;
+                e = br.re - rb.j0;
+            } else {
+                s = a.rowptr[r] - rb.j0;
+                e = a.rowptr[r + 1] - rb.j0;
+            }
+            T acc = T(0);
+            for (int q = s; q < e; ++q) acc += lds[q];
+            row_out<T, EPI>(a, r, acc, d0, d1);
+        }
+    } else {
+        const int gi = threadIdx.x / lanes, l = threadIdx.x & (lanes - 1);
+        const int ng = kThreads / lanes;
+        for (int t0 = 0; t0 < nr; t0 += ng) {
+            const int t = t0 + gi;
+            T acc = T(0);
+            const int r = rb.r0 + t;
+            if (t < nr) {
+                int s, e;
+                if (t0 == 0) {
+                    s = br.rs - rb.j0;
+                    e = br.re - rb.j0;
+                } else {
+                    s = a.rowptr[r] - rb.j0;
+                    e = a.rowptr[r + 1] - rb.j0;
+                }
+                for (int q = s + l; q < e; q += lanes) acc += lds[q];
+            }
+            for (int o = lanes >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+            if (t < nr && l == 0) row_out<T, EPI>(a, r, acc, d0, d1);
+        }
+    }
+    __syncthreads();   // LDS is reused by the next row block
+}
+
+// NPT = block_nnz / 256 entries per thread.  A finished system returns
+// before any load; otherwise the first row block's loads are issued before
+// the rest of the state prologue, so they overlap it.
+template <typename T, int EPI, int NPT>
 __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -70,13 +180,19 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
 
     const int p = xcd_remap(blockIdx.x, a.ngroups);
     const Group g = a.groups[p];
+    if (a.st_in && a.check_done && a.st_in[g.sys].done) {
+        if (a.st_out && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_out[g.sys] = a.st_in[g.sys];
+        return;
+    }
+    int b = g.rb0;
+    RowBlock rb = a.blocks[b];
+    BlockAux ax = a.aux[b];
+    BlockRegs<T, NPT> br;
+    load_block<T, NPT>(a, rb, ax, br);
+
     if (a.st_in) {
         if (threadIdx.x == 0) sh = a.st_in[g.sys];
         __syncthreads();
-        if (a.check_done && sh.done) {
-            if (a.st_out && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_out[g.sys] = sh;
-            return;
-        }
         if (a.fin_in == FIN_S) {
             const SysRange sr = a.sys[g.sys];
             double v[1];
@@ -92,60 +208,12 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
     if (a.st_mark && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_mark[g.sys].pending = 1;
 
     Acc<T> d0, d1;
-    const T *__restrict__ x = a.x;
-    for (int b = g.rb0; b < g.rb1; ++b) {
-        const RowBlock rb = a.blocks[b];
-        const BlockAux ax = a.aux[b];
-        if (ax.kind == 1) {
-            // long row: ordered sum of its chunk sums
-            if (threadIdx.x == 0) {
-                T total = T(0);
-                for (int c = 0; c < ax.nchunks; ++c) total += (T)a.chunk_sums[ax.first + c];
-                row_out<T, EPI>(a, rb.r0, total, d0, d1);
-            }
-            continue;
-        }
-        // phase 1: coalesced col/val streams, x gathered, one product per entry
-        const int nz = rb.j1 - rb.j0;
-        const int *__restrict__ colp = a.col + rb.j0;
-        const T *__restrict__ valp = a.val + rb.j0;
-        if (a.stream_nt) {
-            // col/val are read once: non-temporal, so they do not push x out of L2
-#pragma unroll 4
-            for (int k = threadIdx.x; k < nz; k += kThreads)
-                lds[k] = __builtin_nontemporal_load(valp + k) * x[__builtin_nontemporal_load(colp + k)];
-        } else {
-#pragma unroll 4
-            for (int k = threadIdx.x; k < nz; k += kThreads) lds[k] = valp[k] * x[colp[k]];
-        }
-        __syncthreads();
-        const int nr = rb.r1 - rb.r0;
-        const int lanes = ax.lanes;
-        if (lanes <= 1) {
-            // order-exact: res = 0; res += p_j in CSR order
-            for (int t = threadIdx.x; t < nr; t += kThreads) {
-                const int r = rb.r0 + t;
-                const int s = a.rowptr[r] - rb.j0, e = a.rowptr[r + 1] - rb.j0;
-                T acc = T(0);
-                for (int q = s; q < e; ++q) acc += lds[q];
-                row_out<T, EPI>(a, r, acc, d0, d1);
-            }
-        } else {
-            const int gi = threadIdx.x / lanes, l = threadIdx.x & (lanes - 1);
-            const int ng = kThreads / lanes;
-            for (int t0 = 0; t0 < nr; t0 += ng) {
-                const int t = t0 + gi;
-                T acc = T(0);
-                const int r = rb.r0 + t;
-                if (t < nr) {
-                    const int s = a.rowptr[r] - rb.j0, e = a.rowptr[r + 1] - rb.j0;
-                    for (int q = s + l; q < e; q += lanes) acc += lds[q];
-                }
-                for (int o = lanes >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-                if (t < nr && l == 0) row_out<T, EPI>(a, r, acc, d0, d1);
-            }
-        }
-        __syncthreads();   // LDS is reused by the next row block
+    for (;;) {
+        sum_block<T, EPI, NPT>(a, rb, ax, br, lds, d0, d1);
+        if (++b >= g.rb1) break;
+        rb = a.blocks[b];
+        ax = a.aux[b];
+        load_block<T, NPT>(a, rb, ax, br);
     }
 
     if constexpr (ND > 0) {
@@ -482,20 +550,29 @@ __global__ void k_state_init(SolverState *st, int nstates, double tol, int maxit
     }
 }
 
+template <typename T, int NPT>
+hipError_t spmv_epi(const SpmvArgs<T> &a, int epi, size_t shmem, hipStream_t s)
+{
+    switch (epi) {
+    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    case EPI_RESID: k_spmv<T, EPI_RESID, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, hipStream_t s)
 {
     if (a.ngroups <= 0) return hipSuccess;
+    if (block_nnz <= 0 || block_nnz > kMaxBlockNnz) return hipErrorInvalidValue;
     const size_t shmem = (size_t)block_nnz * sizeof(T);
-    switch (epi) {
-    case EPI_AXPBY: k_spmv<T, EPI_AXPBY><<<a.ngroups, kThreads, shmem, s>>>(a); break;
-    case EPI_DOT1: k_spmv<T, EPI_DOT1><<<a.ngroups, kThreads, shmem, s>>>(a); break;
-    case EPI_DOT2: k_spmv<T, EPI_DOT2><<<a.ngroups, kThreads, shmem, s>>>(a); break;
-    case EPI_RESID: k_spmv<T, EPI_RESID><<<a.ngroups, kThreads, shmem, s>>>(a); break;
-    case EPI_RESID2: k_spmv<T, EPI_RESID2><<<a.ngroups, kThreads, shmem, s>>>(a); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    if (block_nnz <= 4 * kThreads) return spmv_epi<T, 4>(a, epi, shmem, s);
+    if (block_nnz <= 8 * kThreads) return spmv_epi<T, 8>(a, epi, shmem, s);
+    return spmv_epi<T, 16>(a, epi, shmem, s);
 }
 
 template <typename T>

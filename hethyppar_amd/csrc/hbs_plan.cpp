@@ -90,14 +90,16 @@ void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock>
     }
 }
 
+int g_vec_target = 512;
+
 // Elements per vector workgroup: 512 * u with u in {1, 2, 4, 8}, the
-// smallest that keeps the workgroups (and the partials a consumer sums) near
-// 2048 or fewer.  An FP64 thread then holds u 16-byte packs, an FP32 thread
-// max(1, u/2).
+// smallest that keeps the workgroups (and the partials a consumer sums) at
+// g_vec_target or fewer.  An FP64 thread then holds u 16-byte packs, an FP32
+// thread max(1, u/2).
 int vec_elems_per_block(int n)
 {
     int u = 1;
-    while (u < 8 && (long long)n > 512LL * u * 2048) u *= 2;
+    while (u < 8 && (long long)n > 512LL * u * g_vec_target) u *= 2;
     return 512 * u;
 }
 

@@ -137,6 +137,7 @@ _PROTOS = {
     "hbs_set_spmv_plan": (_i, [_i, _i]),
     "hbs_set_spmv_groups": (_i, [_i]),
     "hbs_set_spmv_stream_nt": (_i, [_i]),
+    "hbs_set_vec_blocks": (_i, [_i]),
     "hbs_csr_plan_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "hbs_shard_pack_host": (_i, [_vp, _vp, _dp]),
     "hbs_shard_spmv2": (_i, [_vp, _vp, _dp, _vp, _d, _d, _vp]),
@@ -682,8 +683,13 @@ class Shard:
             pass
 
 
-def set_spmv_plan(block_nnz=2048, serial_max_avg=64, max_groups=4096, stream_nt=0):
+def set_spmv_plan(block_nnz=1024, serial_max_avg=64, max_groups=2048, stream_nt=0):
     """Process-wide plan knobs used by DeviceCSR objects created afterwards."""
     check(lib().hbs_set_spmv_plan(block_nnz, serial_max_avg))
     check(lib().hbs_set_spmv_groups(max_groups))
     check(lib().hbs_set_spmv_stream_nt(stream_nt))
+
+
+def set_vec_blocks(target=512):
+    """Process-wide: vector workgroups a Solver created afterwards aims at."""
+    check(lib().hbs_set_vec_blocks(target))

@@ -304,9 +304,14 @@ int hbs_bench_copy(hbs_ctx ctx, const double *src_dev, double *dst_dev, size_t n
  * block uses several lanes per row instead of the order-exact serial sum. */
 int hbs_set_spmv_plan(int block_nnz, int serial_max_avg);
 /* Cap on the SpMV workgroups of matrices created afterwards (1..65536,
- * default 4096); each takes a contiguous run of row blocks, and its dot
+ * default 2048); each takes a contiguous run of row blocks, and its dot
  * partial is one of those a consumer sums. */
 int hbs_set_spmv_groups(int max_groups);
+/* Vector workgroups a solver created afterwards aims at (1..65536, default
+ * 512): each takes 512, 1024, 2048 or 4096 elements, the fewest that keep
+ * the count at or below the target.  Fewer workgroups mean fewer partials
+ * for each consumer to re-sum. */
+int hbs_set_vec_blocks(int target);
 /* 1: load the matrix streams (col, val) non-temporally (default 0: measured
  * slower on MI355X for every workload tried). */
 int hbs_set_spmv_stream_nt(int on);

@@ -64,7 +64,7 @@ def test_spmv_plans_power_law(ctx, block_nnz, serial):
     try:
         A, y = _run_spmv(ctx, rp, c, v, x)
     finally:
-        hbs.set_spmv_plan(2048, 64)
+        hbs.set_spmv_plan()
     ref = orc.spmv(orc.CSR(rp, c, v), x)
     bound = seq_bound(rp, c, v, x)
     assert np.all(np.abs(y - ref) <= bound + 1e-300)
