@@ -18,10 +18,11 @@ using namespace hbs;
 
 namespace {
 thread_local std::string g_err;
-int g_block_nnz = 1024;   // defaults from whole-solve sweeps on MI355X
+int g_block_nnz = 0;      // 0: by average row length (defaults from sweeps on MI355X)
 int g_serial_max_avg = 64;
 int g_max_groups = kMaxGroups;
 int g_stream_nt = 0;
+int g_sort_pct = 70;      // kind-2 blocks: sorted gathers touch <= 70% as many lines as entries
 }  // namespace
 
 #define fail ::hbs::set_error
@@ -259,8 +260,8 @@ extern "C" int hbs_scal_f64(hbs_ctx ctx, int n, double a, double *x)
 
 extern "C" int hbs_set_spmv_plan(int block_nnz, int serial_max_avg)
 {
-    if (block_nnz < 256 || block_nnz > kMaxBlockNnz || (block_nnz % 256) != 0)
-        return fail(HBS_ERR_ARG, "block_nnz must be a multiple of 256 in [256, %d]", kMaxBlockNnz);
+    if (block_nnz != 0 && (block_nnz < 256 || block_nnz > kMaxBlockNnz || (block_nnz % 256) != 0))
+        return fail(HBS_ERR_ARG, "block_nnz must be 0 or a multiple of 256 in [256, %d]", kMaxBlockNnz);
     if (serial_max_avg < 0) return fail(HBS_ERR_ARG, "serial_max_avg must be >= 0");
     g_block_nnz = block_nnz;
     g_serial_max_avg = serial_max_avg;
@@ -271,6 +272,13 @@ extern "C" int hbs_set_spmv_groups(int max_groups)
 {
     if (max_groups < 1 || max_groups > 65536) return fail(HBS_ERR_ARG, "max_groups must be in [1, 65536]");
     g_max_groups = max_groups;
+    return HBS_OK;
+}
+
+extern "C" int hbs_set_spmv_sort(int max_line_pct)
+{
+    if (max_line_pct < 0 || max_line_pct > 100) return fail(HBS_ERR_ARG, "max_line_pct must be in [0, 100]");
+    g_sort_pct = max_line_pct;
     return HBS_OK;
 }
 
@@ -296,6 +304,11 @@ extern "C" int hbs_csr_destroy(hbs_csr a)
     dfree(a->col);
     dfree(a->val);
     dfree(a->val32);
+    dfree(a->scol);
+    dfree(a->sval);
+    dfree(a->sval32);
+    dfree(a->sperm);
+    dfree(a->ssrc);
     dfree(a->plan.blocks);
     dfree(a->plan.aux);
     dfree(a->plan.chunks);
@@ -360,6 +373,9 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     }
     HostPlan hp;
     std::vector<SysRange> ranges;
+    // entries per row block: short rows (under 10 on average) gather little
+    // per row and do best with smaller blocks and more of them
+    const int block_nnz = g_block_nnz > 0 ? g_block_nnz : ((long long)nnz < 10LL * std::max(m, 1) ? 1024 : 2048);
     if (!sys_rows) {
         a->sys_rows = {0, m};
         nsys = 1;
@@ -370,7 +386,7 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     for (int q = 0; q < nsys; ++q) {
         const int r0 = a->sys_rows[(size_t)q], r1 = a->sys_rows[(size_t)q + 1];
         const int rb0 = (int)hp.blocks.size(), g0 = (int)hp.groups.size();
-        plan_rows(r0, r1 - r0, rowptr, g_block_nnz, g_serial_max_avg, q, hp);
+        plan_rows(r0, r1 - r0, rowptr, block_nnz, g_serial_max_avg, q, hp);
         const long long nz = (long long)rowptr[r1] - rowptr[r0];
         const int want = nsys == 1 ? g_max_groups
                                    : (int)std::max(1LL, nnz > 0 ? (long long)g_max_groups * nz / nnz : 1LL);
@@ -378,10 +394,31 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
         ranges.push_back(SysRange{g0, (int)hp.groups.size(), 0, 0});
     }
     a->sys_host = ranges;
+    // column-sorted gathers where they share cache lines
+    std::vector<int> src;
+    std::vector<unsigned short> spos;
+    if (plan_sorted(col, nnz, g_sort_pct, hp, src, spos) > 0) {
+        std::vector<int> sc((size_t)nnz);
+        std::vector<double> sv((size_t)nnz);
+        for (int j = 0; j < nnz; ++j) {
+            sc[(size_t)j] = col[src[(size_t)j]];
+            sv[(size_t)j] = val[src[(size_t)j]];
+        }
+        if ((rc = dalloc(&a->scol, nnz)) || (rc = dalloc(&a->sval, nnz)) || (rc = dalloc(&a->sperm, nnz)) ||
+            (rc = dalloc(&a->ssrc, nnz)) || (rc = upload(a->scol, sc)) || (rc = upload(a->sval, sv)) ||
+            (rc = upload(a->sperm, spos)) || (rc = upload(a->ssrc, src)))
+            return bail(rc);
+        if (a->val32) {
+            if ((rc = dalloc(&a->sval32, nnz))) return bail(rc);
+            if (launch_to_f32(nnz, a->sval, a->sval32, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
+                return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
+        }
+    }
     SpmvPlan &p = a->plan;
     p.nsys = nsys;
     a->stream_nt = g_stream_nt;
-    p.block_nnz = g_block_nnz;
+    p.block_nnz = block_nnz;
     p.nblk = (int)hp.blocks.size();
     p.ngroups = (int)hp.groups.size();
     p.nchunks = (int)hp.chunks.size();
@@ -474,8 +511,14 @@ extern "C" int hbs_csr_set_values(hbs_csr a, const double *val_dev)
     hipStream_t q = a->ctx->stream;
     if (a->nnz == 0) return HBS_OK;
     HIPCHK(hipMemcpyAsync(a->val, val_dev, sizeof(double) * (size_t)a->nnz, hipMemcpyDeviceToDevice, q));
-    if (a->val32) HIPCHK(launch_to_f32(a->nnz, a->val, a->val32, q));
     if (a->tr) HIPCHK(launch_scatter_f64(a->nnz, a->val, a->tr_perm, a->tr->val, q));
+    // the copies derived from val: FP32, column-sorted, of A and of A^T
+    for (hbs_csr t : {a, a->tr}) {
+        if (!t) continue;
+        if (t->val32) HIPCHK(launch_to_f32(t->nnz, t->val, t->val32, q));
+        if (t->sval) HIPCHK(launch_gather_f64(t->nnz, t->val, t->ssrc, t->sval, q));
+        if (t->sval32) HIPCHK(launch_to_f32(t->nnz, t->sval, t->sval32, q));
+    }
     return HBS_OK;
 }
 

@@ -3,6 +3,9 @@
 // the reference-shaped device-scalar kernels of hhp_dp_kernels.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
+#include "hbs_device.h"
 #include "hbs_internal.h"
 
 namespace hbs {
@@ -65,23 +68,32 @@ __global__ __launch_bounds__(kThreads) void k_grad_pattern(int m, const int *row
 }
 
 // Y[r, c] = sum_j val[j] * X[col[j], c], row-major with ld = k.  `tile`
-// lanes share one row (one RHS column each), in CSR order per column, so
-// every column equals the single-RHS order-exact sum.
+// lanes share one row (one RHS column each) and sum it in CSR order, so
+// every column equals the single-RHS order-exact sum.  Each XCD takes a
+// contiguous share of the rows, dealt in tiles round robin over its
+// workgroups (gridDim.x, a multiple of 8), so that neighbouring rows -- which
+// gather the same X rows -- are worked on together under one L2.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_spmm(int m, const int *__restrict__ rowptr,
                                                   const int *__restrict__ col,
                                                   const T *__restrict__ val, int k,
                                                   const T *__restrict__ X, T *__restrict__ Y, int tile)
 {
+    constexpr int RB = 2;   // rows per lane in flight together
     const int lane = threadIdx.x & (tile - 1);
-    const int rows_per_block = kThreads / tile;
-    const int r = blockIdx.x * rows_per_block + threadIdx.x / tile;
+    const int rstep = kThreads / tile;
     const int c = blockIdx.y * tile + lane;
-    if (r >= m || c >= k) return;
-    T acc = T(0);
-    const int e = rowptr[r + 1];
-    for (int j = rowptr[r]; j < e; ++j) acc += val[j] * X[(size_t)col[j] * k + c];
-    Y[(size_t)r * k + c] = acc;
+    if (c >= k) return;
+    const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, per = gridDim.x / kXcds;
+    const int x0 = (int)((long long)m * xcd / kXcds), x1 = (int)((long long)m * (xcd + 1) / kXcds);
+    for (int t0 = x0 + wi * RB * rstep; t0 < x1; t0 += per * RB * rstep) {
+        const int r = t0 + (int)threadIdx.x / tile;
+        T acc[RB];
+        rows_dot<T, RB>(rowptr, col, val, X + c, k, r, rstep, x1, acc);
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+            if (r + q * rstep < x1) Y[(size_t)(r + q * rstep) * k + c] = acc[q];
+    }
 }
 
 // ---- reference-shaped kernels (hhp_dp_kernels.cu:16-74) -------------------
@@ -188,23 +200,31 @@ hipError_t launch_grad_pattern(int m, const int *rowptr, const int *col, const d
     k_grad_pattern<<<grid_for((size_t)m * 64), kThreads, 0, s>>>(m, rowptr, col, lambda, x, g);
     return hipGetLastError();
 }
+// About 2048 workgroups in all, the row workgroups a multiple of 8.
+template <typename T>
+hipError_t spmm_dispatch(int m, const int *rowptr, const int *col, const T *val, int k, const T *X, T *Y,
+                         int tile, hipStream_t s)
+{
+    if (m <= 0 || k <= 0) return hipSuccess;
+    if (tile < 1 || tile > 64 || (tile & (tile - 1))) return hipErrorInvalidValue;
+    const int rstep = kThreads / tile;
+    const int ytiles = (k + tile - 1) / tile;
+    const long long passes = ((long long)m + 2LL * rstep - 1) / (2LL * rstep);
+    long long gx = std::min(std::max(1LL, 2048LL / ytiles), passes);
+    gx = std::max(8LL, (gx + 7) / 8 * 8);
+    dim3 grid((unsigned)gx, (unsigned)ytiles);
+    k_spmm<T><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, X, Y, tile);
+    return hipGetLastError();
+}
 hipError_t launch_spmm_f64(int m, const int *rowptr, const int *col, const double *val, int k,
                            const double *X, double *Y, int tile, hipStream_t s)
 {
-    if (m <= 0 || k <= 0) return hipSuccess;
-    const int rpb = kThreads / tile;
-    dim3 grid((m + rpb - 1) / rpb, (k + tile - 1) / tile);
-    k_spmm<double><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, X, Y, tile);
-    return hipGetLastError();
+    return spmm_dispatch<double>(m, rowptr, col, val, k, X, Y, tile, s);
 }
 hipError_t launch_spmm_f32(int m, const int *rowptr, const int *col, const float *val, int k,
                            const float *X, float *Y, int tile, hipStream_t s)
 {
-    if (m <= 0 || k <= 0) return hipSuccess;
-    const int rpb = kThreads / tile;
-    dim3 grid((m + rpb - 1) / rpb, (k + tile - 1) / tile);
-    k_spmm<float><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, X, Y, tile);
-    return hipGetLastError();
+    return spmm_dispatch<float>(m, rowptr, col, val, k, X, Y, tile, s);
 }
 
 hipError_t launch_dp_vecop_P(double *P, const double *V, const double *R, const double *omega,

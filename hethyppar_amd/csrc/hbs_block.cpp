@@ -28,7 +28,7 @@ struct BlockWork {
     SolverState *st = nullptr;    // device [kBlockCols]
     SolverState *hst = nullptr;   // pinned host [3][kBlockCols]
     double *part = nullptr;       // [kMaxDots][kBlockCols][ngroups], consumed before the next producer
-    int ngroups = 0, rows_per_group = 0;
+    int ngroups = 0;              // a multiple of 8: rows are dealt per XCD
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;   // (first column, columns, chunk)
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
@@ -70,12 +70,12 @@ int block_work(hbs_solver s, int k, BlockWork **out)
     w = new BlockWork();
     w->n = s->n;
     w->k = k;
-    // workgroups of contiguous rows, each pass of 4 waves covers 4 * (64 / kp) rows
+    // up to 2048 workgroups, a multiple of the 8 XCDs; a pass of 4 waves
+    // covers 4 * (64 / kp) rows
     const int kp = pow2_ceil(std::min(k, kBlockCols));
     const long long pass = (long long)kWaves * (64 / kp);
     const long long want = std::min<long long>(2048, (w->n + pass - 1) / pass);
-    w->rows_per_group = (int)((w->n + want - 1) / want);
-    w->ngroups = (w->n + w->rows_per_group - 1) / w->rows_per_group;
+    w->ngroups = (int)std::max<long long>(8, (want + 7) / 8 * 8);
     const size_t nk = (size_t)w->n * k;
     int rc = HBS_OK;
     for (double **p : {&w->X, &w->R, &w->R0, &w->P, &w->V, &w->S, &w->T})
@@ -103,7 +103,6 @@ BlockArgs group_args(hbs_solver s, BlockWork *w, int c0, int kc)
     a.kc = kc;
     a.kp = pow2_ceil(kc);
     a.ngroups = w->ngroups;
-    a.rows_per_group = w->rows_per_group;
     a.rowptr = s->A->rowptr;
     a.col = s->A->col;
     a.val = s->A->val;

@@ -35,6 +35,21 @@ __device__ __forceinline__ LaneMap lane_map(const BlockArgs &a)
     return m;
 }
 
+// Rows of this workgroup: its XCD's contiguous share of the rows, dealt in
+// tiles of `tile` rows round robin over the XCD's workgroups, so that the
+// workgroups of one XCD work on neighbouring rows at the same time and the x
+// rows they gather stay in that XCD's L2.  ngroups is a multiple of 8.
+struct RowSweep {
+    int first, end, stride;
+};
+__device__ __forceinline__ RowSweep row_sweep(int n, int ngroups, int tile)
+{
+    const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds;
+    const int per = ngroups / kXcds;
+    const int r0 = (int)((long long)n * xcd / kXcds), r1 = (int)((long long)n * (xcd + 1) / kXcds);
+    return RowSweep{r0 + wi * tile, r1, per * tile};
+}
+
 // Column sums of ND per-lane values -> part[d][c][p], fixed order: xor
 // butterfly over the lanes of one column inside a wave, then waves 0..3.
 template <int ND>
@@ -74,34 +89,40 @@ template <int EPI>
 __global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a)
 {
     constexpr int ND = EPI == BEPI_DOT1 ? 1 : 2;
-    const int p = xcd_remap(blockIdx.x, a.ngroups);
+    const int p = blockIdx.x;
     const LaneMap m = lane_map(a);
     const bool act = column_active(a, m, EPI == BEPI_DOT2);
     double v[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = 0.0;
     if (__syncthreads_or(act)) {
-        const int r0 = p * a.rows_per_group;
-        const int r1 = min(a.n, r0 + a.rows_per_group);
+        constexpr int RB = 2;   // rows per lane in flight together
+        const RowSweep sw = row_sweep(a.n, a.ngroups, RB * m.rstep);
         const double *__restrict__ x = a.x + m.c;
-        for (int r = r0 + m.ro; r < r1; r += m.rstep) {
+        for (int t0 = sw.first; t0 < sw.end; t0 += sw.stride) {
             if (!act) continue;
-            const int j1 = a.rowptr[r + 1];
-            double acc = 0.0;
-            for (int j = a.rowptr[r]; j < j1; ++j) acc += a.val[j] * x[(size_t)a.col[j] * a.ld];
-            const size_t i = (size_t)r * a.ld + m.c;
-            if constexpr (EPI == BEPI_DOT1) {
-                a.y[i] = acc;
-                v[0] += a.w[i] * acc;
-            } else if constexpr (EPI == BEPI_DOT2) {
-                a.y[i] = acc;
-                v[0] += acc * a.w[i];
-                v[1] += acc * acc;
-            } else {
-                const double o = a.w[i] - acc;
-                a.y[i] = o;
-                v[0] += o * o;
-                v[1] += o * o;
+            const int r = t0 + m.ro;
+            double accs[RB];
+            rows_dot<double, RB>(a.rowptr, a.col, a.val, x, a.ld, r, m.rstep, sw.end, accs);
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int rq = r + q * m.rstep;
+                if (rq >= sw.end) break;
+                const double acc = accs[q];
+                const size_t i = (size_t)rq * a.ld + m.c;
+                if constexpr (EPI == BEPI_DOT1) {
+                    a.y[i] = acc;
+                    v[0] += a.w[i] * acc;
+                } else if constexpr (EPI == BEPI_DOT2) {
+                    a.y[i] = acc;
+                    v[0] += acc * a.w[i];
+                    v[1] += acc * acc;
+                } else {
+                    const double o = a.w[i] - acc;
+                    a.y[i] = o;
+                    v[0] += o * o;
+                    v[1] += o * o;
+                }
             }
         }
     }
@@ -112,7 +133,7 @@ template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_bvec(BlockArgs a)
 {
     constexpr int ND = (KIND == BVEC_XR) ? 2 : ((KIND == BVEC_S || KIND == BVEC_DOT) ? 1 : 0);
-    const int p = xcd_remap(blockIdx.x, a.ngroups);
+    const int p = blockIdx.x;
     const LaneMap m = lane_map(a);
     const bool act = column_active(a, m, false);
     double v[ND > 0 ? ND : 1] = {};
@@ -126,10 +147,10 @@ __global__ __launch_bounds__(kThreads) void k_bvec(BlockArgs a)
             be = s.beta;
             sconv = s.s_conv;
         }
-        const int r0 = p * a.rows_per_group;
-        const int r1 = min(a.n, r0 + a.rows_per_group);
-        for (int r = r0 + m.ro; r < r1; r += m.rstep) {
-            if (!act) continue;
+        const RowSweep sw = row_sweep(a.n, a.ngroups, m.rstep);
+        for (int t0 = sw.first; t0 < sw.end; t0 += sw.stride) {
+            const int r = t0 + m.ro;
+            if (!act || r >= sw.end) continue;
             const size_t i = (size_t)r * a.ld + m.c;
             if constexpr (KIND == BVEC_P) {
                 a.P[i] = ((a.P[i] - a.V[i] * om) * be) + a.R[i];

@@ -125,6 +125,81 @@ __device__ __forceinline__ void reduce_parts(const double *part, int nparts, int
     block_sum<ND>(v, red);
 }
 
+// One row of A times one column of a row-major block: sum over the row's
+// entries in CSR order of val[j] * x[col[j] * ld] (x already offset to the
+// column).  Indices and values of 8 entries are loaded, then their 8
+// gathers, before the ordered adds, so the loads of a row overlap.
+template <typename T>
+__device__ __forceinline__ T row_dot(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                     const T *__restrict__ val, const T *__restrict__ x, int ld, int r)
+{
+    constexpr int kChunk = 8;
+    const int j0 = rowptr[r], j1 = rowptr[r + 1];
+    T acc = T(0);
+    for (int j = j0; j < j1; j += kChunk) {
+        int cc[kChunk];
+        T vv[kChunk], xx[kChunk];
+#pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+            const bool ok = j + i < j1;
+            cc[i] = ok ? col[j + i] : 0;
+            vv[i] = ok ? val[j + i] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < kChunk; ++i) xx[i] = (j + i < j1) ? x[(size_t)cc[i] * ld] : T(0);
+#pragma unroll
+        for (int i = 0; i < kChunk; ++i)
+            if (j + i < j1) acc += vv[i] * xx[i];
+    }
+    return acc;
+}
+
+// RB rows at once (rows r + q * step, q < RB, those below r1): the same
+// ordered sums as row_dot, with every row's loads of a chunk in flight
+// together.  Rows at or past r1 give 0.
+template <typename T, int RB>
+__device__ __forceinline__ void rows_dot(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                         const T *__restrict__ val, const T *__restrict__ x, int ld, int r,
+                                         int step, int r1, T (&acc)[RB])
+{
+    constexpr int kChunk = 8;
+    int jb[RB], je[RB], len = 0;
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        const int rq = r + q * step;
+        jb[q] = rq < r1 ? rowptr[rq] : 0;
+        je[q] = rq < r1 ? rowptr[rq + 1] : 0;
+        len = max(len, je[q] - jb[q]);
+        acc[q] = T(0);
+    }
+    for (int t = 0; t < len; t += kChunk) {
+        int cc[RB][kChunk];
+        T vv[RB][kChunk], xx[RB][kChunk];
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i) {
+                const int j = jb[q] + t + i;
+                const bool ok = j < je[q];
+                cc[q][i] = ok ? col[j] : 0;
+                vv[q][i] = ok ? val[j] : T(0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i)
+                xx[q][i] = (jb[q] + t + i < je[q]) ? x[(size_t)cc[q][i] * ld] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i)
+                if (jb[q] + t + i < je[q]) acc[q] += vv[q][i] * xx[q][i];
+        }
+    }
+}
+
 __device__ __forceinline__ void set_done(SolverState &s, int stop)
 {
     s.done = 1;

@@ -16,7 +16,7 @@ constexpr int kThreads = 256;          // workgroup size of every kernel (4 wave
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxBlockNnz = 4096;     // upper bound of nnz per SpMV row block
 constexpr int kMaxDots = 2;            // reductions fused into one pass
-constexpr int kMaxGroups = 2048;       // SpMV workgroups by default = partials a consumer sums
+constexpr int kMaxGroups = 4096;       // SpMV workgroups by default = partials a consumer sums
 
 // One SpMV row block: rows [r0, r1) with entries [j0, j1).
 struct alignas(16) RowBlock {
@@ -25,6 +25,9 @@ struct alignas(16) RowBlock {
 // kind 0: the block's entries fit in LDS; lanes = lanes per row in the
 // reduction (1 = the order-exact serial sum).  kind 1: one long row r0 whose
 // value is the sum, in order, of the chunk sums [first, first + nchunks).
+// kind 2: as kind 0, but the products are formed in column order (the
+// column-sorted copy scol/sval of the block's entries) and stored at their
+// CSR positions sperm, so a wave's x gathers fall on fewer cache lines.
 struct alignas(16) BlockAux {
     int kind, first, nchunks, lanes;
 };
@@ -112,6 +115,9 @@ struct SpmvArgs {
     T *y;
     const T *w;
     T alpha, beta;
+    const int *scol;            // kind-2 blocks: entries in column order,
+    const T *sval;              //   with their positions in the block
+    const unsigned short *sperm;
     const double *chunk_sums;   // sums of the long-row chunks (k_chunks, same x)
     double *part_out;           // [ngroups * kMaxDots] partials of the fused dots
     const double *part_in;      // partials of the previous vector kernel (FIN_S)
@@ -160,8 +166,7 @@ struct VecArgs {
 constexpr int kBlockCols = 64;
 struct BlockArgs {
     int n, ld, kc, kp;
-    int ngroups;                  // workgroups = partials per column
-    int rows_per_group;
+    int ngroups;                  // workgroups (a multiple of 8) = partials per column
     const int *rowptr, *col;
     const double *val;
     const double *x;              // SpMM input
@@ -202,6 +207,14 @@ struct HostPlan {
 // Row blocks of rows [row0, row0 + m) of system sys, appended to the plan.
 void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max_avg, int sys,
                HostPlan &plan);
+// Marks kind-0 blocks whose gathers, taken in column order, touch at most
+// max_line_pct percent as many 128-byte lines of an FP64 x per wave
+// instruction (64 consecutive entries) as in CSR order (kind 2), and fills
+// src[j] (the CSR entry at sorted position j) and pos[j] (its position in
+// the block); identity outside kind-2 blocks.  Returns the number of kind-2
+// blocks.
+int plan_sorted(const int *col, int nnz, int max_line_pct, HostPlan &plan, std::vector<int> &src,
+                std::vector<unsigned short> &pos);
 // Split the row blocks [rb0, rb1) of one system over ngroups workgroups.
 void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan);
 // Vector blocks of elements [e0, e0 + n) of one system.

@@ -65,6 +65,7 @@ __device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, Acc<
 template <typename T, int NPT>
 struct BlockRegs {
     T prod[NPT];
+    unsigned short pos[NPT];   // kind 2: CSR position of each product in the block
     int rs, re;
 };
 
@@ -74,8 +75,16 @@ __device__ __forceinline__ void load_block(const SpmvArgs<T> &a, const RowBlock 
 {
     if (ax.kind == 1) return;
     const int nz = rb.j1 - rb.j0;
-    const int *__restrict__ colp = a.col + rb.j0;
-    const T *__restrict__ valp = a.val + rb.j0;
+    const bool sorted = ax.kind == 2;
+    const int *__restrict__ colp = (sorted ? a.scol : a.col) + rb.j0;
+    const T *__restrict__ valp = (sorted ? a.sval : a.val) + rb.j0;
+    if (sorted) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const int k = (int)threadIdx.x + i * kThreads;
+            br.pos[i] = k < nz ? a.sperm[rb.j0 + k] : (unsigned short)0;
+        }
+    }
     int c[NPT];
     T v[NPT];
 #pragma unroll
@@ -118,10 +127,11 @@ __device__ __forceinline__ void sum_block(const SpmvArgs<T> &a, const RowBlock &
         return;
     }
     const int nz = rb.j1 - rb.j0;
+    const bool sorted = ax.kind == 2;
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
         const int k = (int)threadIdx.x + i * kThreads;
-        if (k < nz) lds[k] = br.prod[i];
+        if (k < nz) lds[sorted ? (int)br.pos[i] : k] = br.prod[i];
     }
     __syncthreads();
     const int nr = rb.r1 - rb.r0;

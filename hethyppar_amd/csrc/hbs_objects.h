@@ -89,6 +89,11 @@ struct hbs_csr_s {
     int *rowptr = nullptr, *col = nullptr;
     double *val = nullptr;
     float *val32 = nullptr;
+    int *scol = nullptr;              // column-sorted copy of kind-2 blocks (else CSR order)
+    double *sval = nullptr;
+    float *sval32 = nullptr;
+    unsigned short *sperm = nullptr;  // position in the block of each sorted entry
+    int *ssrc = nullptr;              // CSR entry of each sorted position (refreshes sval)
     hbs::SpmvPlan plan;
     hbs::SysRange *sys = nullptr;     // [nsys] {g0, g1, 0, 0}
     double *chunk_sums = nullptr;     // [nchunks]
@@ -154,6 +159,10 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
     s.rowptr = a->rowptr;
     s.col = a->col;
     s.val = val;
+    s.scol = a->scol;
+    if constexpr (sizeof(T) == 8) s.sval = reinterpret_cast<const T *>(a->sval);
+    else s.sval = reinterpret_cast<const T *>(a->sval32);
+    s.sperm = a->sperm;
     s.chunk_sums = a->chunk_sums;
     s.part_out = a->part;
     s.stream_nt = a->stream_nt;

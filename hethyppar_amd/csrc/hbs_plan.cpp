@@ -61,6 +61,51 @@ void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max
     }
 }
 
+int plan_sorted(const int *col, int nnz, int max_line_pct, HostPlan &plan, std::vector<int> &src,
+                std::vector<unsigned short> &pos)
+{
+    src.resize((size_t)nnz);
+    pos.resize((size_t)nnz);
+    for (int j = 0; j < nnz; ++j) src[(size_t)j] = j;
+    int nsorted = 0;
+    std::vector<int> idx;
+    for (size_t b = 0; b < plan.blocks.size(); ++b) {
+        const RowBlock &rb = plan.blocks[b];
+        BlockAux &ax = plan.aux[b];
+        const int nz = rb.j1 - rb.j0;
+        for (int k = 0; k < nz; ++k) pos[(size_t)rb.j0 + k] = (unsigned short)(ax.kind == 1 ? 0 : k);
+        if (ax.kind != 0 || nz < 2 || max_line_pct <= 0) continue;
+        idx.resize((size_t)nz);
+        for (int k = 0; k < nz; ++k) idx[(size_t)k] = k;
+        // distinct 128-byte lines of x per wave instruction (64 consecutive
+        // entries), in CSR order and in column order
+        auto wave_lines = [&](bool sorted_order) {
+            long long total = 0;
+            std::vector<int> ls;
+            for (int g = 0; g < nz; g += 64) {
+                ls.clear();
+                for (int k = g; k < std::min(nz, g + 64); ++k)
+                    ls.push_back(col[rb.j0 + (sorted_order ? idx[(size_t)k] : k)] >> 4);
+                std::sort(ls.begin(), ls.end());
+                total += std::unique(ls.begin(), ls.end()) - ls.begin();
+            }
+            return total;
+        };
+        const long long csr_lines = wave_lines(false);
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int a, int c) { return col[rb.j0 + a] < col[rb.j0 + c]; });
+        const long long sorted_lines = wave_lines(true);
+        if (sorted_lines * 100 > (long long)max_line_pct * csr_lines) continue;
+        ax.kind = 2;
+        ++nsorted;
+        for (int k = 0; k < nz; ++k) {
+            src[(size_t)rb.j0 + k] = rb.j0 + idx[(size_t)k];
+            pos[(size_t)rb.j0 + k] = (unsigned short)idx[(size_t)k];
+        }
+    }
+    return nsorted;
+}
+
 // Contiguous, equal-count split of row blocks [rb0, rb1) over ngroups
 // workgroups (row blocks carry about block_nnz entries each).
 void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan)

@@ -138,6 +138,7 @@ _PROTOS = {
     "hbs_set_spmv_groups": (_i, [_i]),
     "hbs_set_spmv_stream_nt": (_i, [_i]),
     "hbs_set_vec_blocks": (_i, [_i]),
+    "hbs_set_spmv_sort": (_i, [_i]),
     "hbs_csr_plan_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "hbs_shard_pack_host": (_i, [_vp, _vp, _dp]),
     "hbs_shard_spmv2": (_i, [_vp, _vp, _dp, _vp, _d, _d, _vp]),
@@ -683,7 +684,7 @@ class Shard:
             pass
 
 
-def set_spmv_plan(block_nnz=1024, serial_max_avg=64, max_groups=2048, stream_nt=0):
+def set_spmv_plan(block_nnz=0, serial_max_avg=64, max_groups=4096, stream_nt=0):
     """Process-wide plan knobs used by DeviceCSR objects created afterwards."""
     check(lib().hbs_set_spmv_plan(block_nnz, serial_max_avg))
     check(lib().hbs_set_spmv_groups(max_groups))
@@ -693,3 +694,9 @@ def set_spmv_plan(block_nnz=1024, serial_max_avg=64, max_groups=2048, stream_nt=
 def set_vec_blocks(target=512):
     """Process-wide: vector workgroups a Solver created afterwards aims at."""
     check(lib().hbs_set_vec_blocks(target))
+
+
+def set_spmv_sort(max_line_pct=70):
+    """Process-wide: column-sorted gathers for row blocks created afterwards
+    whose sorted entries touch at most this percentage of lines (0: off)."""
+    check(lib().hbs_set_spmv_sort(max_line_pct))

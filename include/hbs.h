@@ -300,13 +300,21 @@ int hbs_bench_copy(hbs_ctx ctx, const double *src_dev, double *dst_dev, size_t n
                    int reps, double *avg_ms);
 
 /* ---- tuning knobs (process-wide, for sweeps) -------------------------- */
-/* nnz per row block (<= 4096) and the average row length above which a
- * block uses several lanes per row instead of the order-exact serial sum. */
+/* nnz per row block (a multiple of 256, <= 4096; 0 = by the matrix: 1024
+ * under 10 entries per row on average, else 2048) and the average row
+ * length above which a block uses several lanes per row instead of the
+ * order-exact serial sum. */
 int hbs_set_spmv_plan(int block_nnz, int serial_max_avg);
 /* Cap on the SpMV workgroups of matrices created afterwards (1..65536,
- * default 2048); each takes a contiguous run of row blocks, and its dot
+ * default 4096); each takes a contiguous run of row blocks, and its dot
  * partial is one of those a consumer sums. */
 int hbs_set_spmv_groups(int max_groups);
+/* Row blocks of matrices created afterwards whose entries, taken in column
+ * order, touch at most max_line_pct percent as many 128-byte lines of x as
+ * they have entries form their products in column order (a column-sorted
+ * copy of their entries), so that a wave's gathers share cache lines; the
+ * row sums keep CSR order.  0 disables; default 70. */
+int hbs_set_spmv_sort(int max_line_pct);
 /* Vector workgroups a solver created afterwards aims at (1..65536, default
  * 512): each takes 512, 1024, 2048 or 4096 elements, the fewest that keep
  * the count at or below the target.  Fewer workgroups mean fewer partials

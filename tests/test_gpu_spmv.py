@@ -143,3 +143,25 @@ def test_dot_and_blas1(ctx):
     d = ctx.dot(dx, dy)
     assert abs(d - float(x @ y)) <= 1e-12 * float(np.abs(x) @ np.abs(y))
     assert ctx.dot(dx, dy) == d   # deterministic reduction order
+
+
+def test_set_values_updates_every_copy(ctx):
+    """New values reach the SpMV (including its column-sorted copy), the FP32
+    copy and the transpose."""
+    rp, c, v, xt, b = convdiff64()
+    n = rp.size - 1
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32 | hbs.HBS_CSR_TRANSPOSE)
+    v2 = v * 1.5 + 0.25
+    dv2 = ctx.to_device(v2)
+    A.set_values(dv2)
+    x = np.random.default_rng(5).uniform(-1, 1, n)
+    dx, dy = ctx.to_device(x), ctx.empty(n)
+    A.spmv(dx, dy)
+    assert np.array_equal(dy.download(), orc.spmv(orc.CSR(rp, c, v2), x))
+    A.spmv_t(dx, dy)
+    tp, tc, tv = orc.csr_to_csc(orc.CSR(rp, c, v2))
+    assert np.array_equal(dy.download(), orc.spmv(orc.CSR(tp, tc, tv), x))
+    dx32, dy32 = ctx.to_device(x.astype(np.float32)), ctx.empty(n, np.float32)
+    A.spmv(dx32, dy32)
+    ref = orc.spmv(orc.CSR(rp, c, v2), x.astype(np.float32).astype(np.float64))
+    assert np.allclose(dy32.download(), ref, rtol=1e-5, atol=1e-5)
