@@ -36,9 +36,10 @@ struct EpiDots {
     static constexpr int n = (EPI == EPI_DOT2 || EPI == EPI_RESID2) ? 2 : (EPI == EPI_AXPBY ? 0 : 1);
 };
 
-// Row epilogue: write y[r] and feed the fused dots.
+// Row epilogue: write y[r] and feed the fused dots (wr = w[r], loaded by the
+// caller for the epilogues that read w).
 template <typename T, int EPI>
-__device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, Acc<T> &d0, Acc<T> &d1)
+__device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, T wr, Acc<T> &d0, Acc<T> &d1)
 {
     if constexpr (EPI == EPI_AXPBY) {
         T out = a.alpha * acc;
@@ -46,17 +47,23 @@ __device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, Acc<
         a.y[r] = out;
     } else if constexpr (EPI == EPI_DOT1) {
         a.y[r] = acc;
-        d0.add(a.w[r], acc);
+        d0.add(wr, acc);
     } else if constexpr (EPI == EPI_DOT2) {
         a.y[r] = acc;
-        d0.add(acc, a.w[r]);
+        d0.add(acc, wr);
         d1.add(acc, acc);
     } else {
-        const T out = a.w[r] - acc;
+        const T out = wr - acc;
         a.y[r] = out;
         d0.add(out, out);
         if constexpr (EPI == EPI_RESID2) d1.add(out, out);
     }
+}
+
+template <int EPI>
+__device__ __forceinline__ constexpr bool reads_w()
+{
+    return EPI != EPI_AXPBY;
 }
 
 // The entries of one row block a thread holds in registers (entries
@@ -122,34 +129,40 @@ __device__ __forceinline__ void sum_block(const SpmvArgs<T> &a, const RowBlock &
         if (threadIdx.x == 0) {
             T total = T(0);
             for (int c = 0; c < ax.nchunks; ++c) total += (T)a.chunk_sums[ax.first + c];
-            row_out<T, EPI>(a, rb.r0, total, d0, d1);
+            row_out<T, EPI>(a, rb.r0, total, reads_w<EPI>() ? a.w[rb.r0] : T(0), d0, d1);
         }
         return;
     }
     const int nz = rb.j1 - rb.j0;
     const bool sorted = ax.kind == 2;
+    const int nr = rb.r1 - rb.r0;
+    const int lanes = ax.lanes;
+    // the epilogue operand of this thread's first row, loaded before the barrier
+    const int first = lanes <= 1 ? (int)threadIdx.x : (int)threadIdx.x / lanes;
+    T w0 = T(0);
+    if (reads_w<EPI>() && first < nr) w0 = a.w[rb.r0 + first];
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
         const int k = (int)threadIdx.x + i * kThreads;
         if (k < nz) lds[sorted ? (int)br.pos[i] : k] = br.prod[i];
     }
     __syncthreads();
-    const int nr = rb.r1 - rb.r0;
-    const int lanes = ax.lanes;
     if (lanes <= 1) {
         for (int t = threadIdx.x; t < nr; t += kThreads) {
             const int r = rb.r0 + t;
             int s, e;
+            T wr = w0;
             if (t == (int)threadIdx.x) {
                 s = br.rs - rb.j0;
                 e = br.re - rb.j0;
             } else {
                 s = a.rowptr[r] - rb.j0;
                 e = a.rowptr[r + 1] - rb.j0;
+                if (reads_w<EPI>()) wr = a.w[r];
             }
             T acc = T(0);
             for (int q = s; q < e; ++q) acc += lds[q];
-            row_out<T, EPI>(a, r, acc, d0, d1);
+            row_out<T, EPI>(a, r, acc, wr, d0, d1);
         }
     } else {
         const int gi = threadIdx.x / lanes, l = threadIdx.x & (lanes - 1);
@@ -157,6 +170,7 @@ __device__ __forceinline__ void sum_block(const SpmvArgs<T> &a, const RowBlock &
         for (int t0 = 0; t0 < nr; t0 += ng) {
             const int t = t0 + gi;
             T acc = T(0);
+            T wr = w0;
             const int r = rb.r0 + t;
             if (t < nr) {
                 int s, e;
@@ -166,11 +180,12 @@ __device__ __forceinline__ void sum_block(const SpmvArgs<T> &a, const RowBlock &
                 } else {
                     s = a.rowptr[r] - rb.j0;
                     e = a.rowptr[r + 1] - rb.j0;
+                    if (reads_w<EPI>() && l == 0) wr = a.w[r];
                 }
                 for (int q = s + l; q < e; q += lanes) acc += lds[q];
             }
             for (int o = lanes >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-            if (t < nr && l == 0) row_out<T, EPI>(a, r, acc, d0, d1);
+            if (t < nr && l == 0) row_out<T, EPI>(a, r, acc, wr, d0, d1);
         }
     }
     __syncthreads();   // LDS is reused by the next row block
@@ -447,23 +462,26 @@ __global__ __launch_bounds__(kThreads) void k_vec_XR(VecArgs<T> a)
 }
 
 // Apply a pending FIN_XR in place, one workgroup per system (end of a chunk
-// of iterations, so the host sees the state of the last iteration).
+// of iterations, so the host sees the state of the last iteration), and
+// store the state in the host-mapped mirror the host polls (if any).
 __global__ __launch_bounds__(kThreads) void k_apply_xr(SolverState *st, const SysRange *sys, const double *part,
-                                                      int nvb)
+                                                      int nvb, SolverState *mirror)
 {
     __shared__ double red[kMaxDots * kWaves];
     __shared__ SolverState sh;
     const int q = blockIdx.x;
     if (threadIdx.x == 0) sh = st[q];
     __syncthreads();
-    if (sh.done || !sh.pending) return;
-    const SysRange sr = sys[q];
-    double v[2];
-    reduce_parts<2>(part, nvb, sr.v0, sr.v1, v, red);
-    if (threadIdx.x == 0) {
-        apply_fin(sh, FIN_XR, v[0], v[1]);
-        st[q] = sh;
+    if (!sh.done && sh.pending) {
+        const SysRange sr = sys[q];
+        double v[2];
+        reduce_parts<2>(part, nvb, sr.v0, sr.v1, v, red);
+        if (threadIdx.x == 0) {
+            apply_fin(sh, FIN_XR, v[0], v[1]);
+            st[q] = sh;
+        }
     }
+    if (mirror && threadIdx.x == 0) mirror[q] = sh;
 }
 
 __global__ __launch_bounds__(kThreads) void k_finalize(SolverState *st, const SysRange *sys, const double *part,
@@ -634,10 +652,10 @@ hipError_t launch_vec_S(const VecArgs<float> &a, hipStream_t s) { return k_vec_S
 hipError_t launch_vec_XR(const VecArgs<float> &a, hipStream_t s) { return k_vec_XR_dispatch<float>(a, s); }
 
 hipError_t launch_apply_xr(SolverState *st, const SysRange *sys, int nsys, const double *part, int nvb,
-                           hipStream_t s)
+                           SolverState *mirror, hipStream_t s)
 {
     if (nsys <= 0) return hipSuccess;
-    k_apply_xr<<<nsys, kThreads, 0, s>>>(st, sys, part, nvb);
+    k_apply_xr<<<nsys, kThreads, 0, s>>>(st, sys, part, nvb, mirror);
     return hipGetLastError();
 }
 

@@ -124,6 +124,7 @@ struct hbs_solver_s {
           *T32 = nullptr;
     hbs::SolverState *st = nullptr;    // device [kNumSlots][nsys]
     hbs::SolverState *hst = nullptr;   // pinned host [3][nsys]: two poll buffers, one read buffer
+    hbs::SolverState *hst_dev = nullptr;   // device view of hst (host-mapped, coherent)
     hbs::VecBlock *vblocks = nullptr;
     hbs::SysRange *sys = nullptr;      // [nsys] {g0, g1, v0, v1}
     int nvb = 0;
@@ -135,8 +136,8 @@ struct hbs_solver_s {
     double *partE = nullptr;      // residual SpMV: ||b - Ax||^2
     double *partF = nullptr;      // dots: (b, b)
     double *chunk_sums = nullptr; // long-row chunk sums
-    hipGraphExec_t graph[2] = {nullptr, nullptr};   // [0] FP64, [1] FP32
-    int graph_chunk[2] = {0, 0};
+    static constexpr int kGraphLens = 33;             // chunks of 1..32 iterations
+    hipGraphExec_t graph[2][kGraphLens][2] = {};       // [FP64, FP32][chunk length][poll slot]
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hbs_solver_s *adj = nullptr;

@@ -48,7 +48,9 @@ extern "C" int hbs_solver_destroy(hbs_solver s)
     dfree(s->vblocks);
     dfree(s->sys);
     for (int i = 0; i < 2; ++i)
-        if (s->graph[i]) (void)hipGraphExecDestroy(s->graph[i]);
+        for (auto &bylen : s->graph[i])
+            for (hipGraphExec_t g : bylen)
+                if (g) (void)hipGraphExecDestroy(g);
     for (hipEvent_t e : {s->ev[0], s->ev[1], s->e0, s->e1})
         if (e) (void)hipEventDestroy(e);
     if (s->adj) hbs_solver_destroy(s->adj);
@@ -78,7 +80,10 @@ extern "C" int hbs_solver_create(hbs_csr a, hbs_solver *out)
     for (double **p : {&s->X, &s->R, &s->R0, &s->P, &s->V, &s->S, &s->T})
         if ((rc = dalloc(p, s->n))) return bail(rc);
     if ((rc = dalloc(&s->st, (size_t)kNumSlots * s->nsys))) return bail(rc);
-    if (hipHostMalloc((void **)&s->hst, 3 * sizeof(SolverState) * (size_t)s->nsys) != hipSuccess)
+    // host-mapped and coherent: k_apply_xr writes the polled states straight into it
+    if (hipHostMalloc((void **)&s->hst, 3 * sizeof(SolverState) * (size_t)s->nsys,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&s->hst_dev, s->hst, 0) != hipSuccess)
         return bail(fail(HBS_ERR_ALLOC, "pinned host allocation failed"));
     // vector workgroups system by system; one block size for the launch
     std::vector<VecBlock> vb;
@@ -196,45 +201,47 @@ hipError_t launch_iteration(hbs_solver s, const Work<T> &w)
     return launch_vec_XR(va, q);
 }
 
+// k iterations, then the state the host polls (FIN_XR of the chunk's last
+// iteration applied), mirrored into poll slot `slot` of the host buffer.
 template <typename T>
-hipError_t launch_chunk(hbs_solver s, const Work<T> &w, int k)
+hipError_t launch_chunk(hbs_solver s, const Work<T> &w, int k, int slot)
 {
     hipError_t e = hipSuccess;
     for (int i = 0; i < k && e == hipSuccess; ++i) e = launch_iteration<T>(s, w);
     if (e != hipSuccess) return e;
-    // the state the host polls: FIN_XR of the chunk's last iteration applied
-    return launch_apply_xr(s->slot(kSlotA), s->sys, s->nsys, s->partD, s->nvb, s->A->ctx->stream);
+    return launch_apply_xr(s->slot(kSlotA), s->sys, s->nsys, s->partD, s->nvb,
+                           s->hst_dev + (size_t)slot * s->nsys, s->A->ctx->stream);
 }
 
 template <typename T>
-int run_chunk(hbs_solver s, const Work<T> &w, int k, bool use_graph)
+int run_chunk(hbs_solver s, const Work<T> &w, int k, int slot, bool use_graph)
 {
     hipStream_t q = s->A->ctx->stream;
     const int gi = sizeof(T) == 8 ? 0 : 1;
     if (!use_graph) {
-        HIPCHK(launch_chunk<T>(s, w, k));
+        HIPCHK(launch_chunk<T>(s, w, k, slot));
         return HBS_OK;
     }
-    if (!s->graph[gi] || s->graph_chunk[gi] != k) {
-        if (s->graph[gi]) {
-            HIPCHK(hipGraphExecDestroy(s->graph[gi]));
-            s->graph[gi] = nullptr;
-        }
+    // one graph per chunk length and poll slot, kept; longer chunks are captured each time
+    const bool keep = k < hbs_solver_s::kGraphLens;
+    hipGraphExec_t scratch_graph = nullptr;
+    hipGraphExec_t &ge = keep ? s->graph[gi][k][slot] : scratch_graph;
+    if (!ge) {
         hipGraph_t gr = nullptr;
         HIPCHK(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
-        hipError_t le = launch_chunk<T>(s, w, k);
+        hipError_t le = launch_chunk<T>(s, w, k, slot);
         hipError_t ce = hipStreamEndCapture(q, &gr);
         if (le != hipSuccess) return fail(HBS_ERR_HIP, "launch during capture failed: %s", hipGetErrorString(le));
         if (ce != hipSuccess) return fail(HBS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
-        hipError_t ie = hipGraphInstantiate(&s->graph[gi], gr, nullptr, nullptr, 0);
+        hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
         if (ie != hipSuccess) {
-            s->graph[gi] = nullptr;
+            ge = nullptr;
             return fail(HBS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
         }
-        s->graph_chunk[gi] = k;
     }
-    HIPCHK(hipGraphLaunch(s->graph[gi], q));
+    HIPCHK(hipGraphLaunch(ge, q));
+    if (!keep) HIPCHK(hipGraphExecDestroy(ge));
     return HBS_OK;
 }
 
@@ -246,30 +253,57 @@ bool all_done(const SolverState *h, int nsys)
 }
 
 // Queue iteration chunks until the device states say every system is done.
-// The states of chunk c are read after chunk c + 1 is queued, so the GPU
-// never idles on the poll; iterations queued after convergence return at once.
+// One chunk is kept queued ahead of the one whose state the host reads, so
+// the GPU never idles on the poll -- until the residual history of a single
+// system predicts convergence within the chunks already queued; then the
+// host waits instead of queueing iterations that would only return at once,
+// and sizes the next chunk to the iterations still predicted.
 template <typename T>
 int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use_graph)
 {
     hipStream_t q = s->A->ctx->stream;
-    const int k = check_every > 0 ? check_every : 4;
-    const size_t bytes = sizeof(SolverState) * (size_t)s->nsys;
+    const int k = std::max(1, check_every > 0 ? check_every : 4);
     long issued = 0;
-    int c = 0, pending = -1;
+    int slot = 0, nfly = 0;
+    int fly_slot[2] = {0, 0}, fly_len[2] = {0, 0};   // queued chunks, oldest first
+    auto queue = [&](int len) -> int {
+        CHK(run_chunk<T>(s, w, len, slot, use_graph));
+        HIPCHK(hipEventRecord(s->ev[slot], q));
+        fly_slot[nfly] = slot;
+        fly_len[nfly] = len;
+        ++nfly;
+        slot ^= 1;
+        issued += len;
+        return HBS_OK;
+    };
+    int next = (int)std::min<long>(k, maxit);
+    bool ahead = true;
+    double rr_prev = -1.0;
+    int it_prev = 0;
+    CHK(queue(next));
     for (;;) {
-        CHK(run_chunk<T>(s, w, k, use_graph));
-        issued += k;
-        HIPCHK(hipMemcpyAsync(s->hst + (size_t)(c & 1) * s->nsys, s->slot(kSlotA), bytes, hipMemcpyDeviceToHost, q));
-        HIPCHK(hipEventRecord(s->ev[c & 1], q));
-        if (pending >= 0) {
-            HIPCHK(hipEventSynchronize(s->ev[pending & 1]));
-            if (all_done(s->hst + (size_t)(pending & 1) * s->nsys, s->nsys)) break;
+        if (ahead && nfly < 2 && issued < maxit) CHK(queue((int)std::min<long>(next, maxit - issued)));
+        HIPCHK(hipEventSynchronize(s->ev[fly_slot[0]]));
+        const SolverState *h = s->hst + (size_t)fly_slot[0] * s->nsys;
+        const bool done = all_done(h, s->nsys);
+        fly_slot[0] = fly_slot[1];
+        fly_len[0] = fly_len[1];
+        --nfly;
+        if (done || (issued >= maxit && nfly == 0)) break;
+        ahead = true;
+        next = k;
+        if (s->nsys == 1 && !h->fixed && h->iter > 0 && h->rr > 0.0 && h->thr > 0.0) {
+            if (rr_prev > 0.0 && h->iter > it_prev && h->rr < rr_prev) {
+                const double per = std::log(h->rr / rr_prev) / (h->iter - it_prev);
+                const double need = std::log(h->thr * h->thr / h->rr) / per;   // iterations after h
+                const double left = need - (nfly ? fly_len[0] : 0);
+                if (left <= 0.5) ahead = false;
+                else next = (int)std::min<double>(k, std::max(1.0, std::ceil(left)));
+            }
+            rr_prev = h->rr;
+            it_prev = h->iter;
         }
-        pending = c++;
-        if (issued >= maxit) {
-            HIPCHK(hipEventSynchronize(s->ev[pending & 1]));
-            break;
-        }
+        if (nfly == 0 && issued < maxit) CHK(queue((int)std::min<long>(next, maxit - issued)));
     }
     return HBS_OK;
 }
