@@ -65,7 +65,26 @@ def build(verbose=False):
     _run(["g++", "-shared", "-o", tmp] + objs +
          [f"-L{ROCM}/lib", "-lamdhip64", "-fopenmp", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, LIB)
+    build_cli()
     return LIB
+
+
+CLI = ["hbs_bicgstab_gpu", "hbs_spmv_gpu"]
+BIN = os.path.join(HERE, "bin")
+
+
+def build_cli():
+    """Command-line programs with the reference's options and output
+    (hethyppar_amd/bin/), linked against libhbs.so."""
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+    for name in CLI:
+        exe = os.path.join(BIN, name)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+              os.path.join(HERE, "cli", name + ".cpp"), "-o", exe,
+              f"-L{HERE}", "-lhbs", "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{ROCM}/lib"])
+        out.append(exe)
+    return out
 
 
 def build_oracle():

@@ -254,6 +254,9 @@ hipError_t launch_ir_update(const VecArgs<float> &a, double *x64, hipStream_t s)
 // d0 = (x, y) partials into part_out.
 hipError_t launch_dot(const VecArgs<double> &a, const double *x, const double *y, hipStream_t s);
 hipError_t launch_state_init(SolverState *st, int nstates, double tol, int maxit, int fixed, hipStream_t s);
+// X = x and the initial states of nstates slots, in one launch.
+hipError_t launch_solve_start(const double *x, double *X, int n, SolverState *st, int nstates, double tol,
+                              int maxit, int fixed, hipStream_t s);
 
 // BLAS-1 helpers on raw device pointers.
 hipError_t launch_axpy_f64(int n, double a, const double *x, double *y, hipStream_t s);

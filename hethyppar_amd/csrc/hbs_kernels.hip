@@ -25,6 +25,8 @@
 //               updated scalars in the other state slot.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "hbs_device.h"
 #include "hbs_internal.h"
 
@@ -563,6 +565,25 @@ __global__ __launch_bounds__(kThreads) void k_dot(VecArgs<double> a, const doubl
     write_parts<1, double>(a, d0, d1, red);
 }
 
+// Start of a solve: X = x (grid-stride) and, in workgroup 0, the states.
+__global__ __launch_bounds__(kThreads) void k_solve_start(const double *x, double *X, int n, SolverState *st,
+                                                         int nstates, double tol, int maxit, int fixed)
+{
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) X[i] = x[i];
+    if (blockIdx.x != 0) return;
+    for (int q = threadIdx.x; q < nstates; q += blockDim.x) {
+        SolverState z{};
+        z.rho = 1.0;
+        z.alpha = 1.0;
+        z.omega = 1.0;
+        z.tol = tol;
+        z.maxit = maxit;
+        z.fixed = fixed;
+        z.iter = -1;
+        st[q] = z;
+    }
+}
+
 __global__ void k_state_init(SolverState *st, int nstates, double tol, int maxit, int fixed)
 {
     for (int q = threadIdx.x; q < nstates; q += blockDim.x) {
@@ -692,6 +713,14 @@ hipError_t launch_dot(const VecArgs<double> &a, const double *x, const double *y
 {
     if (a.nvb <= 0) return hipSuccess;
     k_dot<<<a.nvb, kThreads, 0, s>>>(a, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_solve_start(const double *x, double *X, int n, SolverState *st, int nstates, double tol,
+                              int maxit, int fixed, hipStream_t s)
+{
+    const int grid = std::max(1, std::min(2048, (n + kThreads * 4 - 1) / (kThreads * 4)));
+    k_solve_start<<<grid, kThreads, 0, s>>>(x, X, n, st, nstates, tol, maxit, fixed);
     return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -257,16 +258,21 @@ bool all_done(const SolverState *h, int nsys)
 // the GPU never idles on the poll -- until the residual history of a single
 // system predicts convergence within the chunks already queued; then the
 // host waits instead of queueing iterations that would only return at once,
-// and sizes the next chunk to the iterations still predicted.
+// sizes the next chunk to the iterations still predicted, and queues `tail`
+// (the work that follows convergence) right behind the last chunk.
+// *tail_queued says whether that tail still follows the last chunk.
 template <typename T>
-int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use_graph)
+int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use_graph,
+            const std::function<int()> &tail = {}, bool *tail_queued = nullptr)
 {
     hipStream_t q = s->A->ctx->stream;
     const int k = std::max(1, check_every > 0 ? check_every : 4);
     long issued = 0;
     int slot = 0, nfly = 0;
     int fly_slot[2] = {0, 0}, fly_len[2] = {0, 0};   // queued chunks, oldest first
+    bool spec = false;   // tail queued after the last queued chunk
     auto queue = [&](int len) -> int {
+        spec = false;
         CHK(run_chunk<T>(s, w, len, slot, use_graph));
         HIPCHK(hipEventRecord(s->ev[slot], q));
         fly_slot[nfly] = slot;
@@ -304,7 +310,12 @@ int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use
             it_prev = h->iter;
         }
         if (nfly == 0 && issued < maxit) CHK(queue((int)std::min<long>(next, maxit - issued)));
+        if (!ahead && tail && !spec) {
+            CHK(tail());
+            spec = true;
+        }
     }
+    if (tail_queued) *tail_queued = spec;
     return HBS_OK;
 }
 
@@ -377,8 +388,7 @@ int solve_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts &o,
     Work<double> w = work64(s);
 
     HIPCHK(hipEventRecord(s->e0, q));
-    HIPCHK(hipMemcpyAsync(s->X, x, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
-    HIPCHK(launch_state_init(s->st, 2 * s->nsys, o.tol, o.maxit, o.fixed_iters ? 1 : 0, q));
+    HIPCHK(launch_solve_start(x, s->X, n, s->st, 2 * s->nsys, o.tol, o.maxit, o.fixed_iters ? 1 : 0, q));
     // R = B - A X ; R0 = R ; P = V = 0 ; (R, R0) = ||R||^2 pending in slot A
     // (bicgstab_cpu.c:176-188)
     {
@@ -393,12 +403,19 @@ int solve_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts &o,
         va.part_out = s->partD;
         HIPCHK(launch_init_vectors(va, q));
     }
-    CHK(iterate<double>(s, w, std::max(o.maxit, 1), o.check_every, o.use_graph != 0));
     // final relative residual sqrt(||B - AX||^2 / ||B||^2) (bicgstab_cpu.c:251-255)
-    HIPCHK(residual(s, s->X, b, s->V, kSlotA));
-    HIPCHK(bnorm2(s, b, kSlotA));
-    HIPCHK(hipMemcpyAsync(x, s->X, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
-    HIPCHK(hipEventRecord(s->e1, q));
+    // into S (dead between iterations), ||B||^2, x = X; queued by iterate
+    // behind the chunk predicted to be the last
+    auto tail = [&]() -> int {
+        HIPCHK(residual(s, s->X, b, s->S, kSlotA));
+        HIPCHK(bnorm2(s, b, kSlotA));
+        HIPCHK(hipMemcpyAsync(x, s->X, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
+        HIPCHK(hipEventRecord(s->e1, q));
+        return HBS_OK;
+    };
+    bool tail_done = false;
+    CHK(iterate<double>(s, w, std::max(o.maxit, 1), o.check_every, o.use_graph != 0, tail, &tail_done));
+    if (!tail_done) CHK(tail());
     std::vector<SolverState> hs((size_t)s->nsys);
     CHK(read_states(s, kSlotA, hs.data()));
     float ms = 0.f;
