@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 #include "hbs.h"
 #include "hbs_err.h"
 
@@ -53,6 +55,93 @@ T *hmalloc(size_t n)
     return static_cast<T *>(std::malloc(sizeof(T) * (n ? n : 1)));
 }
 
+// Parse one line "r c v" of [p, e) into (r-1, c-1, v); false unless the line
+// holds exactly those three numbers within range.
+bool parse_line(const char *p, const char *e, long M, long N, int &r0, int &c0, double &v)
+{
+    char *q = nullptr;
+    const long r = std::strtol(p, &q, 10);
+    if (q == p || q > e) return false;
+    p = q;
+    const long c = std::strtol(p, &q, 10);
+    if (q == p || q > e) return false;
+    p = q;
+    v = std::strtod(p, &q);
+    if (q == p || q > e) return false;
+    for (p = q; p < e; ++p)
+        if (!std::isspace((unsigned char)*p)) return false;
+    if (r < 1 || r > M || c < 1 || c > N) return false;
+    r0 = (int)(r - 1);
+    c0 = (int)(c - 1);
+    return true;
+}
+
+// The triplet section [p, end) parsed by several threads, each a run of whole
+// lines, the runs concatenated in file order -- the same values (strtol /
+// strtod, as the reference's fscanf) in the same order as one pass.  Taken
+// only for large files laid out one entry per line (blank lines allowed);
+// false otherwise, and the caller parses the token stream in one pass.
+bool parse_triplets_parallel(const char *p, const char *end, long M, long N, long NZ, std::vector<int> &ri,
+                             std::vector<int> &ci, std::vector<double> &vv)
+{
+    const size_t len = (size_t)(end - p);
+    int T = omp_get_max_threads();
+    if (T > 64) T = 64;
+    if (T < 2 || NZ < 100000 || len < ((size_t)1 << 20)) return false;
+    std::vector<const char *> cut((size_t)T + 1);
+    cut[0] = p;
+    cut[(size_t)T] = end;
+    for (int t = 1; t < T; ++t) {
+        const char *c = p + len * (size_t)t / (size_t)T;
+        const char *nl = static_cast<const char *>(std::memchr(c, '\n', (size_t)(end - c)));
+        cut[(size_t)t] = nl ? nl + 1 : end;
+        if (cut[(size_t)t] < cut[(size_t)t - 1]) cut[(size_t)t] = cut[(size_t)t - 1];
+    }
+    std::vector<std::vector<int>> lr((size_t)T), lc((size_t)T);
+    std::vector<std::vector<double>> lv((size_t)T);
+    std::vector<int> ok((size_t)T, 1);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; ++t) {
+        const char *a = cut[(size_t)t], *b = cut[(size_t)t + 1];
+        const size_t guess = (size_t)(b - a) / 16 + 16;
+        lr[(size_t)t].reserve(guess);
+        lc[(size_t)t].reserve(guess);
+        lv[(size_t)t].reserve(guess);
+        while (a < b) {
+            const char *nl = static_cast<const char *>(std::memchr(a, '\n', (size_t)(b - a)));
+            const char *e = nl ? nl : b;
+            const char *f = a;
+            while (f < e && std::isspace((unsigned char)*f)) ++f;
+            if (f < e) {
+                int r0, c0;
+                double v;
+                if (!parse_line(f, e, M, N, r0, c0, v)) {
+                    ok[(size_t)t] = 0;
+                    break;
+                }
+                lr[(size_t)t].push_back(r0);
+                lc[(size_t)t].push_back(c0);
+                lv[(size_t)t].push_back(v);
+            }
+            a = nl ? nl + 1 : b;
+        }
+    }
+    size_t total = 0;
+    for (int t = 0; t < T; ++t) {
+        if (!ok[(size_t)t]) return false;
+        total += lr[(size_t)t].size();
+    }
+    if (total < (size_t)NZ) return false;
+    size_t k = 0;
+    for (int t = 0; t < T && k < (size_t)NZ; ++t)
+        for (size_t i = 0; i < lr[(size_t)t].size() && k < (size_t)NZ; ++i, ++k) {
+            ri[k] = lr[(size_t)t][i];
+            ci[k] = lc[(size_t)t][i];
+            vv[k] = lv[(size_t)t][i];
+        }
+    return true;
+}
+
 }  // namespace
 
 extern "C" void hbs_host_free(void *p) { std::free(p); }
@@ -63,7 +152,9 @@ extern "C" void hbs_host_free(void *p) { std::free(p); }
 // (:36-41), and pattern too since its entries carry no value; comment lines
 // skipped before the size line (mmio.c:189-218); 1-based triplets; within a
 // row the entries keep FILE order and duplicates are kept.  Here the file is
-// read in one piece and the triplets bucketed by a stable counting sort.
+// read in one piece, large line-structured files parsed by several threads
+// (runs of whole lines, concatenated in file order), and the triplets
+// bucketed by a stable counting sort.
 extern "C" int hbs_mm_read(const char *fname, int *m, int *n, int *nnz, int **rowptr, int **col,
                            double **val)
 {
@@ -111,20 +202,23 @@ extern "C" int hbs_mm_read(const char *fname, int *m, int *n, int *nnz, int **ro
         return lfail(HBS_ERR_FORMAT);
     std::vector<int> ri((size_t)NZ), ci((size_t)NZ);
     std::vector<double> vv((size_t)NZ);
-    for (long k = 0; k < NZ; ++k) {
-        long r = std::strtol(p, &q, 10);
-        if (q == p) return lfail(HBS_ERR_FORMAT);
-        p = q;
-        long c = std::strtol(p, &q, 10);
-        if (q == p) return lfail(HBS_ERR_FORMAT);
-        p = q;
-        double v = std::strtod(p, &q);
-        if (q == p) return lfail(HBS_ERR_FORMAT);
-        p = q;
-        if (r < 1 || r > M || c < 1 || c > N) return lfail(HBS_ERR_FORMAT);
-        ri[(size_t)k] = (int)(r - 1);
-        ci[(size_t)k] = (int)(c - 1);
-        vv[(size_t)k] = v;
+    if (!parse_triplets_parallel(p, end, M, N, NZ, ri, ci, vv)) {
+        // one token stream, whatever the line structure (the reference's fscanf loop)
+        for (long k = 0; k < NZ; ++k) {
+            long r = std::strtol(p, &q, 10);
+            if (q == p) return lfail(HBS_ERR_FORMAT);
+            p = q;
+            long c = std::strtol(p, &q, 10);
+            if (q == p) return lfail(HBS_ERR_FORMAT);
+            p = q;
+            double v = std::strtod(p, &q);
+            if (q == p) return lfail(HBS_ERR_FORMAT);
+            p = q;
+            if (r < 1 || r > M || c < 1 || c > N) return lfail(HBS_ERR_FORMAT);
+            ri[(size_t)k] = (int)(r - 1);
+            ci[(size_t)k] = (int)(c - 1);
+            vv[(size_t)k] = v;
+        }
     }
     int *rp = hmalloc<int>((size_t)M + 1);
     int *cc = hmalloc<int>((size_t)NZ);

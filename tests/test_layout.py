@@ -168,3 +168,29 @@ def test_gen_fem_properties():
         vals = v[rp[i]:rp[i + 1]]
         d = int(np.flatnonzero(cols == i)[0])
         assert np.isclose(vals[d], 1.1 * np.abs(np.delete(vals, d)).sum(), rtol=1e-14)
+
+
+def test_mm_read_large_file_parallel_parse_bit_exact(tmp_path):
+    """A file large enough for the multi-threaded parse (runs of whole lines):
+    same CSR, bit for bit, as the oracle's one-pass reader -- shuffled rows,
+    duplicates, blank lines, tabs, exponents."""
+    rng = np.random.default_rng(21)
+    m, n, k = 20000, 15000, 400000
+    r = rng.integers(1, m + 1, k)
+    c = rng.integers(1, n + 1, k)
+    r[:50] = r[50:100]
+    c[:50] = c[50:100]          # duplicates
+    v = rng.standard_normal(k) * 10.0 ** rng.integers(-12, 12, k)
+    path = tmp_path / "big.mtx"
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n% comment\n")
+        f.write(f"{m} {n} {k}\n")
+        for i in range(k):
+            sep = "\t" if i % 7 == 0 else " "
+            f.write(f"{r[i]}{sep}{c[i]} {float(v[i])!r}\n")
+            if i % 50000 == 0:
+                f.write("\n")
+    m1, n1, rp1, c1, v1 = hbs.mm_read(str(path))
+    m2, n2, rp2, c2, v2 = orc.mm_read(str(path))
+    assert (m1, n1) == (m2, n2)
+    assert np.array_equal(rp1, rp2) and np.array_equal(c1, c2) and np.array_equal(v1, v2)
