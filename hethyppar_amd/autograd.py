@@ -7,13 +7,13 @@
 Forward: x = A(values)^-1 b with hbs_bicgstab_f64 (x0 = 0).  Backward: the
 adjoint solve A^T lambda = dL/dx (hbs_adjoint_f64, BiCGStab on the stored
 transpose), then dL/db = lambda and dL/dA_ij = -lambda_i x_j on A's pattern
-in CSR order.  torch only supplies device memory and the stream: the
-library's context runs on torch's current stream of the device, so its
-launches are ordered with torch's.  The HIP kernels are the product; with
-libhbs.so missing the import fails.
+in CSR order.  torch only supplies device memory: the library's context
+has its own stream, so forward and backward first wait for torch's current
+stream (the inputs, and the output buffers torch allocated and cleared,
+are then complete), and every library call returns only once its results
+are.  The HIP kernels are the product; with libhbs.so missing the import
+fails.
 """
-import ctypes as C
-
 import torch
 
 from . import hbs
@@ -47,8 +47,7 @@ class SparseSystem:
         hbs.lib()
         _check_one_runtime()
         self.device = torch.device("cuda", device)
-        stream = torch.cuda.current_stream(self.device).cuda_stream
-        self.ctx = hbs.Context(device, stream=C.c_void_p(stream))
+        self.ctx = hbs.Context(device)
         v = values.detach().cpu().numpy() if torch.is_tensor(values) else np.asarray(values)
         self.A = hbs.DeviceCSR(self.ctx, rowptr, col, v, flags=hbs.HBS_CSR_TRANSPOSE)
         self.S = hbs.Solver(self.A)
@@ -64,6 +63,9 @@ class SparseSystem:
     def __call__(self, values, b):
         return SparseSolve.apply(values, b, self)
 
+    def wait_for_torch(self):
+        torch.cuda.current_stream(self.device).synchronize()
+
     def close(self):
         self.ctx.close()
 
@@ -73,8 +75,9 @@ class SparseSolve(torch.autograd.Function):
     def forward(ctx, values, b, system):
         values = system._check(values, system.nnz, "values")
         b = system._check(b, system.n, "b")
-        system.A.set_values(hbs.DevPtr(values))
         x = torch.zeros_like(b)
+        system.wait_for_torch()
+        system.A.set_values(hbs.DevPtr(values))
         system.last_info = system.S.bicgstab(hbs.DevPtr(b), hbs.DevPtr(x), system.opts)
         ctx.save_for_backward(values, x)
         ctx.system = system
@@ -85,9 +88,10 @@ class SparseSolve(torch.autograd.Function):
         values, x = ctx.saved_tensors
         s = ctx.system
         gx = s._check(gx, s.n, "grad of x")
-        s.A.set_values(hbs.DevPtr(values))     # the forward's values, whatever ran since
         lam = torch.empty_like(x)
         gv = torch.empty_like(values) if ctx.needs_input_grad[0] else None
+        s.wait_for_torch()
+        s.A.set_values(hbs.DevPtr(values))     # the forward's values, whatever ran since
         s.S.adjoint(hbs.DevPtr(gx), hbs.DevPtr(x), hbs.DevPtr(lam), hbs.DevPtr(gv) if gv is not None else None,
                     s.opts)
         return gv, (lam if ctx.needs_input_grad[1] else None), None

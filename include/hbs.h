@@ -43,7 +43,9 @@ const char *hbs_version(void);
 
 /* ---- context: one device + one stream -------------------------------- */
 typedef struct hbs_ctx_s *hbs_ctx;
-/* stream: a hipStream_t, or NULL to create a private non-blocking stream. */
+/* stream: a hipStream_t, or NULL to create a private non-blocking stream
+ * (so the legacy default stream, whose handle is NULL, cannot be chosen:
+ * callers on it synchronise it before handing buffers to the library). */
 int hbs_ctx_create(int device, void *stream, hbs_ctx *out);
 int hbs_ctx_destroy(hbs_ctx ctx);
 int hbs_ctx_sync(hbs_ctx ctx);

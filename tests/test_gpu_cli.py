@@ -20,7 +20,7 @@ BIN = os.path.join(ROOT, "hethyppar_amd", "bin")
 def _metrics(text):
     out = {}
     for line in text.splitlines():
-        m = re.match(r"^\s*([A-Za-z_]+)\s*:\s*(\S+)", line)
+        m = re.match(r"^\s*([A-Za-z_][A-Za-z0-9_]*)\s*:\s*(\S+)", line)
         if m:
             out[m.group(1)] = m.group(2)
     return out
