@@ -343,6 +343,19 @@ int validate_csr(int m, int n, int nnz, const int *rowptr, const int *col, const
 // Systems: sys_rows[nsys + 1] row offsets of independent diagonal blocks
 // (nullptr: one system of all m rows).  Each system gets its own row blocks
 // and a share of the SpMV workgroups proportional to its entries.
+// Entries per row block (measured on MI355X).  Short rows (under 10 on
+// average) gather little per row and do best with blocks of 1024 entries
+// and many of them.  Longer rows gain from big blocks: column-sorted gathers
+// share more x lines and the 512-thread workgroups keep their loads in
+// flight; the block is sized for about 1024 of them, between 2048 and 8192
+// entries, so a small matrix still fills the GPU.
+int default_block_nnz(int m, int nnz)
+{
+    if ((long long)nnz < 10LL * std::max(m, 1)) return 1024;
+    const long long per = (long long)nnz / 1024 / 256 * 256;
+    return (int)std::min<long long>(8192, std::max<long long>(2048, per));
+}
+
 int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int *col, const double *val, int flags,
                int nsys, const int *sys_rows, hbs_csr *out)
 {
@@ -373,9 +386,7 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     }
     HostPlan hp;
     std::vector<SysRange> ranges;
-    // entries per row block: short rows (under 10 on average) gather little
-    // per row and do best with smaller blocks and more of them
-    const int block_nnz = g_block_nnz > 0 ? g_block_nnz : ((long long)nnz < 10LL * std::max(m, 1) ? 1024 : 2048);
+    const int block_nnz = g_block_nnz > 0 ? g_block_nnz : default_block_nnz(m, nnz);
     if (!sys_rows) {
         a->sys_rows = {0, m};
         nsys = 1;
@@ -415,6 +426,8 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
                 return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
         }
     }
+    if (prepare_spmv(block_nnz) != hipSuccess)
+        return bail(fail(HBS_ERR_HIP, "SpMV LDS limit for %d-entry row blocks not granted", block_nnz));
     SpmvPlan &p = a->plan;
     p.nsys = nsys;
     a->stream_nt = g_stream_nt;

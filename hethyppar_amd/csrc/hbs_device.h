@@ -57,9 +57,10 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
-// Workgroup sum of ND values; every thread gets the totals.  Fixed order:
-// xor butterfly inside each wave, then waves 0..3 in sequence.
-template <int ND>
+// Workgroup sum of ND values over NW waves; every thread gets the totals.
+// Fixed order: xor butterfly inside each wave, then waves 0..NW-1 in
+// sequence.  red holds ND * NW doubles.
+template <int ND, int NW = kWaves>
 __device__ __forceinline__ void block_sum(double (&v)[ND], double *red)
 {
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -67,14 +68,14 @@ __device__ __forceinline__ void block_sum(double (&v)[ND], double *red)
     for (int d = 0; d < ND; ++d) v[d] = wave_sum(v[d]);
     if (l == 0) {
 #pragma unroll
-        for (int d = 0; d < ND; ++d) red[d * kWaves + w] = v[d];
+        for (int d = 0; d < ND; ++d) red[d * NW + w] = v[d];
     }
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-        double t = red[d * kWaves];
+        double t = red[d * NW];
 #pragma unroll
-        for (int k = 1; k < kWaves; ++k) t += red[d * kWaves + k];
+        for (int k = 1; k < NW; ++k) t += red[d * NW + k];
         v[d] = t;
     }
     __syncthreads();
@@ -95,18 +96,20 @@ __device__ __forceinline__ T block_sum1(T v, T *red)
 }
 
 // Sum of partials [p0, p1) (stride nparts between the ND rows), fixed order:
-// thread t adds p0 + t, p0 + t + 256, ... in turn, then block_sum.  The loads
-// of a span of kSpan partials per thread are all issued before the first
-// add, so a consumer's prologue costs one L2 round trip per span rather than
-// one per partial.
+// thread t < 256 adds p0 + t, p0 + t + 256, ... in turn, then block_sum.  The
+// loads of a span of kSpan partials per thread are all issued before the
+// first add, so a consumer's prologue costs one L2 round trip per span rather
+// than one per partial.  In workgroups of NW > 4 waves the threads past 256
+// add nothing, so every workgroup size derives bit-identical sums.
 constexpr int kSpan = 16;
-template <int ND>
+template <int ND, int NW = kWaves>
 __device__ __forceinline__ void reduce_parts(const double *part, int nparts, int p0, int p1, double (&v)[ND],
                                              double *red)
 {
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = 0.0;
-    for (int b = p0 + (int)threadIdx.x; b < p1; b += kSpan * kThreads) {
+    const int b0 = threadIdx.x < kThreads ? p0 + (int)threadIdx.x : p1;
+    for (int b = b0; b < p1; b += kSpan * kThreads) {
         double buf[ND][kSpan];
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
@@ -122,7 +125,7 @@ __device__ __forceinline__ void reduce_parts(const double *part, int nparts, int
             for (int k = 0; k < kSpan; ++k) v[d] += buf[d][k];
         }
     }
-    block_sum<ND>(v, red);
+    block_sum<ND, NW>(v, red);
 }
 
 // One row of A times one column of a row-major block: sum over the row's

@@ -14,7 +14,12 @@ namespace hbs {
 
 constexpr int kThreads = 256;          // workgroup size of every kernel (4 wave64)
 constexpr int kWaves = kThreads / 64;
-constexpr int kMaxBlockNnz = 4096;     // upper bound of nnz per SpMV row block
+constexpr int kWideThreads = 512;      // SpMV workgroup size for row blocks above 2048 entries
+constexpr int kMaxBlockNnz = 16384;    // upper bound of nnz per SpMV row block (128 KiB of FP64 LDS)
+constexpr int kLongRow = 1024;         // rows longer than this are summed in chunks (k_chunks)
+
+// Threads of the SpMV workgroups that sum row blocks of block_nnz entries.
+constexpr int spmv_threads(int block_nnz) { return block_nnz > 2048 ? kWideThreads : kThreads; }
 constexpr int kMaxDots = 2;            // reductions fused into one pass
 constexpr int kMaxGroups = 4096;       // SpMV workgroups by default = partials a consumer sums
 
@@ -224,6 +229,9 @@ extern int g_vec_target;      // vector workgroups a plan aims at (hbs_set_vec_b
 int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 
 // ---- launchers (hbs_kernels.hip) ----
+// Raise the dynamic LDS limit of the SpMV kernels for row blocks of block_nnz
+// entries (once per plan, outside any stream capture).
+hipError_t prepare_spmv(int block_nnz);
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, hipStream_t s);
 hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, hipStream_t s);
 hipError_t launch_chunks(const ChunkArgs<double> &a, int block_nnz, hipStream_t s);

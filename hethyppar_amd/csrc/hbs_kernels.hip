@@ -4,14 +4,18 @@
 // MFMA): coalesced 16-byte streams, wave64 reductions, XCD-aware placement.
 //
 //  * k_spmv     CSR-stream SpMV.  A workgroup owns a contiguous run of row
-//               blocks whose entries fit in LDS: phase 1 streams col/val
-//               coalesced and stores one product per entry in LDS; phase 2
-//               sums each row.  Short-row blocks sum serially in CSR order
+//               blocks whose entries fit in LDS: passes stream col/val
+//               coalesced and store one product per entry in LDS (in column
+//               order where that shares x lines, at the CSR position); then
+//               each row is summed.  The passes are software-pipelined: the
+//               next pass's streams load while this pass's gathers, stores
+//               and row sums run.  Short-row blocks sum serially in CSR order
 //               (bit-identical to the reference CSR_spmxv_seq,
-//               hhp_cpu.c:445-455); longer rows use 2..64 lanes; rows longer
-//               than a block are cut into chunks summed by k_chunks and
-//               combined here in chunk order.  Epilogues fuse the dot
-//               products BiCGStab needs after each SpMV.
+//               hhp_cpu.c:445-455); blocks of longer rows use 2..64 lanes per
+//               row; rows longer than 1024 entries (or than a block) are cut
+//               into chunks summed by k_chunks and combined here in chunk
+//               order.  Epilogues fuse the dot products BiCGStab needs after
+//               each SpMV.
 //  * k_vec_*    the three BiCGStab vector updates (reference k_vecop_P/S/XR,
 //               hhp_dp_kernels.cu:51-74), each fused with the reductions of
 //               bicgstab_cpu.c:203,219,229,242 that follow it.  Each thread
@@ -68,140 +72,139 @@ __device__ __forceinline__ constexpr bool reads_w()
     return EPI != EPI_AXPBY;
 }
 
-// The entries of one row block a thread holds in registers (entries
-// t, t + 256, ..., t + 256 * (NPT - 1) of the block), plus the row pointers
-// of its first row, all loaded before they are needed.
+// One pass over a row block: the entries base + t + TH * i (i < NPT) of the
+// block, as loaded column indices, values and (kind 2) CSR positions.  A wave
+// instruction covers 64 consecutive entries.
 template <typename T, int NPT>
-struct BlockRegs {
-    T prod[NPT];
-    unsigned short pos[NPT];   // kind 2: CSR position of each product in the block
-    int rs, re;
-};
-
-template <typename T, int NPT>
-__device__ __forceinline__ void load_block(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax,
-                                           BlockRegs<T, NPT> &br)
-{
-    if (ax.kind == 1) return;
-    const int nz = rb.j1 - rb.j0;
-    const bool sorted = ax.kind == 2;
-    const int *__restrict__ colp = (sorted ? a.scol : a.col) + rb.j0;
-    const T *__restrict__ valp = (sorted ? a.sval : a.val) + rb.j0;
-    if (sorted) {
-#pragma unroll
-        for (int i = 0; i < NPT; ++i) {
-            const int k = (int)threadIdx.x + i * kThreads;
-            br.pos[i] = k < nz ? a.sperm[rb.j0 + k] : (unsigned short)0;
-        }
-    }
+struct PassLoad {
     int c[NPT];
     T v[NPT];
+    unsigned short pos[NPT];
+};
+
+// Issue the stream loads of one pass (entries at or past the block's end
+// load nothing).
+template <typename T, int NPT, int TH>
+__device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, int base,
+                                           PassLoad<T, NPT> &L)
+{
+    const bool sorted = ax.kind == 2;
+    const int lim = rb.j1 - rb.j0 - base;
+    const int *__restrict__ colp = (sorted ? a.scol : a.col) + rb.j0 + base;
+    const T *__restrict__ valp = (sorted ? a.sval : a.val) + rb.j0 + base;
+    if (sorted) {
+        const unsigned short *__restrict__ pp = a.sperm + rb.j0 + base;
 #pragma unroll
-    for (int i = 0; i < NPT; ++i) {
-        const int k = (int)threadIdx.x + i * kThreads;
-        if (a.stream_nt) {
-            c[i] = k < nz ? __builtin_nontemporal_load(colp + k) : 0;
-            v[i] = k < nz ? __builtin_nontemporal_load(valp + k) : T(0);
-        } else {
-            c[i] = k < nz ? colp[k] : 0;
-            v[i] = k < nz ? valp[k] : T(0);
+        for (int i = 0; i < NPT; ++i) {
+            const int k = (int)threadIdx.x + i * TH;
+            L.pos[i] = k < lim ? pp[k] : (unsigned short)0;
         }
     }
-    const int first = ax.lanes <= 1 ? (int)threadIdx.x : (int)threadIdx.x / ax.lanes;
-    if (first < rb.r1 - rb.r0) {
-        br.rs = a.rowptr[rb.r0 + first];
-        br.re = a.rowptr[rb.r0 + first + 1];
-    }
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
-        const int k = (int)threadIdx.x + i * kThreads;
-        br.prod[i] = k < nz ? v[i] * a.x[c[i]] : T(0);
+        const int k = (int)threadIdx.x + i * TH;
+        if (a.stream_nt) {
+            L.c[i] = k < lim ? __builtin_nontemporal_load(colp + k) : 0;
+            L.v[i] = k < lim ? __builtin_nontemporal_load(valp + k) : T(0);
+        } else {
+            L.c[i] = k < lim ? colp[k] : 0;
+            L.v[i] = k < lim ? valp[k] : T(0);
+        }
     }
 }
 
-// Sums the rows of a loaded block and feeds the epilogue.  Serial blocks add
-// each row's products in CSR order from 0 (the reference's order); others
-// use `lanes` lanes per row and an xor butterfly.
-template <typename T, int EPI, int NPT>
-__device__ __forceinline__ void sum_block(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax,
-                                          const BlockRegs<T, NPT> &br, T *lds, Acc<T> &d0, Acc<T> &d1)
+// Rows per row block: at most kRowsPerThread per thread, so the serial row
+// sums take one round of loads.  The block's row pointers, relative to its
+// first entry, are staged in LDS as 16-bit offsets (block_nnz <= 16384).
+constexpr int kRowsPerThread = 4;
+template <int TH>
+constexpr int rowptr_lds_bytes()
 {
-    if (ax.kind == 1) {
-        // long row: ordered sum of its chunk sums
-        if (threadIdx.x == 0) {
-            T total = T(0);
-            for (int c = 0; c < ax.nchunks; ++c) total += (T)a.chunk_sums[ax.first + c];
-            row_out<T, EPI>(a, rb.r0, total, reads_w<EPI>() ? a.w[rb.r0] : T(0), d0, d1);
-        }
-        return;
-    }
-    const int nz = rb.j1 - rb.j0;
-    const bool sorted = ax.kind == 2;
+    return ((kRowsPerThread * TH + 1) * 2 + 15) / 16 * 16;
+}
+
+// Epilogue operand of this thread's first row of a block, loaded long before
+// the row sums need it.
+template <typename T, int EPI>
+__device__ __forceinline__ T issue_w0(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax)
+{
+    const int first = ax.lanes <= 1 ? (int)threadIdx.x : (int)threadIdx.x / ax.lanes;
+    if (reads_w<EPI>() && first < rb.r1 - rb.r0) return a.w[rb.r0 + first];
+    return T(0);
+}
+
+// Sums the rows of a block whose products (lds) and row offsets (lrp) are in
+// LDS and feeds the epilogue.  Serial blocks add each row's products in CSR
+// order from 0 (the reference's order): thread t sums rows t, t + TH, ...,
+// with their epilogue operands loaded together (the first one, w0, was
+// prefetched).  Other blocks use `lanes` lanes per row and an xor butterfly.
+template <typename T, int EPI, int TH>
+__device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, T w0,
+                                         const T *lds, const unsigned short *lrp, Acc<T> &d0, Acc<T> &d1)
+{
     const int nr = rb.r1 - rb.r0;
     const int lanes = ax.lanes;
-    // the epilogue operand of this thread's first row, loaded before the barrier
-    const int first = lanes <= 1 ? (int)threadIdx.x : (int)threadIdx.x / lanes;
-    T w0 = T(0);
-    if (reads_w<EPI>() && first < nr) w0 = a.w[rb.r0 + first];
-#pragma unroll
-    for (int i = 0; i < NPT; ++i) {
-        const int k = (int)threadIdx.x + i * kThreads;
-        if (k < nz) lds[sorted ? (int)br.pos[i] : k] = br.prod[i];
-    }
-    __syncthreads();
     if (lanes <= 1) {
-        for (int t = threadIdx.x; t < nr; t += kThreads) {
-            const int r = rb.r0 + t;
-            int s, e;
-            T wr = w0;
-            if (t == (int)threadIdx.x) {
-                s = br.rs - rb.j0;
-                e = br.re - rb.j0;
-            } else {
-                s = a.rowptr[r] - rb.j0;
-                e = a.rowptr[r + 1] - rb.j0;
-                if (reads_w<EPI>()) wr = a.w[r];
+        T wr[kRowsPerThread];
+#pragma unroll
+        for (int q = 0; q < kRowsPerThread; ++q) {
+            const int t = (int)threadIdx.x + q * TH;
+            wr[q] = q == 0 ? w0 : (reads_w<EPI>() && t < nr ? a.w[rb.r0 + t] : T(0));
+        }
+#pragma unroll
+        for (int q = 0; q < kRowsPerThread; ++q) {
+            const int t = (int)threadIdx.x + q * TH;
+            if (t < nr) {
+                const int e = lrp[t + 1];
+                T acc = T(0);
+                for (int j = lrp[t]; j < e; ++j) acc += lds[j];
+                row_out<T, EPI>(a, rb.r0 + t, acc, wr[q], d0, d1);
             }
-            T acc = T(0);
-            for (int q = s; q < e; ++q) acc += lds[q];
-            row_out<T, EPI>(a, r, acc, wr, d0, d1);
         }
     } else {
         const int gi = threadIdx.x / lanes, l = threadIdx.x & (lanes - 1);
-        const int ng = kThreads / lanes;
+        const int ng = TH / lanes;
         for (int t0 = 0; t0 < nr; t0 += ng) {
             const int t = t0 + gi;
             T acc = T(0);
             T wr = w0;
-            const int r = rb.r0 + t;
             if (t < nr) {
-                int s, e;
-                if (t0 == 0) {
-                    s = br.rs - rb.j0;
-                    e = br.re - rb.j0;
-                } else {
-                    s = a.rowptr[r] - rb.j0;
-                    e = a.rowptr[r + 1] - rb.j0;
-                    if (reads_w<EPI>() && l == 0) wr = a.w[r];
-                }
-                for (int q = s + l; q < e; q += lanes) acc += lds[q];
+                if (t0 > 0 && reads_w<EPI>() && l == 0) wr = a.w[rb.r0 + t];
+                const int e = lrp[t + 1];
+                for (int q = lrp[t] + l; q < e; q += lanes) acc += lds[q];
             }
             for (int o = lanes >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-            if (t < nr && l == 0) row_out<T, EPI>(a, r, acc, wr, d0, d1);
+            if (t < nr && l == 0) row_out<T, EPI>(a, rb.r0 + t, acc, wr, d0, d1);
         }
     }
-    __syncthreads();   // LDS is reused by the next row block
 }
 
-// NPT = block_nnz / 256 entries per thread.  A finished system returns
-// before any load; otherwise the first row block's loads are issued before
-// the rest of the state prologue, so they overlap it.
-template <typename T, int EPI, int NPT>
-__global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
+// Position of a workgroup in its run of row blocks: block b (rb, ax), entry
+// offset base of the pass in flight.
+struct Cursor {
+    int b, base;
+    RowBlock rb;
+    BlockAux ax;
+};
+
+// TH threads, NPT entries per thread per pass: a row block of up to
+// block_nnz entries takes ceil(block_nnz / (NPT * TH)) passes, each storing
+// its products in LDS, before the row sums.  The passes of a workgroup's
+// blocks form one stream, software-pipelined over two register sets: the
+// stream loads of pass i + 1 are issued right after the x gathers of pass i,
+// so they are in flight while pass i's products are stored and, after a
+// block's last pass, while its rows are summed.  A finished system returns
+// before any load; otherwise the first pass's loads are issued before the
+// rest of the state prologue, so they overlap it.
+template <typename T, int EPI, int NPT, int TH>
+__global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_spmv(SpmvArgs<T> a)
 {
+    constexpr int NW = TH / 64;
+    constexpr int PASS = NPT * TH;
     extern __shared__ __align__(16) unsigned char smem[];
-    T *lds = reinterpret_cast<T *>(smem);
-    __shared__ double red[kMaxDots * kWaves];
+    unsigned short *lrp = reinterpret_cast<unsigned short *>(smem);
+    T *lds = reinterpret_cast<T *>(smem + rowptr_lds_bytes<TH>());
+    __shared__ double red[kMaxDots * NW];
     __shared__ SolverState sh;
     constexpr int ND = EpiDots<EPI>::n;
 
@@ -211,11 +214,10 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
         if (a.st_out && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_out[g.sys] = a.st_in[g.sys];
         return;
     }
-    int b = g.rb0;
-    RowBlock rb = a.blocks[b];
-    BlockAux ax = a.aux[b];
-    BlockRegs<T, NPT> br;
-    load_block<T, NPT>(a, rb, ax, br);
+    Cursor cur{g.rb0, 0, a.blocks[g.rb0], a.aux[g.rb0]};
+    PassLoad<T, NPT> LA, LB;
+    if (cur.ax.kind != 1) issue_pass<T, NPT, TH>(a, cur.rb, cur.ax, 0, LA);
+    T w0 = issue_w0<T, EPI>(a, cur.rb, cur.ax);
 
     if (a.st_in) {
         if (threadIdx.x == 0) sh = a.st_in[g.sys];
@@ -223,7 +225,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
         if (a.fin_in == FIN_S) {
             const SysRange sr = a.sys[g.sys];
             double v[1];
-            reduce_parts<1>(a.part_in, a.nparts_in, sr.v0, sr.v1, v, red);
+            reduce_parts<1, NW>(a.part_in, a.nparts_in, sr.v0, sr.v1, v, red);
             if (threadIdx.x == 0) {
                 apply_fin(sh, FIN_S, v[0], 0.0);
                 if (p == sr.g0) a.st_out[g.sys] = sh;
@@ -235,19 +237,91 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvArgs<T> a)
     if (a.st_mark && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_mark[g.sys].pending = 1;
 
     Acc<T> d0, d1;
+    // One pass of the stream: the current pass's loads are in `in`; the next
+    // pass's go to `out`.  Returns false after the workgroup's last block.
+    auto step = [&](const PassLoad<T, NPT> &in, PassLoad<T, NPT> &out) -> bool {
+        const int nz = cur.rb.j1 - cur.rb.j0;
+        const bool gather = cur.ax.kind != 1;
+        const bool last = !gather || cur.base + PASS >= nz;
+        T xv[NPT];
+        if (gather) {
+            const int lim = nz - cur.base;
+#pragma unroll
+            for (int i = 0; i < NPT; ++i) {
+                const int k = (int)threadIdx.x + i * TH;
+                // 32-bit byte offset from the uniform base (n * sizeof(T) < 2^32)
+                xv[i] = k < lim ? *reinterpret_cast<const T *>(reinterpret_cast<const char *>(a.x) +
+                                                              (unsigned)in.c[i] * (unsigned)sizeof(T))
+                                : T(0);
+            }
+        }
+        // first pass of a block: its row pointers, staged in LDS below
+        int rpv[kRowsPerThread];
+        const bool stage = gather && cur.base == 0;
+        if (stage) {
+#pragma unroll
+            for (int q = 0; q < kRowsPerThread; ++q) {
+                const int t = (int)threadIdx.x + q * TH;
+                rpv[q] = t <= cur.rb.r1 - cur.rb.r0 ? a.rowptr[cur.rb.r0 + t] : 0;
+            }
+        }
+        Cursor nxt = cur;
+        if (last) {
+            nxt.b = cur.b + 1;
+            nxt.base = 0;
+            if (nxt.b < g.rb1) {
+                nxt.rb = a.blocks[nxt.b];
+                nxt.ax = a.aux[nxt.b];
+            }
+        } else {
+            nxt.base = cur.base + PASS;
+        }
+        const bool more = nxt.b < g.rb1;
+        if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH>(a, nxt.rb, nxt.ax, nxt.base, out);
+        if (gather) {
+            const int lim = nz - cur.base;
+            const bool sorted = cur.ax.kind == 2;
+#pragma unroll
+            for (int i = 0; i < NPT; ++i) {
+                const int k = (int)threadIdx.x + i * TH;
+                if (k < lim) lds[sorted ? (int)in.pos[i] : cur.base + k] = in.v[i] * xv[i];
+            }
+        }
+        if (stage) {
+#pragma unroll
+            for (int q = 0; q < kRowsPerThread; ++q) {
+                const int t = (int)threadIdx.x + q * TH;
+                if (t <= cur.rb.r1 - cur.rb.r0) lrp[t] = (unsigned short)(rpv[q] - cur.rb.j0);
+            }
+        }
+        if (last) {
+            if (!gather) {
+                // long row: ordered sum of its chunk sums
+                if (threadIdx.x == 0) {
+                    T total = T(0);
+                    for (int c = 0; c < cur.ax.nchunks; ++c) total += (T)a.chunk_sums[cur.ax.first + c];
+                    row_out<T, EPI>(a, cur.rb.r0, total, reads_w<EPI>() ? a.w[cur.rb.r0] : T(0), d0, d1);
+                }
+            } else {
+                __syncthreads();
+                sum_rows<T, EPI, TH>(a, cur.rb, cur.ax, w0, lds, lrp, d0, d1);
+                __syncthreads();   // LDS is reused by the next row block
+            }
+            if (more) w0 = issue_w0<T, EPI>(a, nxt.rb, nxt.ax);
+        }
+        cur = nxt;
+        return more;
+    };
     for (;;) {
-        sum_block<T, EPI, NPT>(a, rb, ax, br, lds, d0, d1);
-        if (++b >= g.rb1) break;
-        rb = a.blocks[b];
-        ax = a.aux[b];
-        load_block<T, NPT>(a, rb, ax, br);
+        if (!step(LA, LB)) break;
+        if (!step(LB, LA)) break;
     }
 
     if constexpr (ND > 0) {
         double v[ND];
         v[0] = d0.get();
         if constexpr (ND > 1) v[1] = d1.get();
-        block_sum<ND>(v, red);
+        block_sum<ND, NW>(v, red);
         if (threadIdx.x == 0) {
 #pragma unroll
             for (int q = 0; q < ND; ++q) a.part_out[(size_t)q * a.ngroups + p] = v[q];
@@ -599,29 +673,46 @@ __global__ void k_state_init(SolverState *st, int nstates, double tol, int maxit
     }
 }
 
-template <typename T, int NPT>
+template <typename T, int NPT, int TH>
 hipError_t spmv_epi(const SpmvArgs<T> &a, int epi, size_t shmem, hipStream_t s)
 {
     switch (epi) {
-    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
-    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
-    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
-    case EPI_RESID: k_spmv<T, EPI_RESID, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
-    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT><<<a.ngroups, kThreads, shmem, s>>>(a); break;
+    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+// Allow the dynamic LDS of the wide blocks (above the 64 KiB default).
+template <typename T, int NPT, int TH>
+hipError_t spmv_allow_lds(size_t shmem)
+{
+    const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
+    hipError_t e = hipSuccess;
+    const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH>, (const void *)k_spmv<T, EPI_DOT1, NPT, TH>,
+                       (const void *)k_spmv<T, EPI_DOT2, NPT, TH>, (const void *)k_spmv<T, EPI_RESID, NPT, TH>,
+                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH>};
+    for (const void *f : k)
+        if (e == hipSuccess) e = hipFuncSetAttribute(f, at, (int)shmem);
+    return e;
+}
+
+// Row blocks up to 2048 entries: 256 threads, one pass; larger ones: 512
+// threads, passes of 4096 entries (NPT 8 keeps FP64 under 128 VGPRs, so two
+// such workgroups share a CU).
 template <typename T>
 hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, hipStream_t s)
 {
     if (a.ngroups <= 0) return hipSuccess;
     if (block_nnz <= 0 || block_nnz > kMaxBlockNnz) return hipErrorInvalidValue;
-    const size_t shmem = (size_t)block_nnz * sizeof(T);
-    if (block_nnz <= 4 * kThreads) return spmv_epi<T, 4>(a, epi, shmem, s);
-    if (block_nnz <= 8 * kThreads) return spmv_epi<T, 8>(a, epi, shmem, s);
-    return spmv_epi<T, 16>(a, epi, shmem, s);
+    const size_t prod = (size_t)block_nnz * sizeof(T);
+    if (block_nnz <= 4 * kThreads) return spmv_epi<T, 4, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
+    if (block_nnz <= 8 * kThreads) return spmv_epi<T, 8, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
+    return spmv_epi<T, 8, kWideThreads>(a, epi, prod + rowptr_lds_bytes<kWideThreads>(), s);
 }
 
 template <typename T>
@@ -654,6 +745,14 @@ HBS_VEC_KERNEL(k_vec_XR)
 
 }  // namespace
 
+hipError_t prepare_spmv(int block_nnz)
+{
+    if (spmv_threads(block_nnz) == kThreads) return hipSuccess;
+    const size_t rows = rowptr_lds_bytes<kWideThreads>();
+    hipError_t e = spmv_allow_lds<double, 8, kWideThreads>((size_t)block_nnz * sizeof(double) + rows);
+    if (e == hipSuccess) e = spmv_allow_lds<float, 8, kWideThreads>((size_t)block_nnz * sizeof(float) + rows);
+    return e;
+}
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, hipStream_t s)
 {
     return spmv_dispatch<double>(a, epi, block_nnz, s);

@@ -6,7 +6,9 @@
 namespace hbs {
 
 namespace {
-constexpr int kMaxRowsPerBlock = 1024;   // 4 rows per thread in the serial sum
+// Rows per block: fewer than 4 per thread of the SpMV workgroup (the kernel
+// stages the block's nr + 1 row offsets with 4 loads per thread).
+constexpr int max_rows_per_block(int threads) { return 4 * threads - 1; }
 
 int pow2_floor(int v)
 {
@@ -17,25 +19,31 @@ int pow2_floor(int v)
 }  // namespace
 
 // Greedy row blocking: a block takes consecutive rows while their entries fit
-// in block_nnz (the LDS product buffer).  A row longer than block_nnz becomes
-// one kind-1 block whose value is the ordered sum of its chunk sums, each
-// chunk (<= block_nnz entries) summed by its own workgroup of k_chunks.
+// in block_nnz (the LDS product buffer).  A row longer than
+// min(block_nnz, kLongRow) becomes one kind-1 block whose value is the
+// ordered sum of its chunk sums, each chunk (<= that many entries) summed by
+// its own workgroup of k_chunks, so no thread sums more than kLongRow
+// products in turn.
 // Blocks whose average row length is at most serial_max_avg use the
-// order-exact serial row sum; others pow2_floor(256 / rows) lanes per row.
+// order-exact serial row sum; others pow2_floor(threads / rows) lanes per
+// row, threads = spmv_threads(block_nnz).
 void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max_avg, int sys,
                HostPlan &plan)
 {
     const int end = row0 + m;
+    const int threads = spmv_threads(block_nnz);
+    const int max_rows = max_rows_per_block(threads);
+    const int long_row = std::min(block_nnz, kLongRow);
     int r = row0;
     while (r < end) {
         const int len = rowptr[r + 1] - rowptr[r];
         plan.max_row = std::max(plan.max_row, len);
-        if (len > block_nnz) {
-            const int nch = (len + block_nnz - 1) / block_nnz;
+        if (len > long_row) {
+            const int nch = (len + long_row - 1) / long_row;
             const int first = (int)plan.chunks.size();
             for (int c = 0; c < nch; ++c) {
-                const int j0 = rowptr[r] + c * block_nnz;
-                plan.chunks.push_back(Chunk{j0, std::min(j0 + block_nnz, rowptr[r + 1]), r, sys});
+                const int j0 = rowptr[r] + c * long_row;
+                plan.chunks.push_back(Chunk{j0, std::min(j0 + long_row, rowptr[r + 1]), r, sys});
             }
             plan.blocks.push_back(RowBlock{r, r + 1, rowptr[r], rowptr[r + 1]});
             plan.aux.push_back(BlockAux{1, first, nch, 1});
@@ -44,7 +52,8 @@ void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max
         }
         const int r0 = r;
         const int j0 = rowptr[r];
-        while (r < end && rowptr[r + 1] - j0 <= block_nnz && r - r0 < kMaxRowsPerBlock) {
+        while (r < end && rowptr[r + 1] - j0 <= block_nnz && rowptr[r + 1] - rowptr[r] <= long_row &&
+               r - r0 < max_rows) {
             plan.max_row = std::max(plan.max_row, rowptr[r + 1] - rowptr[r]);
             ++r;
         }
@@ -52,7 +61,7 @@ void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max
         const int nz = rowptr[r] - j0;
         int lanes = 1;
         if (nr > 0 && nz > serial_max_avg * nr) {
-            lanes = std::min(64, pow2_floor(kThreads / nr));
+            lanes = std::min(64, pow2_floor(threads / nr));
             if (lanes < 2) lanes = 1;
         }
         if (lanes == 1) plan.serial_rows += nr;

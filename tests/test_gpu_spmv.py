@@ -70,6 +70,44 @@ def test_spmv_plans_power_law(ctx, block_nnz, serial):
     assert np.all(np.abs(y - ref) <= bound + 1e-300)
 
 
+@pytest.mark.parametrize("block_nnz", [6144, 8192, 16384])
+@pytest.mark.parametrize("which", ["markov50k", "circuit60k"])
+def test_spmv_wide_blocks_bit_exact(ctx, block_nnz, which):
+    """512-thread row blocks, several passes of products into LDS (column-sorted
+    where it pays): serial rows stay bit-exact in FP64 and within the FP32
+    bound, long rows within the summation bound."""
+    if which == "markov50k":
+        rp, c, v = markov(50000)
+    else:
+        rp, c, v = hbs.gen_circuit(60000, nheavy=12, heavy_lo=300, heavy_hi=30000, seed=11)
+    n = rp.size - 1
+    x = np.random.default_rng(8).uniform(-1, 1, n)
+    hbs.set_spmv_plan(block_nnz, 1 << 20)   # every block that fits sums serially
+    try:
+        A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+        dx, dy = ctx.to_device(x), ctx.empty(n)
+        A.spmv(dx, dy)
+        y = dy.download()
+        x32 = x.astype(np.float32)
+        dx32, dy32 = ctx.to_device(x32), ctx.empty(n, np.float32)
+        A.spmv(dx32, dy32)
+        y32 = dy32.download().astype(np.float64)
+        info = A.info()
+    finally:
+        hbs.set_spmv_plan()
+    print(which, block_nnz, info, flush=True)
+    assert info["block_nnz"] == block_nnz
+    ref = orc.spmv(orc.CSR(rp, c, v), x)
+    k = np.diff(rp)
+    serial = k <= min(block_nnz, 1024)   # longer rows are summed in chunks
+    assert np.array_equal(y[serial], ref[serial])
+    assert np.all(np.abs(y - ref) <= seq_bound(rp, c, v, x) + 1e-300)
+    ref32 = orc.spmv(orc.CSR(rp, c, v.astype(np.float32).astype(np.float64)), x32.astype(np.float64))
+    u = np.finfo(np.float32).eps / 2
+    bound = 2 * (2 * k * u / (1 - k * u)) * orc.spmv(orc.CSR(rp, c, np.abs(v)), np.abs(x))
+    assert np.all(np.abs(y32 - ref32) <= bound * 1.01 + 1e-30)
+
+
 def test_spmv_alpha_beta_and_empty_rows(ctx):
     rp, c, v = random_csr(500, 400, 0.01, 9)   # has empty rows
     assert (np.diff(rp) == 0).any()
