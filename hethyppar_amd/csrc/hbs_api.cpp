@@ -409,16 +409,27 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     std::vector<int> src;
     std::vector<unsigned short> spos;
     if (plan_sorted(col, nnz, g_sort_pct, hp, src, spos) > 0) {
+        // When the column index and the position in the block fit in 32 bits
+        // together, the position rides in the column's high bits and the
+        // separate position stream is not needed.
+        auto bits = [](long long v) {
+            int b = 1;
+            while (b < 31 && (1LL << b) <= v) ++b;
+            return b;
+        };
+        const int cb = bits((long long)n - 1), pb = bits((long long)block_nnz - 1);
+        a->pos_shift = cb + pb <= 32 ? cb : 0;
         std::vector<int> sc((size_t)nnz);
         std::vector<double> sv((size_t)nnz);
         for (int j = 0; j < nnz; ++j) {
-            sc[(size_t)j] = col[src[(size_t)j]];
+            const unsigned c = (unsigned)col[src[(size_t)j]];
+            sc[(size_t)j] = (int)(a->pos_shift ? c | ((unsigned)spos[(size_t)j] << a->pos_shift) : c);
             sv[(size_t)j] = val[src[(size_t)j]];
         }
-        if ((rc = dalloc(&a->scol, nnz)) || (rc = dalloc(&a->sval, nnz)) || (rc = dalloc(&a->sperm, nnz)) ||
-            (rc = dalloc(&a->ssrc, nnz)) || (rc = upload(a->scol, sc)) || (rc = upload(a->sval, sv)) ||
-            (rc = upload(a->sperm, spos)) || (rc = upload(a->ssrc, src)))
+        if ((rc = dalloc(&a->scol, nnz)) || (rc = dalloc(&a->sval, nnz)) || (rc = dalloc(&a->ssrc, nnz)) ||
+            (rc = upload(a->scol, sc)) || (rc = upload(a->sval, sv)) || (rc = upload(a->ssrc, src)))
             return bail(rc);
+        if (!a->pos_shift && ((rc = dalloc(&a->sperm, nnz)) || (rc = upload(a->sperm, spos)))) return bail(rc);
         if (a->val32) {
             if ((rc = dalloc(&a->sval32, nnz))) return bail(rc);
             if (launch_to_f32(nnz, a->sval, a->sval32, ctx->stream) != hipSuccess ||

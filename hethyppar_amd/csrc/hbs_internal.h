@@ -32,7 +32,8 @@ struct alignas(16) RowBlock {
 // value is the sum, in order, of the chunk sums [first, first + nchunks).
 // kind 2: as kind 0, but the products are formed in column order (the
 // column-sorted copy scol/sval of the block's entries) and stored at their
-// CSR positions sperm, so a wave's x gathers fall on fewer cache lines.
+// CSR positions (sperm, or the high bits of scol), so a wave's x gathers fall
+// on fewer cache lines.
 struct alignas(16) BlockAux {
     int kind, first, nchunks, lanes;
 };
@@ -121,8 +122,9 @@ struct SpmvArgs {
     const T *w;
     T alpha, beta;
     const int *scol;            // kind-2 blocks: entries in column order,
-    const T *sval;              //   with their positions in the block
-    const unsigned short *sperm;
+    const T *sval;              //   with their positions in the block:
+    const unsigned short *sperm;    //   sperm[j], or (pos_shift > 0)
+    int pos_shift;              //   scol[j] >> pos_shift over column bits
     const double *chunk_sums;   // sums of the long-row chunks (k_chunks, same x)
     double *part_out;           // [ngroups * kMaxDots] partials of the fused dots
     const double *part_in;      // partials of the previous vector kernel (FIN_S)

@@ -92,7 +92,7 @@ __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock 
     const int lim = rb.j1 - rb.j0 - base;
     const int *__restrict__ colp = (sorted ? a.scol : a.col) + rb.j0 + base;
     const T *__restrict__ valp = (sorted ? a.sval : a.val) + rb.j0 + base;
-    if (sorted) {
+    if (sorted && !a.pos_shift) {
         const unsigned short *__restrict__ pp = a.sperm + rb.j0 + base;
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
@@ -219,19 +219,17 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (cur.ax.kind != 1) issue_pass<T, NPT, TH>(a, cur.rb, cur.ax, 0, LA);
     T w0 = issue_w0<T, EPI>(a, cur.rb, cur.ax);
 
-    if (a.st_in) {
+    // ||s|| test: the system's first workgroup applies it to the state it
+    // hands on; the product is formed either way (when the test fires, the
+    // consumer ignores it), so no other workgroup waits on the reduction.
+    if (a.st_in && a.fin_in == FIN_S && p == a.sys[g.sys].g0) {
+        const SysRange sr = a.sys[g.sys];
         if (threadIdx.x == 0) sh = a.st_in[g.sys];
-        __syncthreads();
-        if (a.fin_in == FIN_S) {
-            const SysRange sr = a.sys[g.sys];
-            double v[1];
-            reduce_parts<1, NW>(a.part_in, a.nparts_in, sr.v0, sr.v1, v, red);
-            if (threadIdx.x == 0) {
-                apply_fin(sh, FIN_S, v[0], 0.0);
-                if (p == sr.g0) a.st_out[g.sys] = sh;
-            }
-            __syncthreads();
-            if (sh.s_conv) return;
+        double v[1];
+        reduce_parts<1, NW>(a.part_in, a.nparts_in, sr.v0, sr.v1, v, red);
+        if (threadIdx.x == 0) {
+            apply_fin(sh, FIN_S, v[0], 0.0);
+            a.st_out[g.sys] = sh;
         }
     }
     if (a.st_mark && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_mark[g.sys].pending = 1;
@@ -244,6 +242,9 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const bool gather = cur.ax.kind != 1;
         const bool last = !gather || cur.base + PASS >= nz;
         T xv[NPT];
+        // kind-2 blocks with packed positions: column in the low pos_shift bits
+        const bool packed = cur.ax.kind == 2 && a.pos_shift;
+        const unsigned cmask = packed ? (1u << a.pos_shift) - 1u : ~0u;
         if (gather) {
             const int lim = nz - cur.base;
 #pragma unroll
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const int k = (int)threadIdx.x + i * TH;
                 // 32-bit byte offset from the uniform base (n * sizeof(T) < 2^32)
                 xv[i] = k < lim ? *reinterpret_cast<const T *>(reinterpret_cast<const char *>(a.x) +
-                                                              (unsigned)in.c[i] * (unsigned)sizeof(T))
+                                                              ((unsigned)in.c[i] & cmask) * (unsigned)sizeof(T))
                                 : T(0);
             }
         }
@@ -284,7 +285,9 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
             for (int i = 0; i < NPT; ++i) {
                 const int k = (int)threadIdx.x + i * TH;
-                if (k < lim) lds[sorted ? (int)in.pos[i] : cur.base + k] = in.v[i] * xv[i];
+                const int at = !sorted ? cur.base + k
+                                       : packed ? (int)((unsigned)in.c[i] >> a.pos_shift) : (int)in.pos[i];
+                if (k < lim) lds[at] = in.v[i] * xv[i];
             }
         }
         if (stage) {

@@ -92,7 +92,8 @@ struct hbs_csr_s {
     int *scol = nullptr;              // column-sorted copy of kind-2 blocks (else CSR order)
     double *sval = nullptr;
     float *sval32 = nullptr;
-    unsigned short *sperm = nullptr;  // position in the block of each sorted entry
+    unsigned short *sperm = nullptr;  // position in the block of each sorted entry (pos_shift 0)
+    int pos_shift = 0;                // > 0: scol = col | position << pos_shift, no sperm
     int *ssrc = nullptr;              // CSR entry of each sorted position (refreshes sval)
     hbs::SpmvPlan plan;
     hbs::SysRange *sys = nullptr;     // [nsys] {g0, g1, 0, 0}
@@ -164,6 +165,7 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
     if constexpr (sizeof(T) == 8) s.sval = reinterpret_cast<const T *>(a->sval);
     else s.sval = reinterpret_cast<const T *>(a->sval32);
     s.sperm = a->sperm;
+    s.pos_shift = a->pos_shift;
     s.chunk_sums = a->chunk_sums;
     s.part_out = a->part;
     s.stream_nt = a->stream_nt;

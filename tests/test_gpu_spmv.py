@@ -70,14 +70,18 @@ def test_spmv_plans_power_law(ctx, block_nnz, serial):
     assert np.all(np.abs(y - ref) <= bound + 1e-300)
 
 
-@pytest.mark.parametrize("block_nnz", [6144, 8192, 16384])
-@pytest.mark.parametrize("which", ["markov50k", "circuit60k"])
+@pytest.mark.parametrize("block_nnz,which", [(b, w) for b in (6144, 8192, 16384) for w in ("markov50k", "circuit60k")]
+                         + [(16384, "markov600k")])
 def test_spmv_wide_blocks_bit_exact(ctx, block_nnz, which):
     """512-thread row blocks, several passes of products into LDS (column-sorted
     where it pays): serial rows stay bit-exact in FP64 and within the FP32
-    bound, long rows within the summation bound."""
+    bound, long rows within the summation bound.  markov600k's column (20
+    bits) and in-block position (14 bits) do not fit one word, so its sorted
+    blocks read the separate position stream; the others pack them."""
     if which == "markov50k":
         rp, c, v = markov(50000)
+    elif which == "markov600k":
+        rp, c, v = markov(600000)
     else:
         rp, c, v = hbs.gen_circuit(60000, nheavy=12, heavy_lo=300, heavy_hi=30000, seed=11)
     n = rp.size - 1
