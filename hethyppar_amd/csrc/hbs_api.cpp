@@ -22,6 +22,7 @@ int g_block_nnz = 0;      // 0: by average row length (defaults from sweeps on M
 int g_serial_max_avg = 64;
 int g_max_groups = kMaxGroups;
 int g_stream_nt = 0;
+int g_spmv_npt = 0;       // entries per thread per pass of 512-thread SpMV blocks (0: 4)
 int g_sort_pct = 70;      // kind-2 blocks: sorted gathers touch <= 70% as many lines as entries
 }  // namespace
 
@@ -295,6 +296,14 @@ extern "C" int hbs_set_spmv_stream_nt(int on)
     return HBS_OK;
 }
 
+extern "C" int hbs_set_spmv_pass(int entries_per_thread)
+{
+    if (entries_per_thread != 0 && entries_per_thread != 4 && entries_per_thread != 8)
+        return fail(HBS_ERR_ARG, "entries_per_thread must be 0, 4 or 8");
+    g_spmv_npt = entries_per_thread;
+    return HBS_OK;
+}
+
 extern "C" int hbs_csr_destroy(hbs_csr a)
 {
     if (!a) return HBS_OK;
@@ -346,14 +355,15 @@ int validate_csr(int m, int n, int nnz, const int *rowptr, const int *col, const
 // Entries per row block (measured on MI355X).  Short rows (under 10 on
 // average) gather little per row and do best with blocks of 1024 entries
 // and many of them.  Longer rows gain from big blocks: column-sorted gathers
-// share more x lines and the 512-thread workgroups keep their loads in
-// flight; the block is sized for about 1024 of them, between 2048 and 8192
-// entries, so a small matrix still fills the GPU.
+// share more x lines.  Up to 6144 entries, three 512-thread workgroups (4
+// entries per thread per pass) fit a CU's LDS and keep their loads in
+// flight.  The block is sized for about 1024 of them, so a small matrix
+// still fills the GPU.
 int default_block_nnz(int m, int nnz)
 {
     if ((long long)nnz < 10LL * std::max(m, 1)) return 1024;
     const long long per = (long long)nnz / 1024 / 256 * 256;
-    return (int)std::min<long long>(8192, std::max<long long>(2048, per));
+    return (int)std::min<long long>(6144, std::max<long long>(2048, per));
 }
 
 int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int *col, const double *val, int flags,
@@ -437,12 +447,15 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
                 return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
         }
     }
-    if (prepare_spmv(block_nnz) != hipSuccess)
+    const int npt = spmv_threads(block_nnz) == kThreads ? (block_nnz <= 4 * kThreads ? 4 : 8)
+                                                        : (g_spmv_npt ? g_spmv_npt : 4);
+    if (prepare_spmv(block_nnz, npt) != hipSuccess)
         return bail(fail(HBS_ERR_HIP, "SpMV LDS limit for %d-entry row blocks not granted", block_nnz));
     SpmvPlan &p = a->plan;
     p.nsys = nsys;
     a->stream_nt = g_stream_nt;
     p.block_nnz = block_nnz;
+    p.npt = npt;
     p.nblk = (int)hp.blocks.size();
     p.ngroups = (int)hp.groups.size();
     p.nchunks = (int)hp.chunks.size();

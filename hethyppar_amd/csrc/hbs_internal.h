@@ -98,6 +98,7 @@ struct SpmvPlan {
     int nchunks = 0;
     int nsys = 1;
     int block_nnz = 0;
+    int npt = 0;                  // SpMV entries per thread per pass
     int serial_rows = 0;
     int max_row = 0;
     RowBlock *blocks = nullptr;   // device [nblk]
@@ -232,10 +233,11 @@ int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 
 // ---- launchers (hbs_kernels.hip) ----
 // Raise the dynamic LDS limit of the SpMV kernels for row blocks of block_nnz
-// entries (once per plan, outside any stream capture).
-hipError_t prepare_spmv(int block_nnz);
-hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, hipStream_t s);
-hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, hipStream_t s);
+// entries, npt entries per thread per pass (once per plan, outside any
+// stream capture).
+hipError_t prepare_spmv(int block_nnz, int npt);
+hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, int npt, hipStream_t s);
+hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, int npt, hipStream_t s);
 hipError_t launch_chunks(const ChunkArgs<double> &a, int block_nnz, hipStream_t s);
 hipError_t launch_chunks(const ChunkArgs<float> &a, int block_nnz, hipStream_t s);
 hipError_t launch_vec_P(const VecArgs<double> &a, hipStream_t s);

@@ -705,17 +705,19 @@ hipError_t spmv_allow_lds(size_t shmem)
 }
 
 // Row blocks up to 2048 entries: 256 threads, one pass; larger ones: 512
-// threads, passes of 4096 entries (NPT 8 keeps FP64 under 128 VGPRs, so two
-// such workgroups share a CU).
+// threads, passes of npt * 512 entries (npt 4: under 72 VGPRs, so three such
+// workgroups share a CU when their LDS fits; npt 8: under 128 VGPRs, two).
 template <typename T>
-hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, hipStream_t s)
+hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, int npt, hipStream_t s)
 {
     if (a.ngroups <= 0) return hipSuccess;
     if (block_nnz <= 0 || block_nnz > kMaxBlockNnz) return hipErrorInvalidValue;
     const size_t prod = (size_t)block_nnz * sizeof(T);
     if (block_nnz <= 4 * kThreads) return spmv_epi<T, 4, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
     if (block_nnz <= 8 * kThreads) return spmv_epi<T, 8, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
-    return spmv_epi<T, 8, kWideThreads>(a, epi, prod + rowptr_lds_bytes<kWideThreads>(), s);
+    const size_t shmem = prod + rowptr_lds_bytes<kWideThreads>();
+    if (npt == 4) return spmv_epi<T, 4, kWideThreads>(a, epi, shmem, s);
+    return spmv_epi<T, 8, kWideThreads>(a, epi, shmem, s);
 }
 
 template <typename T>
@@ -748,21 +750,28 @@ HBS_VEC_KERNEL(k_vec_XR)
 
 }  // namespace
 
-hipError_t prepare_spmv(int block_nnz)
+hipError_t prepare_spmv(int block_nnz, int npt)
 {
     if (spmv_threads(block_nnz) == kThreads) return hipSuccess;
     const size_t rows = rowptr_lds_bytes<kWideThreads>();
-    hipError_t e = spmv_allow_lds<double, 8, kWideThreads>((size_t)block_nnz * sizeof(double) + rows);
-    if (e == hipSuccess) e = spmv_allow_lds<float, 8, kWideThreads>((size_t)block_nnz * sizeof(float) + rows);
+    const size_t d = (size_t)block_nnz * sizeof(double) + rows, f = (size_t)block_nnz * sizeof(float) + rows;
+    hipError_t e;
+    if (npt == 4) {
+        e = spmv_allow_lds<double, 4, kWideThreads>(d);
+        if (e == hipSuccess) e = spmv_allow_lds<float, 4, kWideThreads>(f);
+    } else {
+        e = spmv_allow_lds<double, 8, kWideThreads>(d);
+        if (e == hipSuccess) e = spmv_allow_lds<float, 8, kWideThreads>(f);
+    }
     return e;
 }
-hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, hipStream_t s)
+hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, int npt, hipStream_t s)
 {
-    return spmv_dispatch<double>(a, epi, block_nnz, s);
+    return spmv_dispatch<double>(a, epi, block_nnz, npt, s);
 }
-hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, hipStream_t s)
+hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, int npt, hipStream_t s)
 {
-    return spmv_dispatch<float>(a, epi, block_nnz, s);
+    return spmv_dispatch<float>(a, epi, block_nnz, npt, s);
 }
 hipError_t launch_chunks(const ChunkArgs<double> &a, int, hipStream_t s) { return chunk_dispatch<double>(a, s); }
 hipError_t launch_chunks(const ChunkArgs<float> &a, int, hipStream_t s) { return chunk_dispatch<float>(a, s); }

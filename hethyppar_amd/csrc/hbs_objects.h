@@ -191,7 +191,7 @@ hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a, int epi, const Solver
         hipError_t e = launch_chunks(c, a->plan.block_nnz, q);
         if (e != hipSuccess) return e;
     }
-    return launch_spmv(s, epi, a->plan.block_nnz, q);
+    return launch_spmv(s, epi, a->plan.block_nnz, a->plan.npt, q);
 }
 
 }  // namespace hbs

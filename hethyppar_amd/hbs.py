@@ -137,6 +137,7 @@ _PROTOS = {
     "hbs_set_spmv_plan": (_i, [_i, _i]),
     "hbs_set_spmv_groups": (_i, [_i]),
     "hbs_set_spmv_stream_nt": (_i, [_i]),
+    "hbs_set_spmv_pass": (_i, [_i]),
     "hbs_set_vec_blocks": (_i, [_i]),
     "hbs_set_spmv_sort": (_i, [_i]),
     "hbs_csr_plan_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
@@ -684,11 +685,12 @@ class Shard:
             pass
 
 
-def set_spmv_plan(block_nnz=0, serial_max_avg=64, max_groups=4096, stream_nt=0):
+def set_spmv_plan(block_nnz=0, serial_max_avg=64, max_groups=4096, stream_nt=0, npt=0):
     """Process-wide plan knobs used by DeviceCSR objects created afterwards."""
     check(lib().hbs_set_spmv_plan(block_nnz, serial_max_avg))
     check(lib().hbs_set_spmv_groups(max_groups))
     check(lib().hbs_set_spmv_stream_nt(stream_nt))
+    check(lib().hbs_set_spmv_pass(npt))
 
 
 def set_vec_blocks(target=512):

@@ -325,6 +325,10 @@ int hbs_set_vec_blocks(int target);
 /* 1: load the matrix streams (col, val) non-temporally (default 0: measured
  * slower on MI355X for every workload tried). */
 int hbs_set_spmv_stream_nt(int on);
+/* Entries per thread per pass of the 512-thread SpMV workgroups (row blocks
+ * above 2048 entries) of matrices created afterwards: 4 or 8, 0 for the
+ * default 4. */
+int hbs_set_spmv_pass(int entries_per_thread);
 /* Plan of a device CSR: SpMV workgroups, long-row chunks, nnz per row
  * block, longest row. */
 int hbs_csr_plan_info(hbs_csr a, int *ngroups, int *nchunks, int *block_nnz, int *max_row);
