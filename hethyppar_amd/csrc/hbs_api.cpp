@@ -382,14 +382,21 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
         return rc;
     };
     int rc;
-    if ((rc = dalloc(&a->rowptr, (size_t)m + 1)) || (rc = dalloc(&a->col, nnz)) || (rc = dalloc(&a->val, nnz)))
+    // col / val / val32 carry kRowPad zero entries past nnz: the 64-column
+    // SpMM reads whole chunks of a row's entries through the scalar cache
+    const size_t padded = (size_t)nnz + kRowPad;
+    if ((rc = dalloc(&a->rowptr, (size_t)m + 1)) || (rc = dalloc(&a->col, padded)) || (rc = dalloc(&a->val, padded)))
         return bail(rc);
     if (hipMemcpy(a->rowptr, rowptr, sizeof(int) * ((size_t)m + 1), hipMemcpyHostToDevice) != hipSuccess ||
         (nnz && hipMemcpy(a->col, col, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) ||
-        (nnz && hipMemcpy(a->val, val, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess))
+        (nnz && hipMemcpy(a->val, val, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemset(a->col + nnz, 0, sizeof(int) * kRowPad) != hipSuccess ||
+        hipMemset(a->val + nnz, 0, sizeof(double) * kRowPad) != hipSuccess)
         return bail(fail(HBS_ERR_HIP, "CSR upload failed"));
     if (flags & HBS_CSR_F32) {
-        if ((rc = dalloc(&a->val32, nnz))) return bail(rc);
+        if ((rc = dalloc(&a->val32, padded))) return bail(rc);
+        if (hipMemset(a->val32 + nnz, 0, sizeof(float) * kRowPad) != hipSuccess)
+            return bail(fail(HBS_ERR_HIP, "CSR upload failed"));
         if (launch_to_f32(nnz, a->val, a->val32, ctx->stream) != hipSuccess ||
             hipStreamSynchronize(ctx->stream) != hipSuccess)
             return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_pattern(int m, const int *row
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_spmm(int m, const int *__restrict__ rowptr,
                                                   const int *__restrict__ col,
-                                                  const T *__restrict__ val, int k,
+                                                  const T *__restrict__ val, int k, int ld,
                                                   const T *__restrict__ X, T *__restrict__ Y, int tile)
 {
     constexpr int RB = 2;   // rows per lane in flight together
@@ -89,10 +89,34 @@ __global__ __launch_bounds__(kThreads) void k_spmm(int m, const int *__restrict_
     for (int t0 = x0 + wi * RB * rstep; t0 < x1; t0 += per * RB * rstep) {
         const int r = t0 + (int)threadIdx.x / tile;
         T acc[RB];
-        rows_dot<T, RB>(rowptr, col, val, X + c, k, r, rstep, x1, acc);
+        rows_dot<T, RB>(rowptr, col, val, X + c, ld, r, rstep, x1, acc);
 #pragma unroll
         for (int q = 0; q < RB; ++q)
-            if (r + q * rstep < x1) Y[(size_t)(r + q * rstep) * k + c] = acc[q];
+            if (r + q * rstep < x1) Y[(size_t)(r + q * rstep) * ld + c] = acc[q];
+    }
+}
+
+// Tiles of 64 columns: one wave per row step (wave_rows64), so the row's
+// indices and values are read once per wave through the scalar cache and
+// every gather is a contiguous 64-column stretch of an X row.  Rows are
+// dealt per XCD as in k_spmm, in steps of RB rows per wave.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_spmm64(int m, const int *__restrict__ rowptr,
+                                                    const int *__restrict__ col, const T *__restrict__ val,
+                                                    int k, int ld, const T *__restrict__ X, T *__restrict__ Y)
+{
+    constexpr int RB = 2;
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.y * 64 + lane;
+    const bool ok = c < k;
+    const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, per = gridDim.x / kXcds;
+    const int x0 = (int)((long long)m * xcd / kXcds), x1 = (int)((long long)m * (xcd + 1) / kXcds);
+    const int wv = wi * kWaves + (int)(threadIdx.x >> 6), nw = per * kWaves;
+    for (int r = x0 + wv * RB; r < x1; r += nw * RB) {
+        const int ru = __builtin_amdgcn_readfirstlane(r);
+        wave_rows64<T, RB>(rowptr, col, val, X, ld, c, ru, x1, ok, [&](int q, T acc) {
+            if (ok && ru + q < x1) Y[(size_t)(ru + q) * ld + c] = acc;
+        });
     }
 }
 
@@ -213,7 +237,8 @@ hipError_t spmm_dispatch(int m, const int *rowptr, const int *col, const T *val,
     long long gx = std::min(std::max(1LL, 2048LL / ytiles), passes);
     gx = std::max(8LL, (gx + 7) / 8 * 8);
     dim3 grid((unsigned)gx, (unsigned)ytiles);
-    k_spmm<T><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, X, Y, tile);
+    if (tile == 64) k_spmm64<T><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y);
+    else k_spmm<T><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, tile);
     return hipGetLastError();
 }
 hipError_t launch_spmm_f64(int m, const int *rowptr, const int *col, const double *val, int k,

@@ -85,8 +85,12 @@ __device__ __forceinline__ bool column_active(const BlockArgs &a, const LaneMap 
     return !s.done && !(need_not_sconv && s.s_conv);
 }
 
-template <int EPI>
-__global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a)
+// WIDE (kp == 64): one row per wave step; otherwise 64 / kp rows per wave.
+// The CSR arrays come as their own restrict arguments, so the compiler may
+// read them through the scalar cache.
+template <int EPI, bool WIDE>
+__global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ col, const double *__restrict__ val)
 {
     constexpr int ND = EPI == BEPI_DOT1 ? 1 : 2;
     const int p = blockIdx.x;
@@ -95,7 +99,45 @@ __global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a)
     double v[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = 0.0;
-    if (__syncthreads_or(act)) {
+    // epilogue of row rq of this lane's column (w = the row's operand)
+    auto emit = [&](int rq, double acc, double wr) {
+        const size_t i = (size_t)rq * a.ld + m.c;
+        if constexpr (EPI == BEPI_DOT1) {
+            a.y[i] = acc;
+            v[0] += wr * acc;
+        } else if constexpr (EPI == BEPI_DOT2) {
+            a.y[i] = acc;
+            v[0] += acc * wr;
+            v[1] += acc * acc;
+        } else {
+            const double o = wr - acc;
+            a.y[i] = o;
+            v[0] += o * o;
+            v[1] += o * o;
+        }
+    };
+    const bool any = __syncthreads_or(act);
+    if constexpr (WIDE) {
+        // one row per wave step, RB rows per step: indices and values through
+        // the scalar cache, one contiguous 64-column gather per entry
+        constexpr int RB = 2;
+        if (!any) {
+            column_partials<ND>(a, v, p);
+            return;
+        }
+        const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, per = a.ngroups / kXcds;
+        const int x0 = (int)((long long)a.n * xcd / kXcds), x1 = (int)((long long)a.n * (xcd + 1) / kXcds);
+        const int wv = wi * kWaves + (int)(threadIdx.x >> 6), nw = per * kWaves;
+        for (int r = x0 + wv * RB; r < x1; r += nw * RB) {
+            const int ru = __builtin_amdgcn_readfirstlane(r);
+            double wr[RB];
+#pragma unroll
+            for (int q = 0; q < RB; ++q) wr[q] = act && ru + q < x1 ? a.w[(size_t)(ru + q) * a.ld + m.c] : 0.0;
+            wave_rows64<double, RB>(rowptr, col, val, a.x, a.ld, m.c, ru, x1, act, [&](int q, double acc) {
+                if (act && ru + q < x1) emit(ru + q, acc, wr[q]);
+            });
+        }
+    } else if (any) {
         constexpr int RB = 2;   // rows per lane in flight together
         const RowSweep sw = row_sweep(a.n, a.ngroups, RB * m.rstep);
         const double *__restrict__ x = a.x + m.c;
@@ -108,21 +150,7 @@ __global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a)
             for (int q = 0; q < RB; ++q) {
                 const int rq = r + q * m.rstep;
                 if (rq >= sw.end) break;
-                const double acc = accs[q];
-                const size_t i = (size_t)rq * a.ld + m.c;
-                if constexpr (EPI == BEPI_DOT1) {
-                    a.y[i] = acc;
-                    v[0] += a.w[i] * acc;
-                } else if constexpr (EPI == BEPI_DOT2) {
-                    a.y[i] = acc;
-                    v[0] += acc * a.w[i];
-                    v[1] += acc * acc;
-                } else {
-                    const double o = a.w[i] - acc;
-                    a.y[i] = o;
-                    v[0] += o * o;
-                    v[1] += o * o;
-                }
+                emit(rq, accs[q], a.w[(size_t)rq * a.ld + m.c]);
             }
         }
     }
@@ -210,9 +238,18 @@ hipError_t launch_block_spmm(const BlockArgs &a, int epi, hipStream_t s)
 {
     if (a.ngroups <= 0) return hipSuccess;
     switch (epi) {
-    case BEPI_DOT1: k_bspmm<BEPI_DOT1><<<a.ngroups, kThreads, 0, s>>>(a); break;
-    case BEPI_DOT2: k_bspmm<BEPI_DOT2><<<a.ngroups, kThreads, 0, s>>>(a); break;
-    case BEPI_RESID: k_bspmm<BEPI_RESID><<<a.ngroups, kThreads, 0, s>>>(a); break;
+    case BEPI_DOT1:
+        if (a.kp == 64) k_bspmm<BEPI_DOT1, true><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
+        else k_bspmm<BEPI_DOT1, false><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
+        break;
+    case BEPI_DOT2:
+        if (a.kp == 64) k_bspmm<BEPI_DOT2, true><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
+        else k_bspmm<BEPI_DOT2, false><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
+        break;
+    case BEPI_RESID:
+        if (a.kp == 64) k_bspmm<BEPI_RESID, true><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
+        else k_bspmm<BEPI_RESID, false><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

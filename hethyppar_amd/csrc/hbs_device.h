@@ -203,6 +203,72 @@ __device__ __forceinline__ void rows_dot(const int *__restrict__ rowptr, const i
     }
 }
 
+// One step of one wave over 64 consecutive columns of row-major blocks:
+// rows r .. r + RB - 1 (those at or past `end` are empty) of X (column 0 of
+// row 0 at xb, leading dimension ld), lane l on column lc, each row summed
+// over its entries in CSR order, the order of row_dot.  r must be
+// wave-uniform.  The row pointers, column indices and values are then the
+// same for the whole wave, so they come through the scalar cache in chunks
+// of kRowPad entries (the CSR arrays carry kRowPad entries of padding past
+// nnz); each entry is one 64-lane gather of a contiguous stretch of an X
+// row.  The first kRowPad entries of all RB rows are loaded, then gathered,
+// before any is summed: gathers past a row's end re-read the row's first
+// column (or column 0), and lanes with lane_ok false read column 0, so
+// every load is in bounds and none waits on a branch.  out(q, acc) receives
+// row r + q.
+template <typename T, int RB, typename Out>
+__device__ __forceinline__ void wave_rows64(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                            const T *__restrict__ val, const T *__restrict__ xb, int ld, int lc,
+                                            int r, int end, bool lane_ok, Out &&out)
+{
+    constexpr int C = kRowPad;
+    const T *__restrict__ xl = xb + (lane_ok ? lc : 0);
+    int p[RB + 1];
+#pragma unroll
+    for (int q = 0; q <= RB; ++q) p[q] = rowptr[min(r + q, end)];
+    int cc[RB][C];
+    T vv[RB][C];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            cc[q][i] = col[p[q] + i];
+            vv[q][i] = val[p[q] + i];
+        }
+    }
+    T xv[RB][C];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        const int c0 = p[q] < p[q + 1] ? cc[q][0] : 0;
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const int cj = p[q] + i < p[q + 1] ? cc[q][i] : c0;
+            xv[q][i] = xl[(size_t)cj * ld];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        const int c0 = p[q] < p[q + 1] ? cc[q][0] : 0;
+        T acc = T(0);
+#pragma unroll
+        for (int i = 0; i < C; ++i)
+            if (p[q] + i < p[q + 1]) acc += vv[q][i] * xv[q][i];
+        for (int j0 = p[q] + C; j0 < p[q + 1]; j0 += C) {
+            T cv[C], cx[C];
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                const int cj = j0 + i < p[q + 1] ? col[j0 + i] : c0;
+                cv[i] = val[j0 + i];
+                cx[i] = xl[(size_t)cj * ld];
+            }
+#pragma unroll
+            for (int i = 0; i < C; ++i)
+                if (j0 + i < p[q + 1]) acc += cv[i] * cx[i];
+        }
+        out(q, acc);
+    }
+}
+
 __device__ __forceinline__ void set_done(SolverState &s, int stop)
 {
     s.done = 1;

@@ -17,6 +17,7 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kWideThreads = 512;      // SpMV workgroup size for row blocks above 2048 entries
 constexpr int kMaxBlockNnz = 16384;    // upper bound of nnz per SpMV row block (128 KiB of FP64 LDS)
 constexpr int kLongRow = 1024;         // rows longer than this are summed in chunks (k_chunks)
+constexpr int kRowPad = 8;             // entries of padding past nnz in col / val / val32
 
 // Threads of the SpMV workgroups that sum row blocks of block_nnz entries.
 constexpr int spmv_threads(int block_nnz) { return block_nnz > 2048 ? kWideThreads : kThreads; }
