@@ -20,7 +20,7 @@ namespace {
 thread_local std::string g_err;
 int g_block_nnz = 0;      // 0: by average row length (defaults from sweeps on MI355X)
 int g_serial_max_avg = 64;
-int g_max_groups = kMaxGroups;
+int g_max_groups = 0;     // 0: kMaxGroups, or the resident workgroups of the 512-thread SpMV
 int g_stream_nt = 0;
 int g_spmv_npt = 0;       // entries per thread per pass of 512-thread SpMV blocks (0: 4)
 int g_sort_pct = 70;      // kind-2 blocks: sorted gathers touch <= 70% as many lines as entries
@@ -271,7 +271,7 @@ extern "C" int hbs_set_spmv_plan(int block_nnz, int serial_max_avg)
 
 extern "C" int hbs_set_spmv_groups(int max_groups)
 {
-    if (max_groups < 1 || max_groups > 65536) return fail(HBS_ERR_ARG, "max_groups must be in [1, 65536]");
+    if (max_groups < 0 || max_groups > 65536) return fail(HBS_ERR_ARG, "max_groups must be in [0, 65536]");
     g_max_groups = max_groups;
     return HBS_OK;
 }
@@ -411,13 +411,28 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
         a->sys_rows.assign(sys_rows, sys_rows + nsys + 1);
     }
     a->nsys = nsys;
+    const int npt = spmv_threads(block_nnz) == kThreads ? (block_nnz <= 4 * kThreads ? 4 : 8)
+                                                        : (g_spmv_npt ? g_spmv_npt : 4);
+    if (prepare_spmv(block_nnz, npt) != hipSuccess)
+        return bail(fail(HBS_ERR_HIP, "SpMV LDS limit for %d-entry row blocks not granted", block_nnz));
+    // SpMV workgroups: 512-thread ones as many as are resident at once, each
+    // looping over its run of blocks with the next block's loads in flight;
+    // 256-thread ones up to kMaxGroups
+    int max_groups = g_max_groups;
+    if (max_groups == 0) {
+        max_groups = kMaxGroups;
+        if (spmv_threads(block_nnz) == kWideThreads) {
+            const int res = spmv_resident_groups(block_nnz, npt, ctx->device);
+            if (res > 0) max_groups = std::min(max_groups, res);
+        }
+    }
     for (int q = 0; q < nsys; ++q) {
         const int r0 = a->sys_rows[(size_t)q], r1 = a->sys_rows[(size_t)q + 1];
         const int rb0 = (int)hp.blocks.size(), g0 = (int)hp.groups.size();
         plan_rows(r0, r1 - r0, rowptr, block_nnz, g_serial_max_avg, q, hp);
         const long long nz = (long long)rowptr[r1] - rowptr[r0];
-        const int want = nsys == 1 ? g_max_groups
-                                   : (int)std::max(1LL, nnz > 0 ? (long long)g_max_groups * nz / nnz : 1LL);
+        const int want = nsys == 1 ? max_groups
+                                   : (int)std::max(1LL, nnz > 0 ? (long long)max_groups * nz / nnz : 1LL);
         plan_groups(rb0, (int)hp.blocks.size(), want, q, hp);
         ranges.push_back(SysRange{g0, (int)hp.groups.size(), 0, 0});
     }
@@ -454,10 +469,6 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
                 return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
         }
     }
-    const int npt = spmv_threads(block_nnz) == kThreads ? (block_nnz <= 4 * kThreads ? 4 : 8)
-                                                        : (g_spmv_npt ? g_spmv_npt : 4);
-    if (prepare_spmv(block_nnz, npt) != hipSuccess)
-        return bail(fail(HBS_ERR_HIP, "SpMV LDS limit for %d-entry row blocks not granted", block_nnz));
     SpmvPlan &p = a->plan;
     p.nsys = nsys;
     a->stream_nt = g_stream_nt;

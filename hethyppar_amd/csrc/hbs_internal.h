@@ -237,6 +237,9 @@ int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 // entries, npt entries per thread per pass (once per plan, outside any
 // stream capture).
 hipError_t prepare_spmv(int block_nnz, int npt);
+// FP64 SpMV workgroups of row blocks of block_nnz entries (npt per thread per
+// pass) resident at once on the device (0 when the query fails).
+int spmv_resident_groups(int block_nnz, int npt, int device);
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, int npt, hipStream_t s);
 hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, int npt, hipStream_t s);
 hipError_t launch_chunks(const ChunkArgs<double> &a, int block_nnz, hipStream_t s);

@@ -765,6 +765,20 @@ hipError_t prepare_spmv(int block_nnz, int npt)
     }
     return e;
 }
+int spmv_resident_groups(int block_nnz, int npt, int device)
+{
+    const int th = spmv_threads(block_nnz);
+    const size_t shmem = (size_t)block_nnz * sizeof(double) +
+                         (th == kWideThreads ? rowptr_lds_bytes<kWideThreads>() : rowptr_lds_bytes<kThreads>());
+    const void *f = th == kThreads ? (block_nnz <= 4 * kThreads ? (const void *)k_spmv<double, EPI_DOT2, 4, kThreads>
+                                                                 : (const void *)k_spmv<double, EPI_DOT2, 8, kThreads>)
+                                   : (npt == 4 ? (const void *)k_spmv<double, EPI_DOT2, 4, kWideThreads>
+                                               : (const void *)k_spmv<double, EPI_DOT2, 8, kWideThreads>);
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, th, shmem) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    return per_cu * cus;
+}
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, int npt, hipStream_t s)
 {
     return spmv_dispatch<double>(a, epi, block_nnz, npt, s);

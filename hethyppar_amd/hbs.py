@@ -685,7 +685,7 @@ class Shard:
             pass
 
 
-def set_spmv_plan(block_nnz=0, serial_max_avg=64, max_groups=4096, stream_nt=0, npt=0):
+def set_spmv_plan(block_nnz=0, serial_max_avg=64, max_groups=0, stream_nt=0, npt=0):
     """Process-wide plan knobs used by DeviceCSR objects created afterwards."""
     check(lib().hbs_set_spmv_plan(block_nnz, serial_max_avg))
     check(lib().hbs_set_spmv_groups(max_groups))

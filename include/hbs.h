@@ -302,14 +302,15 @@ int hbs_bench_copy(hbs_ctx ctx, const double *src_dev, double *dst_dev, size_t n
                    int reps, double *avg_ms);
 
 /* ---- tuning knobs (process-wide, for sweeps) -------------------------- */
-/* nnz per row block (a multiple of 256, <= 4096; 0 = by the matrix: 1024
- * under 10 entries per row on average, else 2048) and the average row
- * length above which a block uses several lanes per row instead of the
- * order-exact serial sum. */
+/* nnz per row block (a multiple of 256, <= 16384; 0 = by the matrix: 1024
+ * under 10 entries per row on average, else nnz / 1024 within [2048,
+ * 6144]) and the average row length above which a block uses several lanes
+ * per row instead of the order-exact serial sum. */
 int hbs_set_spmv_plan(int block_nnz, int serial_max_avg);
-/* Cap on the SpMV workgroups of matrices created afterwards (1..65536,
- * default 4096); each takes a contiguous run of row blocks, and its dot
- * partial is one of those a consumer sums. */
+/* Cap on the SpMV workgroups of matrices created afterwards (1..65536; 0,
+ * the default: 4096, or for row blocks above 2048 entries as many as are
+ * resident on the device at once); each takes a contiguous run of row
+ * blocks, and its dot partial is one of those a consumer sums. */
 int hbs_set_spmv_groups(int max_groups);
 /* Row blocks of matrices created afterwards whose entries, taken in column
  * order, touch at most max_line_pct percent as many 128-byte lines of x as
