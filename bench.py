@@ -103,7 +103,7 @@ def main():
     S = hbs.Solver(A)
     db = ctx.to_device(b)
     dx = ctx.empty(n)
-    opts = hbs.default_opts(tol=1e-8, maxit=1000, use_graph=1, check_every=args.check_every)
+    opts = hbs.default_opts(tol=1e-8, maxit=1000, use_graph=2, check_every=args.check_every)
 
     def solve():
         dx.zero()
@@ -268,7 +268,7 @@ def extras_batched(hbs, ctx):
     A = hbs.DeviceCSR(ctx, rp, c, v, sys_rows=rows)
     S = hbs.Solver(A)
     db, dx = ctx.to_device(b), ctx.empty(n)
-    o = hbs.default_opts(tol=1e-8, maxit=5000, use_graph=1, check_every=8)
+    o = hbs.default_opts(tol=1e-8, maxit=5000, use_graph=2, check_every=8)
 
     def systems():
         dx.zero()
@@ -298,7 +298,7 @@ def extras_adjoint(hbs, ctx):
     S = hbs.Solver(A)
     db, dx, dxt = ctx.to_device(b), ctx.empty(n), ctx.to_device(xt)
     dg, dl, dgv = ctx.empty(n), ctx.empty(n), ctx.empty(nnz)
-    o = hbs.default_opts(tol=1e-8, maxit=2000, use_graph=1, check_every=4)
+    o = hbs.default_opts(tol=1e-8, maxit=2000, use_graph=2, check_every=4)
 
     def forward():
         dx.zero()
@@ -360,7 +360,7 @@ def extras(hbs, ctx, rp, c, v, n):
     res = {}
     for mode in ("fp64", "mixed"):
         fn = S3.bicgstab if mode == "fp64" else S3.bicgstab_mixed
-        o = hbs.default_opts(tol=1e-8, maxit=5000, use_graph=1, check_every=8)
+        o = hbs.default_opts(tol=1e-8, maxit=5000, use_graph=2, check_every=8)
         dx3.zero()
         fn(db3, dx3, o, raise_on_error=False)
         ts = []

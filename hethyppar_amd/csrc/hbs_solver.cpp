@@ -29,7 +29,7 @@ extern "C" void hbs_solve_opts_default(hbs_solve_opts *o)
     o->maxit = 1000;
     o->fixed_iters = 0;
     o->check_every = 4;
-    o->use_graph = 1;
+    o->use_graph = 2;
     o->inner_maxit = 1000;
     o->inner_tol = 1e-4;
     o->max_sweeps = 8;
@@ -253,6 +253,16 @@ bool all_done(const SolverState *h, int nsys)
     return true;
 }
 
+// Whether iteration chunks replay captured graphs.  Auto (2): from 4M
+// entries an iteration's kernels run long enough (~75 us at 8.5M on
+// MI355X) that launching them one by one keeps the GPU fed, and each graph
+// launch would add ~13 us between chunks; smaller matrices replay graphs.
+bool graphs(const hbs_csr a, int use_graph)
+{
+    if (use_graph == 2) return (long long)a->nnz < 4000000LL;
+    return use_graph != 0;
+}
+
 // Queue iteration chunks until the device states say every system is done.
 // One chunk is kept queued ahead of the one whose state the host reads, so
 // the GPU never idles on the poll -- until the residual history of a single
@@ -414,7 +424,7 @@ int solve_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts &o,
         return HBS_OK;
     };
     bool tail_done = false;
-    CHK(iterate<double>(s, w, std::max(o.maxit, 1), o.check_every, o.use_graph != 0, tail, &tail_done));
+    CHK(iterate<double>(s, w, std::max(o.maxit, 1), o.check_every, graphs(s->A, o.use_graph), tail, &tail_done));
     if (!tail_done) CHK(tail());
     std::vector<SolverState> hs((size_t)s->nsys);
     CHK(read_states(s, kSlotA, hs.data()));
@@ -535,7 +545,7 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
         va.st_out = s->slot(kSlotA);
         va.part_out = s->partD;
         HIPCHK(launch_ir_start(va, s->R, s->slot(kSlotOuter), q));
-        CHK(iterate<float>(s, w32, imax, o.check_every, o.use_graph != 0));
+        CHK(iterate<float>(s, w32, imax, o.check_every, graphs(s->A, o.use_graph)));
         va.st_in = s->slot(kSlotA);
         HIPCHK(launch_ir_update(va, s->X, q));
         HIPCHK(residual(s, s->X, b, s->R, kSlotOuter));

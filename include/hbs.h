@@ -119,7 +119,9 @@ typedef struct {
     int maxit;           /* iteration cap                                    */
     int fixed_iters;     /* 1: run exactly maxit iterations, no stop test    */
     int check_every;     /* iterations per convergence poll (0 = default 4)  */
-    int use_graph;       /* 1: replay hipGraph-captured iteration chunks     */
+    int use_graph;       /* 1: replay hipGraph-captured iteration chunks;    */
+                         /* 0: launch the kernels one by one; 2 (default):   */
+                         /* graphs below 4M matrix entries, else one by one  */
     int inner_maxit;     /* mixed: FP32 inner iteration cap per sweep        */
     double inner_tol;    /* mixed: FP32 inner relative tolerance per sweep   */
     int max_sweeps;      /* mixed: FP64 refinement sweeps cap                */

@@ -40,7 +40,7 @@ def test_convdiff64_fp64(ctx):
     _check(x, info, xo, ito)
 
 
-@pytest.mark.parametrize("graph,every", [(0, 1), (1, 4), (1, 8), (0, 16)])
+@pytest.mark.parametrize("graph,every", [(0, 1), (1, 4), (1, 8), (0, 16), (2, 4)])
 def test_convdiff64_deterministic_any_schedule(ctx, graph, every):
     rp, c, v, xt, b = convdiff64()
     x_ref, i_ref = _solve(ctx, rp, c, v, b, use_graph=1, check_every=8)
