@@ -319,6 +319,10 @@ __device__ void apply_fin(SolverState &s, int fin, double d0, double d1)
     case FIN_STORE:
         s.out = d0;
         break;
+    case FIN_RESIDB:
+        s.resid2 = d0;
+        s.out = d1;
+        break;
     default:
         break;
     }

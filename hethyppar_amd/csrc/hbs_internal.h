@@ -81,6 +81,7 @@ enum Epi : int {
     EPI_DOT2 = 2,    // y = Ax ; d0 = (y, w), d1 = (y, y)
     EPI_RESID = 3,   // y = w - Ax ; d0 = (y, y)
     EPI_RESID2 = 4,  // y = w - Ax ; d0 = d1 = (y, y)   (start of a solve)
+    EPI_RESIDB = 5,  // y = w - Ax ; d0 = (y, y), d1 = (w, w)   (end of a solve)
 };
 
 enum Fin : int {
@@ -91,6 +92,7 @@ enum Fin : int {
     FIN_S = 4,       // ss = d0: ||s|| stop test
     FIN_RESID = 5,   // resid2 = d0
     FIN_STORE = 6,   // out = d0
+    FIN_RESIDB = 7,  // resid2 = d0, out = d1
 };
 
 struct SpmvPlan {

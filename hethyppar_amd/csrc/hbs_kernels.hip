@@ -39,7 +39,7 @@ namespace {
 
 template <int EPI>
 struct EpiDots {
-    static constexpr int n = (EPI == EPI_DOT2 || EPI == EPI_RESID2) ? 2 : (EPI == EPI_AXPBY ? 0 : 1);
+    static constexpr int n = (EPI == EPI_DOT2 || EPI == EPI_RESID2 || EPI == EPI_RESIDB) ? 2 : (EPI == EPI_AXPBY ? 0 : 1);
 };
 
 // Row epilogue: write y[r] and feed the fused dots (wr = w[r], loaded by the
@@ -63,6 +63,7 @@ __device__ __forceinline__ void row_out(const SpmvArgs<T> &a, int r, T acc, T wr
         a.y[r] = out;
         d0.add(out, out);
         if constexpr (EPI == EPI_RESID2) d1.add(out, out);
+        if constexpr (EPI == EPI_RESIDB) d1.add(wr, wr);
     }
 }
 
@@ -571,11 +572,17 @@ __global__ __launch_bounds__(kThreads) void k_finalize(SolverState *st, const Sy
     const int q = blockIdx.x;
     if (threadIdx.x == 0) sh = st[q];
     const SysRange sr = sys[q];
-    double v[1];
-    if (in_spmv) reduce_parts<1>(part, nparts, sr.g0, sr.g1, v, red);
-    else reduce_parts<1>(part, nparts, sr.v0, sr.v1, v, red);
+    const int p0 = in_spmv ? sr.g0 : sr.v0, p1 = in_spmv ? sr.g1 : sr.v1;
+    double v[2] = {0.0, 0.0};
+    if (fin == FIN_RESIDB) {
+        reduce_parts<2>(part, nparts, p0, p1, v, red);
+    } else {
+        double v1[1];
+        reduce_parts<1>(part, nparts, p0, p1, v1, red);
+        v[0] = v1[0];
+    }
     if (threadIdx.x == 0) {
-        apply_fin(sh, fin, v[0], 0.0);
+        apply_fin(sh, fin, v[0], v[1]);
         st[q] = sh;
     }
 }
@@ -642,11 +649,13 @@ __global__ __launch_bounds__(kThreads) void k_dot(VecArgs<double> a, const doubl
     write_parts<1, double>(a, d0, d1, red);
 }
 
-// Start of a solve: X = x (grid-stride) and, in workgroup 0, the states.
+// Start of a solve: X = x (grid-stride; none when x is null) and, in
+// workgroup 0, the states.
 __global__ __launch_bounds__(kThreads) void k_solve_start(const double *x, double *X, int n, SolverState *st,
                                                          int nstates, double tol, int maxit, int fixed)
 {
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) X[i] = x[i];
+    if (x)
+        for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) X[i] = x[i];
     if (blockIdx.x != 0) return;
     for (int q = threadIdx.x; q < nstates; q += blockDim.x) {
         SolverState z{};
@@ -685,6 +694,7 @@ hipError_t spmv_epi(const SpmvArgs<T> &a, int epi, size_t shmem, hipStream_t s)
     case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
     case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
     case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESIDB: k_spmv<T, EPI_RESIDB, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -698,7 +708,7 @@ hipError_t spmv_allow_lds(size_t shmem)
     hipError_t e = hipSuccess;
     const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH>, (const void *)k_spmv<T, EPI_DOT1, NPT, TH>,
                        (const void *)k_spmv<T, EPI_DOT2, NPT, TH>, (const void *)k_spmv<T, EPI_RESID, NPT, TH>,
-                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH>};
+                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH>, (const void *)k_spmv<T, EPI_RESIDB, NPT, TH>};
     for (const void *f : k)
         if (e == hipSuccess) e = hipFuncSetAttribute(f, at, (int)shmem);
     return e;
@@ -844,7 +854,7 @@ hipError_t launch_dot(const VecArgs<double> &a, const double *x, const double *y
 hipError_t launch_solve_start(const double *x, double *X, int n, SolverState *st, int nstates, double tol,
                               int maxit, int fixed, hipStream_t s)
 {
-    const int grid = std::max(1, std::min(2048, (n + kThreads * 4 - 1) / (kThreads * 4)));
+    const int grid = x ? std::max(1, std::min(2048, (n + kThreads * 4 - 1) / (kThreads * 4))) : 1;
     k_solve_start<<<grid, kThreads, 0, s>>>(x, X, n, st, nstates, tol, maxit, fixed);
     return hipGetLastError();
 }

@@ -367,6 +367,21 @@ hipError_t residual(hbs_solver s, const double *x, const double *b, double *r, i
     return launch_finalize(s->slot(k), s->sys, s->nsys, s->partE, s->A->plan.ngroups, 1, FIN_RESID, q);
 }
 
+// r = b - A x (FP64) with ||r||^2 into resid2 and ||b||^2 into out of slot
+// k (every system), in one pass.
+hipError_t residual_b(hbs_solver s, const double *x, const double *b, double *r, int k)
+{
+    hipStream_t q = s->A->ctx->stream;
+    SpmvArgs<double> sa = solver_spmv<double>(s, s->A->val);
+    sa.x = x;
+    sa.y = r;
+    sa.w = b;
+    sa.part_out = s->partE;
+    hipError_t e = run_spmv(sa, s->A, EPI_RESIDB, nullptr, q);
+    if (e != hipSuccess) return e;
+    return launch_finalize(s->slot(k), s->sys, s->nsys, s->partE, s->A->plan.ngroups, 1, FIN_RESIDB, q);
+}
+
 // (b, b) into out of slot k (every system).
 hipError_t bnorm2(hbs_solver s, const double *b, int k)
 {
@@ -396,14 +411,21 @@ int solve_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts &o,
     hipStream_t q = A->ctx->stream;
     const int n = s->n;
     Work<double> w = work64(s);
+    // Without graphs (whose captured launches keep the workspace pointers)
+    // the iterate lives in the caller's x itself when its packs are 16-byte
+    // aligned and it is not b: no copies in and out.
+    const bool use_graphs = graphs(A, o.use_graph);
+    const bool direct = !use_graphs && ((uintptr_t)x % 16) == 0 && x != b;
+    if (direct) w.X = x;
 
     HIPCHK(hipEventRecord(s->e0, q));
-    HIPCHK(launch_solve_start(x, s->X, n, s->st, 2 * s->nsys, o.tol, o.maxit, o.fixed_iters ? 1 : 0, q));
+    HIPCHK(launch_solve_start(direct ? nullptr : x, w.X, n, s->st, 2 * s->nsys, o.tol, o.maxit,
+                              o.fixed_iters ? 1 : 0, q));
     // R = B - A X ; R0 = R ; P = V = 0 ; (R, R0) = ||R||^2 pending in slot A
     // (bicgstab_cpu.c:176-188)
     {
         SpmvArgs<double> sa = solver_spmv<double>(s, A->val);
-        sa.x = s->X;
+        sa.x = w.X;
         sa.y = s->R;
         sa.w = b;
         sa.part_out = s->partE;
@@ -417,14 +439,13 @@ int solve_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts &o,
     // into S (dead between iterations), ||B||^2, x = X; queued by iterate
     // behind the chunk predicted to be the last
     auto tail = [&]() -> int {
-        HIPCHK(residual(s, s->X, b, s->S, kSlotA));
-        HIPCHK(bnorm2(s, b, kSlotA));
-        HIPCHK(hipMemcpyAsync(x, s->X, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
+        HIPCHK(residual_b(s, w.X, b, s->S, kSlotA));
+        if (!direct) HIPCHK(hipMemcpyAsync(x, w.X, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
         HIPCHK(hipEventRecord(s->e1, q));
         return HBS_OK;
     };
     bool tail_done = false;
-    CHK(iterate<double>(s, w, std::max(o.maxit, 1), o.check_every, graphs(s->A, o.use_graph), tail, &tail_done));
+    CHK(iterate<double>(s, w, std::max(o.maxit, 1), o.check_every, use_graphs, tail, &tail_done));
     if (!tail_done) CHK(tail());
     std::vector<SolverState> hs((size_t)s->nsys);
     CHK(read_states(s, kSlotA, hs.data()));
