@@ -254,9 +254,10 @@ hipError_t launch_vec_S(const VecArgs<float> &a, hipStream_t s);
 hipError_t launch_vec_XR(const VecArgs<float> &a, hipStream_t s);
 // Apply the pending FIN_XR of every system in place (one workgroup per
 // system) and copy the states to mirror (device view of host memory; may be
-// null).
+// null); a nonzero stamp is then written into each mirrored state's pad,
+// after a system-scope fence.
 hipError_t launch_apply_xr(SolverState *st, const SysRange *sys, int nsys, const double *part, int nvb,
-                           SolverState *mirror, hipStream_t s);
+                           SolverState *mirror, int stamp, hipStream_t s);
 // Reduce partials into st[sys] with fin (FIN_RESID / FIN_STORE), one workgroup per system.
 hipError_t launch_finalize(SolverState *st, const SysRange *sys, int nsys, const double *part,
                            int nparts, int in_spmv, int fin, hipStream_t s);

@@ -545,7 +545,7 @@ __global__ __launch_bounds__(kThreads) void k_vec_XR(VecArgs<T> a)
 // of iterations, so the host sees the state of the last iteration), and
 // store the state in the host-mapped mirror the host polls (if any).
 __global__ __launch_bounds__(kThreads) void k_apply_xr(SolverState *st, const SysRange *sys, const double *part,
-                                                      int nvb, SolverState *mirror)
+                                                      int nvb, SolverState *mirror, int stamp)
 {
     __shared__ double red[kMaxDots * kWaves];
     __shared__ SolverState sh;
@@ -561,7 +561,15 @@ __global__ __launch_bounds__(kThreads) void k_apply_xr(SolverState *st, const Sy
             st[q] = sh;
         }
     }
-    if (mirror && threadIdx.x == 0) mirror[q] = sh;
+    if (mirror && threadIdx.x == 0) {
+        mirror[q] = sh;
+        if (stamp) {
+            // the stamp lands after the state: a host that reads the stamp
+            // then the state sees this chunk's state
+            __threadfence_system();
+            *reinterpret_cast<volatile int *>(&mirror[q].pad) = stamp;
+        }
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void k_finalize(SolverState *st, const SysRange *sys, const double *part,
@@ -808,10 +816,10 @@ hipError_t launch_vec_S(const VecArgs<float> &a, hipStream_t s) { return k_vec_S
 hipError_t launch_vec_XR(const VecArgs<float> &a, hipStream_t s) { return k_vec_XR_dispatch<float>(a, s); }
 
 hipError_t launch_apply_xr(SolverState *st, const SysRange *sys, int nsys, const double *part, int nvb,
-                           SolverState *mirror, hipStream_t s)
+                           SolverState *mirror, int stamp, hipStream_t s)
 {
     if (nsys <= 0) return hipSuccess;
-    k_apply_xr<<<nsys, kThreads, 0, s>>>(st, sys, part, nvb, mirror);
+    k_apply_xr<<<nsys, kThreads, 0, s>>>(st, sys, part, nvb, mirror, stamp);
     return hipGetLastError();
 }
 

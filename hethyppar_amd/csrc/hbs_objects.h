@@ -141,6 +141,7 @@ struct hbs_solver_s {
     hipGraphExec_t graph[2][kGraphLens][2] = {};       // [FP64, FP32][chunk length][poll slot]
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    int seq = 0;                  // chunks launched without graphs: the mirror's completion stamp
     hbs_solver_s *adj = nullptr;
     hbs::BlockWork *blk = nullptr;
     hbs::SolverState *slot(int k) const { return st + (size_t)k * nsys; }

@@ -10,10 +10,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "hbs_objects.h"
@@ -86,6 +89,7 @@ extern "C" int hbs_solver_create(hbs_csr a, hbs_solver *out)
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void **)&s->hst_dev, s->hst, 0) != hipSuccess)
         return bail(fail(HBS_ERR_ALLOC, "pinned host allocation failed"));
+    std::memset((void *)s->hst, 0, 3 * sizeof(SolverState) * (size_t)s->nsys);   // no stamp yet
     // vector workgroups system by system; one block size for the launch
     std::vector<VecBlock> vb;
     std::vector<SysRange> ranges = a->sys_host;
@@ -203,24 +207,32 @@ hipError_t launch_iteration(hbs_solver s, const Work<T> &w)
 }
 
 // k iterations, then the state the host polls (FIN_XR of the chunk's last
-// iteration applied), mirrored into poll slot `slot` of the host buffer.
+// iteration applied), mirrored into poll slot `slot` of the host buffer and
+// stamped with `stamp` (0: none).
 template <typename T>
-hipError_t launch_chunk(hbs_solver s, const Work<T> &w, int k, int slot)
+hipError_t launch_chunk(hbs_solver s, const Work<T> &w, int k, int slot, int stamp)
 {
     hipError_t e = hipSuccess;
     for (int i = 0; i < k && e == hipSuccess; ++i) e = launch_iteration<T>(s, w);
     if (e != hipSuccess) return e;
     return launch_apply_xr(s->slot(kSlotA), s->sys, s->nsys, s->partD, s->nvb,
-                           s->hst_dev + (size_t)slot * s->nsys, s->A->ctx->stream);
+                           s->hst_dev + (size_t)slot * s->nsys, stamp, s->A->ctx->stream);
 }
 
+// Queue one chunk of k iterations polled through slot `slot`.  Without
+// graphs the chunk's states carry a new stamp, returned in *stamp; with
+// graphs (whose launch arguments are fixed at capture) *stamp is 0 and the
+// host waits on an event instead.
 template <typename T>
-int run_chunk(hbs_solver s, const Work<T> &w, int k, int slot, bool use_graph)
+int run_chunk(hbs_solver s, const Work<T> &w, int k, int slot, bool use_graph, int *stamp)
 {
     hipStream_t q = s->A->ctx->stream;
     const int gi = sizeof(T) == 8 ? 0 : 1;
+    *stamp = 0;
     if (!use_graph) {
-        HIPCHK(launch_chunk<T>(s, w, k, slot));
+        s->seq = s->seq == INT_MAX ? 1 : s->seq + 1;
+        *stamp = s->seq;
+        HIPCHK(launch_chunk<T>(s, w, k, slot, s->seq));
         return HBS_OK;
     }
     // one graph per chunk length and poll slot, kept; longer chunks are captured each time
@@ -230,7 +242,7 @@ int run_chunk(hbs_solver s, const Work<T> &w, int k, int slot, bool use_graph)
     if (!ge) {
         hipGraph_t gr = nullptr;
         HIPCHK(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
-        hipError_t le = launch_chunk<T>(s, w, k, slot);
+        hipError_t le = launch_chunk<T>(s, w, k, slot, 0);
         hipError_t ce = hipStreamEndCapture(q, &gr);
         if (le != hipSuccess) return fail(HBS_ERR_HIP, "launch during capture failed: %s", hipGetErrorString(le));
         if (ce != hipSuccess) return fail(HBS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
@@ -244,6 +256,34 @@ int run_chunk(hbs_solver s, const Work<T> &w, int k, int slot, bool use_graph)
     HIPCHK(hipGraphLaunch(ge, q));
     if (!keep) HIPCHK(hipGraphExecDestroy(ge));
     return HBS_OK;
+}
+
+// Wait until every mirrored state of poll slot `slot` carries `stamp`
+// (written by the chunk's last kernel after the states), checking the
+// stream for errors now and then.
+int wait_stamp(hbs_solver s, int slot, int stamp)
+{
+    const volatile SolverState *h = s->hst + (size_t)slot * s->nsys;
+    for (long spin = 0;; ++spin) {
+        bool all = true;
+        for (int i = 0; i < s->nsys && all; ++i) all = h[i].pad == stamp;
+        if (all) {
+            std::atomic_thread_fence(std::memory_order_acquire);
+            return HBS_OK;
+        }
+        if ((spin & 1023) == 1023) {
+            const hipError_t e = hipStreamQuery(s->A->ctx->stream);
+            if (e != hipSuccess && e != hipErrorNotReady)
+                return fail(HBS_ERR_HIP, "solver stream failed: %s", hipGetErrorString(e));
+            if (e == hipSuccess) {
+                // the stream drained: the stamp must be visible now
+                for (int i = 0; i < s->nsys; ++i)
+                    if (h[i].pad != stamp) return fail(HBS_ERR_HIP, "chunk state stamp %d missing", stamp);
+                return HBS_OK;
+            }
+            std::this_thread::yield();
+        }
+    }
 }
 
 bool all_done(const SolverState *h, int nsys)
@@ -279,14 +319,16 @@ int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use
     const int k = std::max(1, check_every > 0 ? check_every : 4);
     long issued = 0;
     int slot = 0, nfly = 0;
-    int fly_slot[2] = {0, 0}, fly_len[2] = {0, 0};   // queued chunks, oldest first
+    int fly_slot[2] = {0, 0}, fly_len[2] = {0, 0}, fly_stamp[2] = {0, 0};   // queued chunks, oldest first
     bool spec = false;   // tail queued after the last queued chunk
     auto queue = [&](int len) -> int {
         spec = false;
-        CHK(run_chunk<T>(s, w, len, slot, use_graph));
-        HIPCHK(hipEventRecord(s->ev[slot], q));
+        int stamp = 0;
+        CHK(run_chunk<T>(s, w, len, slot, use_graph, &stamp));
+        if (!stamp) HIPCHK(hipEventRecord(s->ev[slot], q));
         fly_slot[nfly] = slot;
         fly_len[nfly] = len;
+        fly_stamp[nfly] = stamp;
         ++nfly;
         slot ^= 1;
         issued += len;
@@ -299,11 +341,13 @@ int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use
     CHK(queue(next));
     for (;;) {
         if (ahead && nfly < 2 && issued < maxit) CHK(queue((int)std::min<long>(next, maxit - issued)));
-        HIPCHK(hipEventSynchronize(s->ev[fly_slot[0]]));
+        if (fly_stamp[0]) CHK(wait_stamp(s, fly_slot[0], fly_stamp[0]));
+        else HIPCHK(hipEventSynchronize(s->ev[fly_slot[0]]));
         const SolverState *h = s->hst + (size_t)fly_slot[0] * s->nsys;
         const bool done = all_done(h, s->nsys);
         fly_slot[0] = fly_slot[1];
         fly_len[0] = fly_len[1];
+        fly_stamp[0] = fly_stamp[1];
         --nfly;
         if (done || (issued >= maxit && nfly == 0)) break;
         ahead = true;
