@@ -128,7 +128,7 @@ def main():
     it = infos[-1].iters
     relres = infos[-1].relres
 
-    # dominant kernel: CSR-stream SpMV, HIP events around each launch
+    # dominant kernel: CSR-stream SpMV, one HIP event pair around 50 back-to-back launches
     log("bench: SpMV roofline")
     dxs, dys = ctx.to_device(xt), ctx.empty(n)
     spmv_ms = A.bench_spmv(dxs, dys, args.spmv_reps)

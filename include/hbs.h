@@ -294,8 +294,8 @@ int hbs_shard_spmv2(hbs_shard s, const double *x_loc_dev, const double *recv_hos
 /* ---- timing helpers for benchmarks (hipEvents on the context stream) -- */
 int hbs_timer_start(hbs_ctx ctx);
 int hbs_timer_stop(hbs_ctx ctx, double *ms);
-/* Average device time of the last `reps` launches of the dominant SpMV
- * kernel recorded with events around each launch. */
+/* Average device time per launch of `reps` back-to-back launches of the
+ * SpMV (after one warm-up), from one event pair around all of them. */
 int hbs_bench_spmv_f64(hbs_csr a, const double *x_dev, double *y_dev, int reps,
                        double *avg_ms);
 int hbs_bench_spmv_f32(hbs_csr a, const float *x_dev, float *y_dev, int reps,
