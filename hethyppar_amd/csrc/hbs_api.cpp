@@ -357,13 +357,17 @@ int validate_csr(int m, int n, int nnz, const int *rowptr, const int *col, const
 // and many of them.  Longer rows gain from big blocks: column-sorted gathers
 // share more x lines.  Up to 6144 entries, three 512-thread workgroups (4
 // entries per thread per pass) fit a CU's LDS and keep their loads in
-// flight.  The block is sized for about 1024 of them, so a small matrix
-// still fills the GPU.
-int default_block_nnz(int m, int nnz)
+// flight.  The blocks are sized so that the resident workgroups (`resident`,
+// 0 if unknown) each get the same whole number of them, with 1% slack for
+// blocks ending at row boundaries, and a small matrix still fills the GPU.
+int default_block_nnz(int m, int nnz, int resident)
 {
     if ((long long)nnz < 10LL * std::max(m, 1)) return 1024;
-    const long long per = (long long)nnz / 1024 / 256 * 256;
-    return (int)std::min<long long>(6144, std::max<long long>(2048, per));
+    if (resident <= 0) resident = 1024;
+    const long long per_round = 6144LL * resident;
+    const long long k = std::max(1LL, ((long long)nnz + per_round - 1) / per_round);
+    const long long bn = ((long long)((double)nnz / (double)(k * resident) * 1.01) + 63) / 64 * 64;
+    return (int)std::min<long long>(6144, std::max<long long>(2048, bn));
 }
 
 int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int *col, const double *val, int flags,
@@ -403,7 +407,8 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     }
     HostPlan hp;
     std::vector<SysRange> ranges;
-    const int block_nnz = g_block_nnz > 0 ? g_block_nnz : default_block_nnz(m, nnz);
+    const int block_nnz =
+        g_block_nnz > 0 ? g_block_nnz : default_block_nnz(m, nnz, spmv_resident_groups(6144, 4, ctx->device));
     if (!sys_rows) {
         a->sys_rows = {0, m};
         nsys = 1;
