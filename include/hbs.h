@@ -305,9 +305,10 @@ int hbs_bench_copy(hbs_ctx ctx, const double *src_dev, double *dst_dev, size_t n
 
 /* ---- tuning knobs (process-wide, for sweeps) -------------------------- */
 /* nnz per row block (a multiple of 256, <= 16384; 0 = by the matrix: 1024
- * under 10 entries per row on average, else nnz / 1024 within [2048,
- * 6144]) and the average row length above which a block uses several lanes
- * per row instead of the order-exact serial sum. */
+ * under 10 entries per row on average, else within [2048, 6144], sized so
+ * the resident SpMV workgroups get equal numbers of blocks) and the average
+ * row length above which a block uses several lanes per row instead of the
+ * order-exact serial sum. */
 int hbs_set_spmv_plan(int block_nnz, int serial_max_avg);
 /* Cap on the SpMV workgroups of matrices created afterwards (1..65536; 0,
  * the default: 4096, or for row blocks above 2048 entries as many as are

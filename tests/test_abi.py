@@ -43,3 +43,14 @@ def test_error_path_without_device():
     assert rc == 0
     rc = hbs.lib().hbs_set_spmv_plan(100, 1)
     assert rc == -1 and b"block_nnz" in hbs.lib().hbs_last_error()
+
+
+def test_plan_knob_validation_without_device():
+    L = hbs.lib()
+    assert L.hbs_set_spmv_plan(16384 + 256, 1) == -1 and b"block_nnz" in L.hbs_last_error()
+    assert L.hbs_set_spmv_plan(16384, 64) == 0
+    assert L.hbs_set_spmv_pass(6) == -1 and b"entries_per_thread" in L.hbs_last_error()
+    assert L.hbs_set_spmv_pass(8) == 0 and L.hbs_set_spmv_pass(0) == 0
+    assert L.hbs_set_spmv_groups(-1) == -1 and b"max_groups" in L.hbs_last_error()
+    assert L.hbs_set_spmv_groups(0) == 0
+    assert L.hbs_set_spmv_plan(0, 64) == 0
