@@ -23,7 +23,7 @@ ARCH = os.environ.get("HBS_ARCH", "gfx950")
 
 KERNELS = ["hbs_kernels.hip", "hbs_aux.hip", "hbs_block.hip"]
 HOST = ["hbs_api.cpp", "hbs_solver.cpp", "hbs_block.cpp", "hbs_shard.cpp", "hbs_plan.cpp", "hbs_layout.cpp", "hbs_gen.cpp",
-        "hhp_hip.cpp"]
+        "hhp_hip.cpp", "hbs_part.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -69,7 +69,7 @@ def build(verbose=False):
     return LIB
 
 
-CLI = ["hbs_bicgstab_gpu", "hbs_spmv_gpu"]
+CLI = ["hbs_bicgstab_gpu", "hbs_spmv_gpu", "hbs_patpart"]
 BIN = os.path.join(HERE, "bin")
 
 

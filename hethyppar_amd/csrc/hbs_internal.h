@@ -206,6 +206,14 @@ hipError_t launch_block_vec(const BlockArgs &a, int kind, hipStream_t s);
 // Reduce the partials of every column into st[c] with fin (one workgroup per column).
 hipError_t launch_block_fin(SolverState *st, int kc, const double *part, int ngroups, int fin, hipStream_t s);
 
+// ---- row partitioner (hbs_part.cpp) ----
+// partvec[n] for a square CSR: nparts parts with weights proportional to
+// target[nparts], within 1 + final_imbal; see hbs_partition.
+int partition_rows(int n, const int *rowptr, const int *col, int nparts, const double *target, double final_imbal,
+                   int seed, int *partvec);
+// Connectivity - 1 of partvec over the rows of the CSR.
+long long partition_cut(int n, const int *rowptr, const int *col, const int *partvec);
+
 // ---- host-side planning (hbs_plan.cpp) ----
 struct HostPlan {
     std::vector<RowBlock> blocks;

@@ -120,6 +120,7 @@ _PROTOS = {
     "hbs_gen_uniform": (_i, [_i, _d, _d, C.c_uint64, _dp]),
     "hbs_host_csr_spmv": (_i, [_i, _ip, _ip, _dp, _dp, _dp]),
     "hbs_shard_build_host": (_i, [_i, _ip, _ip, _dp, _ip, _i, _i, C.POINTER(HostShard)]),
+    "hbs_partition": (_i, [_i, _ip, _ip, _i, _dp, C.c_double, _i, _ip, C.POINTER(C.c_longlong)]),
     "hbs_host_shard_free": (None, [C.POINTER(HostShard)]),
     "hbs_shard_create": (_i, [_vp, C.POINTER(HostShard), C.POINTER(_vp)]),
     "hbs_shard_destroy": (_i, [_vp]),
@@ -316,6 +317,23 @@ def host_spmv(rowptr, col, val, x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     check(lib().hbs_host_csr_spmv(m, _ip_of(rowptr), _ip_of(col), _dp_of(val), _dp_of(x), _dp_of(y)))
     return y
+
+
+def partition(rowptr, col, nparts, target=None, final_imbal=0.03, seed=42):
+    """Row partition vector of a square CSR for the shard build
+    (hbs_partition: the CalcPartVec model, this library's own partitioner).
+    Returns (partvec int32[n], connectivity - 1 cut)."""
+    n = rowptr.size - 1
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    pv = np.zeros(n, dtype=np.int32)
+    cut = C.c_longlong()
+    t = None if target is None else np.ascontiguousarray(target, dtype=np.float64)
+    if t is not None and t.size != nparts:
+        raise ValueError("target needs one weight per part")
+    check(lib().hbs_partition(n, _ip_of(rowptr), _ip_of(col), nparts, None if t is None else _dp_of(t),
+                              float(final_imbal), int(seed), _ip_of(pv), C.byref(cut)))
+    return pv, cut.value
 
 
 def shard_build(rowptr, col, val, partvec, nparts, rank):

@@ -250,6 +250,20 @@ int hbs_gen_uniform(int n, double lo, double hi, uint64_t seed, double *out);
 int hbs_host_csr_spmv(int m, const int *rowptr, const int *col, const double *val,
                       const double *x, double *y);
 
+/* ---- row partition --------------------------------------------------- */
+/* Partition vector for the shard build: partvec[r] in [0, nparts) for each
+ * row of a square CSR (n x n), replacing CalcPartVec (hhp_matrix.c:225-300),
+ * which calls the closed PaToH library.  Same model: the rows are nets over
+ * their columns, cell r (column and row r) weighs row r's entries, and the
+ * metric is connectivity - 1 (PATOH_CONPART), the x values the rows' owners
+ * must receive.  Part weights follow target[nparts] (relative; NULL = equal)
+ * within a factor 1 + final_imbal; `seed` fixes the result.  The algorithm
+ * is this library's own multilevel recursive bisection, not PaToH's, so
+ * partitions differ from PaToH's.  *cut (may be NULL) receives the
+ * connectivity - 1 of the result. */
+int hbs_partition(int n, const int *rowptr, const int *col, int nparts, const double *target,
+                  double final_imbal, int seed, int *partvec, long long *cut);
+
 /* ---- one rank's share: shard build, halo pack/unpack, shard SpMV ------ */
 /* Builds rank `rank`'s SHARD_CSC (loc, shr, gind, send, recv) from the global
  * CSR and partition vector in one process; bit-identical to the reference's
