@@ -7,6 +7,7 @@
 // direct construction of one rank's lists without the MPI round trip.
 #include <algorithm>
 #include <cctype>
+#include <climits>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -55,6 +56,27 @@ T *hmalloc(size_t n)
     return static_cast<T *>(std::malloc(sizeof(T) * (n ? n : 1)));
 }
 
+constexpr int kMaxThreads = 64;   // host threads of the parallel builders
+
+// Split items [0, n) into T contiguous runs of about equal weight, given the
+// weight prefix pre[0..n] (pre[i] = weight of the items before i).  Returns
+// the T + 1 run bounds.
+template <typename W>
+std::vector<int> runs(const W *pre, int n, int T)
+{
+    std::vector<int> b((size_t)T + 1, n);
+    b[0] = 0;
+    const double total = (double)(pre[n] - pre[0]);
+    for (int t = 1; t < T; ++t) {
+        const double want = (double)pre[0] + total * t / T;
+        b[(size_t)t] = (int)(std::lower_bound(pre, pre + n + 1, want,
+                                              [](W a, double w) { return (double)a < w; }) -
+                             pre);
+        b[(size_t)t] = std::min(std::max(b[(size_t)t], b[(size_t)t - 1]), n);
+    }
+    return b;
+}
+
 // Parse one line "r c v" of [p, e) into (r-1, c-1, v); false unless the line
 // holds exactly those three numbers within range.
 bool parse_line(const char *p, const char *e, long M, long N, int &r0, int &c0, double &v)
@@ -86,7 +108,7 @@ bool parse_triplets_parallel(const char *p, const char *end, long M, long N, lon
 {
     const size_t len = (size_t)(end - p);
     int T = omp_get_max_threads();
-    if (T > 64) T = 64;
+    if (T > kMaxThreads) T = kMaxThreads;
     if (T < 2 || NZ < 100000 || len < ((size_t)1 << 20)) return false;
     std::vector<const char *> cut((size_t)T + 1);
     cut[0] = p;
@@ -330,10 +352,15 @@ extern "C" void hbs_host_shard_free(hbs_host_shard *s)
 //             columns that q's rows reference, ascending -- the list the
 //             reference obtains by receiving q's recv block and mapping it
 //             through locmap (:1167-1232).
-// Built directly: one pass over all rows collects (peer, local column) pairs,
-// sorted and de-duplicated.  Errors the reference aborts on come back as
-// codes: a partition id out of range (:840-842) or a rank without rows or
-// entries (hhp_cpu.c:19-20, hhp_matrix.c:1484-1486) -> HBS_ERR_PARTITION.
+// Built directly, without the MPI round trip, by the host's threads: each
+// thread takes a contiguous run of rows with about equal entries.  The CSC is
+// a counting sort whose per-thread column offsets follow the thread order, so
+// rows stay ascending inside every column whatever the thread count; the send
+// lists come from per-peer marks over the owned columns, read back in column
+// order.  Errors
+// the reference aborts on come back as codes: a partition id out of range
+// (:840-842) or a rank without rows or entries (hhp_cpu.c:19-20,
+// hhp_matrix.c:1484-1486) -> HBS_ERR_PARTITION.
 extern "C" int hbs_shard_build_host(int n, const int *rowptr, const int *col, const double *val,
                                     const int *partvec, int nparts, int rank, hbs_host_shard *out)
 {
@@ -341,8 +368,11 @@ extern "C" int hbs_shard_build_host(int n, const int *rowptr, const int *col, co
     std::memset(out, 0, sizeof *out);
     if (n <= 0 || !rowptr || !col || !val || !partvec || nparts <= 0 || rank < 0 || rank >= nparts)
         return lfail(HBS_ERR_ARG);
-    for (int i = 0; i < n; ++i)
-        if (partvec[i] < 0 || partvec[i] >= nparts) return lfail(HBS_ERR_PARTITION);
+    const int T = std::max(1, std::min(omp_get_max_threads(), kMaxThreads));
+    int bad = 0;
+#pragma omp parallel for num_threads(T) reduction(| : bad) schedule(static)
+    for (int i = 0; i < n; ++i) bad |= partvec[i] < 0 || partvec[i] >= nparts;
+    if (bad) return lfail(HBS_ERR_PARTITION);
     std::vector<int> locmap((size_t)n, -1), owned;
     for (int i = 0; i < n; ++i)
         if (partvec[i] == rank) {
@@ -351,30 +381,39 @@ extern "C" int hbs_shard_build_host(int n, const int *rowptr, const int *col, co
         }
     const int m_loc = (int)owned.size();
     if (m_loc == 0) return lfail(HBS_ERR_PARTITION);
+    // owned rows in T runs of about equal entries (opre: entries before each)
+    std::vector<long> opre((size_t)m_loc + 1, 0);
+    for (int lr = 0; lr < m_loc; ++lr)
+        opre[(size_t)lr + 1] = opre[(size_t)lr] + rowptr[owned[(size_t)lr] + 1] - rowptr[owned[(size_t)lr]];
+    const std::vector<int> orun = runs(opre.data(), m_loc, T);
     // shr columns: non-owned columns referenced by owned rows
-    std::vector<int> shrmap((size_t)n, -1);
     long nnz_loc = 0, nnz_shr = 0;
     std::vector<char> used((size_t)n, 0);
-    for (int lr = 0; lr < m_loc; ++lr) {
-        const int r = owned[(size_t)lr];
-        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-            const int c = col[j];
-            if (locmap[(size_t)c] >= 0) {
-                ++nnz_loc;
-            } else {
-                ++nnz_shr;
-                used[(size_t)c] = 1;
+#pragma omp parallel for num_threads(T) reduction(+ : nnz_loc, nnz_shr) schedule(static, 1)
+    for (int t = 0; t < T; ++t)
+        for (int lr = orun[(size_t)t]; lr < orun[(size_t)t + 1]; ++lr) {
+            const int r = owned[(size_t)lr];
+            for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+                const int c = col[j];
+                if (locmap[(size_t)c] >= 0) {
+                    ++nnz_loc;
+                } else {
+                    ++nnz_shr;
+                    __atomic_store_n(&used[(size_t)c], (char)1, __ATOMIC_RELAXED);
+                }
             }
         }
-    }
     if (nnz_loc + nnz_shr == 0) return lfail(HBS_ERR_PARTITION);
-    std::vector<int> shrcols;
+    if (nnz_loc > INT_MAX || nnz_shr > INT_MAX) return lfail(HBS_ERR_ARG);
+    // cmap: column -> CSC column (loc columns first, then shr)
+    std::vector<int> cmap(locmap), shrcols;
     for (int c = 0; c < n; ++c)
         if (used[(size_t)c]) {
-            shrmap[(size_t)c] = (int)shrcols.size();
+            cmap[(size_t)c] = m_loc + (int)shrcols.size();
             shrcols.push_back(c);
         }
     const int n_shr = (int)shrcols.size();
+    const int ncols = m_loc + n_shr;
 
     hbs_host_shard &s = *out;
     s.loc.m = m_loc;
@@ -389,45 +428,61 @@ extern "C" int hbs_shard_build_host(int n, const int *rowptr, const int *col, co
     s.shr.I = hmalloc<int>((size_t)n_shr + 1);
     s.shr.J = hmalloc<int>((size_t)nnz_shr);
     s.shr.val = hmalloc<double>((size_t)nnz_shr);
-    s.gind.nvals = (unsigned)(m_loc + n_shr);
-    s.gind.vals = hmalloc<int>((size_t)m_loc + n_shr);
+    s.gind.nvals = (unsigned)ncols;
+    s.gind.vals = hmalloc<int>((size_t)ncols);
     if (!s.loc.I || !s.loc.J || !s.loc.val || !s.shr.I || !s.shr.J || !s.shr.val || !s.gind.vals) {
         hbs_host_shard_free(out);
         return lfail(HBS_ERR_ALLOC);
     }
     std::copy(owned.begin(), owned.end(), s.gind.vals);
     std::copy(shrcols.begin(), shrcols.end(), s.gind.vals + m_loc);
-    // CSC by counting sort over columns; rows visited in ascending order
-    std::fill(s.loc.I, s.loc.I + m_loc + 1, 0);
-    std::fill(s.shr.I, s.shr.I + n_shr + 1, 0);
-    for (int lr = 0; lr < m_loc; ++lr) {
-        const int r = owned[(size_t)lr];
-        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-            const int c = col[j];
-            if (locmap[(size_t)c] >= 0) s.loc.I[locmap[(size_t)c] + 1]++;
-            else s.shr.I[shrmap[(size_t)c] + 1]++;
+    // CSC by a counting sort over the columns: thread t's entries of column k
+    // go after those of threads 0..t-1, whose rows come first
+    std::vector<int> cnt((size_t)T * ncols, 0);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; ++t) {
+        int *ct = cnt.data() + (size_t)t * ncols;
+        for (int lr = orun[(size_t)t]; lr < orun[(size_t)t + 1]; ++lr) {
+            const int r = owned[(size_t)lr];
+            for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) ct[cmap[(size_t)col[j]]]++;
         }
     }
-    for (int c = 0; c < m_loc; ++c) s.loc.I[c + 1] += s.loc.I[c];
-    for (int c = 0; c < n_shr; ++c) s.shr.I[c + 1] += s.shr.I[c];
-    {
-        std::vector<int> fl(s.loc.I, s.loc.I + m_loc), fs(s.shr.I, s.shr.I + n_shr);
-        for (int lr = 0; lr < m_loc; ++lr) {
+    std::vector<int> total((size_t)ncols);
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int k = 0; k < ncols; ++k) {
+        int run = 0;
+        for (int t = 0; t < T; ++t) {
+            int &c = cnt[(size_t)t * ncols + k];
+            const int here = c;
+            c = run;
+            run += here;
+        }
+        total[(size_t)k] = run;
+    }
+    s.loc.I[0] = 0;
+    for (int k = 0; k < m_loc; ++k) s.loc.I[k + 1] = s.loc.I[k] + total[(size_t)k];
+    s.shr.I[0] = 0;
+    for (int k = 0; k < n_shr; ++k) s.shr.I[k + 1] = s.shr.I[k] + total[(size_t)m_loc + k];
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; ++t) {
+        int *ct = cnt.data() + (size_t)t * ncols;
+        for (int lr = orun[(size_t)t]; lr < orun[(size_t)t + 1]; ++lr) {
             const int r = owned[(size_t)lr];
             for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-                const int c = col[j];
-                if (locmap[(size_t)c] >= 0) {
-                    const int d = fl[(size_t)locmap[(size_t)c]]++;
+                const int k = cmap[(size_t)col[j]];
+                if (k < m_loc) {
+                    const int d = s.loc.I[k] + ct[k]++;
                     s.loc.J[d] = lr;
                     s.loc.val[d] = val[j];
                 } else {
-                    const int d = fs[(size_t)shrmap[(size_t)c]]++;
+                    const int d = s.shr.I[k - m_loc] + ct[k]++;
                     s.shr.J[d] = lr;
                     s.shr.val[d] = val[j];
                 }
             }
         }
     }
+    std::vector<int>().swap(cnt);
     // recv: owners ascending; per owner its shr columns ascending
     std::vector<int> ocount((size_t)nparts + 1, 0);
     for (int k = 0; k < n_shr; ++k) ocount[(size_t)partvec[shrcols[(size_t)k]] + 1]++;
@@ -455,40 +510,77 @@ extern "C" int hbs_shard_build_host(int n, const int *rowptr, const int *col, co
         std::vector<int> fo(ocount.begin(), ocount.end() - 1);
         for (int k = 0; k < n_shr; ++k) s.recv.J[fo[(size_t)partvec[shrcols[(size_t)k]]]++] = k;
     }
-    // send: (peer, local column) pairs referenced by the other ranks' rows
-    std::vector<unsigned long long> pairs;
-    for (int r = 0; r < n; ++r) {
-        const int q = partvec[r];
-        if (q == rank) continue;
-        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-            const int lc = locmap[(size_t)col[j]];
-            if (lc >= 0) pairs.push_back(((unsigned long long)(unsigned)q << 32) | (unsigned)lc);
+    // send: the owned columns each peer's rows reference, per peer ascending.
+    // Peers are taken in groups whose marks (one byte per peer and owned
+    // column) fit in about 4n bytes; within a group each thread marks for a
+    // run of the group's rows, and each peer's marks are read back in column
+    // order.
+    std::vector<int> pstart((size_t)nparts + 1, 0), prow((size_t)n);
+    for (int r = 0; r < n; ++r) pstart[(size_t)partvec[r] + 1]++;
+    for (int q = 0; q < nparts; ++q) pstart[(size_t)q + 1] += pstart[(size_t)q];
+    {
+        std::vector<int> fp(pstart.begin(), pstart.end() - 1);
+        for (int r = 0; r < n; ++r) prow[(size_t)fp[(size_t)partvec[r]]++] = r;
+    }
+    const int G = (int)std::max<size_t>(1, std::min<size_t>((size_t)nparts, (size_t)4 * n / (size_t)m_loc));
+    std::vector<char> mark((size_t)G * m_loc);
+    std::vector<int> sendI((size_t)nparts + 1, 0), sendJ;
+    for (int q0 = 0; q0 < nparts; q0 += G) {
+        const int q1 = std::min(nparts, q0 + G), a = pstart[(size_t)q0], nr = pstart[(size_t)q1] - a;
+        std::fill(mark.begin(), mark.begin() + (size_t)(q1 - q0) * m_loc, 0);
+        std::vector<long> gpre((size_t)nr + 1, 0);
+        for (int i = 0; i < nr; ++i) {
+            const int r = prow[(size_t)a + i];
+            gpre[(size_t)i + 1] = gpre[(size_t)i] + rowptr[r + 1] - rowptr[r];
+        }
+        const std::vector<int> grun = runs(gpre.data(), nr, T);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+        for (int t = 0; t < T; ++t)
+            for (int i = grun[(size_t)t]; i < grun[(size_t)t + 1]; ++i) {
+                const int r = prow[(size_t)a + i], q = partvec[r];
+                if (q == rank) continue;
+                char *mq = mark.data() + (size_t)(q - q0) * m_loc;
+                for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+                    const int lc = locmap[(size_t)col[j]];
+                    if (lc >= 0) __atomic_store_n(&mq[lc], (char)1, __ATOMIC_RELAXED);
+                }
+            }
+#pragma omp parallel for num_threads(T) schedule(dynamic, 1)
+        for (int q = q0; q < q1; ++q) {
+            const char *mq = mark.data() + (size_t)(q - q0) * m_loc;
+            int c = 0;
+            for (int lc = 0; lc < m_loc; ++lc) c += mq[lc];
+            sendI[(size_t)q + 1] = c;
+        }
+        for (int q = q0; q < q1; ++q) sendI[(size_t)q + 1] += sendI[(size_t)q];
+        sendJ.resize((size_t)sendI[(size_t)q1]);
+#pragma omp parallel for num_threads(T) schedule(dynamic, 1)
+        for (int q = q0; q < q1; ++q) {
+            const char *mq = mark.data() + (size_t)(q - q0) * m_loc;
+            int d = sendI[(size_t)q];
+            for (int lc = 0; lc < m_loc; ++lc)
+                if (mq[lc]) sendJ[(size_t)d++] = lc;
         }
     }
-    std::sort(pairs.begin(), pairs.end());
-    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
     int snum = 0;
-    for (size_t k = 0; k < pairs.size(); ++k)
-        if (k == 0 || (pairs[k] >> 32) != (pairs[k - 1] >> 32)) ++snum;
+    for (int q = 0; q < nparts; ++q)
+        if (sendI[(size_t)q + 1] > sendI[(size_t)q]) ++snum;
     s.send.num = snum;
     s.send.I = hmalloc<int>((size_t)snum + 1);
     s.send.ranks = hmalloc<int>((size_t)snum);
-    s.send.J = hmalloc<int>(pairs.size());
-    s.send.val = static_cast<double *>(std::calloc(pairs.size() ? pairs.size() : 1, sizeof(double)));
+    s.send.J = hmalloc<int>(sendJ.size());
+    s.send.val = static_cast<double *>(std::calloc(sendJ.size() ? sendJ.size() : 1, sizeof(double)));
     if (!s.send.I || !s.send.ranks || !s.send.J || !s.send.val) {
         hbs_host_shard_free(out);
         return lfail(HBS_ERR_ALLOC);
     }
+    std::copy(sendJ.begin(), sendJ.end(), s.send.J);
     s.send.I[0] = 0;
-    int b = -1;
-    for (size_t k = 0; k < pairs.size(); ++k) {
-        const int q = (int)(pairs[k] >> 32);
-        if (b < 0 || s.send.ranks[b] != q) {
-            ++b;
+    for (int q = 0, b = 0; q < nparts; ++q)
+        if (sendI[(size_t)q + 1] > sendI[(size_t)q]) {
             s.send.ranks[b] = q;
+            s.send.I[b + 1] = sendI[(size_t)q + 1];
+            ++b;
         }
-        s.send.J[k] = (int)(pairs[k] & 0xffffffffu);
-        s.send.I[b + 1] = (int)k + 1;
-    }
     return HBS_OK;
 }

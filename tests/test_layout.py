@@ -82,6 +82,26 @@ def test_shard_build_bit_exact_random(nparts, seed):
         hbs.shard_free(s)
 
 
+def test_shard_build_uneven_parts_many_peers():
+    # rank 0 holds ~90% of the rows among 16 parts, so the per-peer marks of
+    # its send lists are taken in several peer groups; the small ranks have a
+    # handful of rows each (some none of the matrix's entries reference)
+    rp, c, v = markov(6000)
+    n = rp.size - 1
+    rng = np.random.default_rng(5)
+    pv = np.where(rng.random(n) < 0.9, 0, rng.integers(1, 16, n)).astype(np.int32)
+    ref = orc.build_shards(orc.CSR(rp, c, v), pv, 16)
+    got = []
+    for r in range(16):
+        s = hbs.shard_build(rp, c, v, pv, 16, r)
+        assert_same_shard(s, ref[r])
+        got.append(s)
+    assert got[0]["send"]["num"] == 15
+    check_send(got, pv, 16)
+    for s in got:
+        hbs.shard_free(s)
+
+
 def test_shard_build_markov_4_parts():
     rp, c, v = markov(20000)
     n = rp.size - 1
