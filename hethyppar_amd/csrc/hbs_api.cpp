@@ -24,6 +24,7 @@ int g_max_groups = 0;     // 0: kMaxGroups, or the resident workgroups of the 51
 int g_stream_nt = 0;
 int g_spmv_npt = 0;       // entries per thread per pass of 512-thread SpMV blocks (0: 4)
 int g_sort_pct = 70;      // kind-2 blocks: sorted gathers touch <= 70% as many lines as entries
+int g_xseg = 2;           // x-segmented SpMV: 0 never, 1 every one-system matrix, 2 scattered columns
 }  // namespace
 
 #define fail ::hbs::set_error
@@ -304,6 +305,13 @@ extern "C" int hbs_set_spmv_pass(int entries_per_thread)
     return HBS_OK;
 }
 
+extern "C" int hbs_set_spmv_xseg(int mode)
+{
+    if (mode < 0 || mode > 2) return fail(HBS_ERR_ARG, "mode must be 0, 1 or 2");
+    g_xseg = mode;
+    return HBS_OK;
+}
+
 extern "C" int hbs_csr_destroy(hbs_csr a)
 {
     if (!a) return HBS_OK;
@@ -318,6 +326,19 @@ extern "C" int hbs_csr_destroy(hbs_csr a)
     dfree(a->sval32);
     dfree(a->sperm);
     dfree(a->ssrc);
+    dfree(a->xs_sbase);
+    dfree(a->xs_cbase);
+    dfree(a->xs_gbase);
+    dfree(a->xs_gch);
+    dfree(a->xs_off);
+    dfree(a->xs_src);
+    dfree(a->xs_col);
+    dfree(a->xs_pos);
+    dfree(a->xs_delta);
+    dfree(a->xs_val);
+    dfree(a->xs_prod);
+    dfree(a->xs_val32);
+    dfree(a->xs_prod32);
     dfree(a->plan.blocks);
     dfree(a->plan.aux);
     dfree(a->plan.chunks);
@@ -407,8 +428,16 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     }
     HostPlan hp;
     std::vector<SysRange> ranges;
+    // x-segmented SpMV for one system whose columns spread over an x larger
+    // than an XCD's L2 share (8 MiB of FP64): the gathers of 128-byte-line
+    // windows of kSegBlockNnz entries touch a new line for more than half
+    // the entries
+    bool xseg = (nsys <= 1 || !sys_rows) && nnz > 0 &&
+                (g_xseg == 1 || (g_xseg == 2 && (long long)n * 8 >= (8LL << 20) &&
+                                 column_scatter(col, nnz, kSegBlockNnz) > 0.5));
     const int block_nnz =
-        g_block_nnz > 0 ? g_block_nnz : default_block_nnz(m, nnz, spmv_resident_groups(6144, 4, ctx->device));
+        xseg ? (g_block_nnz > 2048 ? g_block_nnz : kSegBlockNnz)
+             : g_block_nnz > 0 ? g_block_nnz : default_block_nnz(m, nnz, spmv_resident_groups(6144, 4, ctx->device));
     if (!sys_rows) {
         a->sys_rows = {0, m};
         nsys = 1;
@@ -416,8 +445,8 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
         a->sys_rows.assign(sys_rows, sys_rows + nsys + 1);
     }
     a->nsys = nsys;
-    const int npt = spmv_threads(block_nnz) == kThreads ? (block_nnz <= 4 * kThreads ? 4 : 8)
-                                                        : (g_spmv_npt ? g_spmv_npt : 4);
+    int npt = spmv_threads(block_nnz) == kThreads ? (block_nnz <= 4 * kThreads ? 4 : 8)
+                                                  : (xseg ? kPreNpt : g_spmv_npt ? g_spmv_npt : 4);
     if (prepare_spmv(block_nnz, npt) != hipSuccess)
         return bail(fail(HBS_ERR_HIP, "SpMV LDS limit for %d-entry row blocks not granted", block_nnz));
     // SpMV workgroups: 512-thread ones as many as are resident at once, each
@@ -427,7 +456,7 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     if (max_groups == 0) {
         max_groups = kMaxGroups;
         if (spmv_threads(block_nnz) == kWideThreads) {
-            const int res = spmv_resident_groups(block_nnz, npt, ctx->device);
+            const int res = spmv_resident_groups(block_nnz, npt, ctx->device, xseg);
             if (res > 0) max_groups = std::min(max_groups, res);
         }
     }
@@ -442,10 +471,47 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
         ranges.push_back(SysRange{g0, (int)hp.groups.size(), 0, 0});
     }
     a->sys_host = ranges;
+    XsegHost xh;
+    if (xseg) {
+        // the segment's chunk tables must fit k_xseg's LDS beside x
+        plan_xseg(col, n, hp, xh);
+        if (xseg_lds_bytes(8, xh.max_chunks, xh.max_granules) > (size_t)kXsegMaxLds ||
+            prepare_xseg(xh.max_chunks, xh.max_granules) != hipSuccess) {
+            // CSR-stream SpMV over the same row blocks
+            xseg = false;
+            npt = 4;
+            if (prepare_spmv(block_nnz, npt) != hipSuccess)
+                return bail(fail(HBS_ERR_HIP, "SpMV LDS limit for %d-entry row blocks not granted", block_nnz));
+        }
+    }
+    if (xseg) {
+        a->xs_nseg = xh.nseg;
+        a->xs_max_chunks = xh.max_chunks;
+        a->xs_max_granules = xh.max_granules;
+        std::vector<double> xv((size_t)nnz);
+        for (int g = 0; g < nnz; ++g) xv[(size_t)g] = val[xh.src[(size_t)g]];
+        if ((rc = dalloc(&a->xs_sbase, xh.sbase.size())) || (rc = dalloc(&a->xs_cbase, xh.cbase.size())) ||
+            (rc = dalloc(&a->xs_gbase, xh.gbase.size())) || (rc = upload(a->xs_cbase, xh.cbase)) ||
+            (rc = dalloc(&a->xs_gch, xh.gch.size())) || (rc = dalloc(&a->xs_off, xh.off.size())) ||
+            (rc = dalloc(&a->xs_src, (size_t)nnz)) || (rc = dalloc(&a->xs_col, (size_t)nnz)) ||
+            (rc = dalloc(&a->xs_pos, (size_t)nnz)) || (rc = dalloc(&a->xs_delta, (size_t)nnz)) ||
+            (rc = dalloc(&a->xs_val, (size_t)nnz)) || (rc = dalloc(&a->xs_prod, (size_t)nnz)) ||
+            (rc = upload(a->xs_sbase, xh.sbase)) || (rc = upload(a->xs_gbase, xh.gbase)) ||
+            (rc = upload(a->xs_gch, xh.gch)) || (rc = upload(a->xs_off, xh.off)) || (rc = upload(a->xs_src, xh.src)) ||
+            (rc = upload(a->xs_col, xh.xcol)) || (rc = upload(a->xs_pos, xh.ppos)) ||
+            (rc = upload(a->xs_delta, xh.delta)) || (rc = upload(a->xs_val, xv)))
+            return bail(rc);
+        if (a->val32) {
+            if ((rc = dalloc(&a->xs_val32, (size_t)nnz)) || (rc = dalloc(&a->xs_prod32, (size_t)nnz))) return bail(rc);
+            if (launch_to_f32(nnz, a->xs_val, a->xs_val32, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
+                return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
+        }
+    }
     // column-sorted gathers where they share cache lines
     std::vector<int> src;
     std::vector<unsigned short> spos;
-    if (plan_sorted(col, nnz, g_sort_pct, hp, src, spos) > 0) {
+    if (!xseg && plan_sorted(col, nnz, g_sort_pct, hp, src, spos) > 0) {
         // When the column index and the position in the block fit in 32 bits
         // together, the position rides in the column's high bits and the
         // separate position stream is not needed.
@@ -578,6 +644,8 @@ extern "C" int hbs_csr_set_values(hbs_csr a, const double *val_dev)
         if (t->val32) HIPCHK(launch_to_f32(t->nnz, t->val, t->val32, q));
         if (t->sval) HIPCHK(launch_gather_f64(t->nnz, t->val, t->ssrc, t->sval, q));
         if (t->sval32) HIPCHK(launch_to_f32(t->nnz, t->sval, t->sval32, q));
+        if (t->xs_val) HIPCHK(launch_gather_f64(t->nnz, t->val, t->xs_src, t->xs_val, q));
+        if (t->xs_val32) HIPCHK(launch_to_f32(t->nnz, t->xs_val, t->xs_val32, q));
     }
     return HBS_OK;
 }
@@ -607,6 +675,13 @@ extern "C" int hbs_csr_plan_info(hbs_csr a, int *ngroups, int *nchunks, int *blo
     if (nchunks) *nchunks = a->plan.nchunks;
     if (block_nnz) *block_nnz = a->plan.block_nnz;
     if (max_row) *max_row = a->plan.max_row;
+    return HBS_OK;
+}
+
+extern "C" int hbs_csr_xseg(hbs_csr a, int *nseg)
+{
+    if (!a || !nseg) return fail(HBS_ERR_ARG, "null argument");
+    *nseg = a->xs_nseg;
     return HBS_OK;
 }
 

@@ -16,8 +16,10 @@ constexpr int kThreads = 256;          // workgroup size of every kernel (4 wave
 constexpr int kWaves = kThreads / 64;
 constexpr int kWideThreads = 512;      // SpMV workgroup size for row blocks above 2048 entries
 constexpr int kMaxBlockNnz = 16384;    // upper bound of nnz per SpMV row block (128 KiB of FP64 LDS)
-constexpr int kLongRow = 1024;         // rows longer than this are summed in chunks (k_chunks)
+constexpr int kLongRow = 256;          // rows longer than this are summed in chunks (k_chunks)
 constexpr int kRowPad = 8;             // entries of padding past nnz in col / val / val32
+constexpr int kSegCols = 8192;         // x-segmented SpMV: columns per segment (64 KiB of FP64 x in LDS)
+constexpr int kSegBlockNnz = 8192;     // x-segmented SpMV: entries per row block
 
 // Threads of the SpMV workgroups that sum row blocks of block_nnz entries.
 constexpr int spmv_threads(int block_nnz) { return block_nnz > 2048 ? kWideThreads : kThreads; }
@@ -139,6 +141,33 @@ struct SpmvArgs {
     int check_done;             // return when the system is done (or ||s|| fired)
     int stream_nt;              // load col/val non-temporally
     SolverState *st_mark;       // start of a solve: set pending in this slot
+    const T *prod;              // x-segmented matrices: the products of k_xseg, row-block major
+    const unsigned short *ppos; //   and each one's position in its row block (CSR order)
+};
+
+// x-segmented SpMV, product pass (k_xseg): workgroup s stages the x segment
+// [s * seg_cols, (s + 1) * seg_cols) in LDS and forms the products of the
+// entries whose column lies in it.  Those entries are stored segment-major
+// (xcol: column within the segment; xval) in chunks, one per (segment, row
+// block); product g goes to prod[g + off[c]], c the chunk of entry g, found
+// as cbase[s] + gch[granule of g] + delta[g] (granules of 64 entries of a
+// segment).  The segment's off and gch entries are staged in LDS too: at
+// most max_chunks and max_granules of them.
+template <typename T>
+struct XsegArgs {
+    int nseg, seg_cols, n;
+    int max_chunks, max_granules;
+    const int *sbase;            // [nseg + 1] first entry of each segment
+    const int *cbase;            // [nseg + 1] first chunk of each segment
+    const int *gbase;            // [nseg + 1] first granule of each segment
+    const int *gch;              // [granules] chunk of each granule's first entry, within its segment
+    const int *off;              // [chunks] row-block-major start minus segment-major start
+    const unsigned short *xcol;
+    const unsigned char *delta;
+    const T *xval;
+    const T *x;
+    T *prod;
+    const SolverState *st;       // null: always run; else skip when st->done
 };
 
 template <typename T>
@@ -150,6 +179,7 @@ struct ChunkArgs {
     const T *x;
     double *chunk_sums;
     const SolverState *st;      // null: always run
+    const T *prod;              // x-segmented matrices: sum these products instead
 };
 
 // Launch parameters of the vector kernels of one BiCGStab iteration.
@@ -234,6 +264,23 @@ void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max
 // blocks.
 int plan_sorted(const int *col, int nnz, int max_line_pct, HostPlan &plan, std::vector<int> &src,
                 std::vector<unsigned short> &pos);
+// x-segmented layout of the row blocks of a plan: every block's entries
+// ordered by column segment (kSegCols columns), CSR order inside a segment
+// (row-block major, ppos = CSR position in the block), and the same entries
+// segment-major in chunks of one (segment, row block); see XsegArgs.
+struct XsegHost {
+    int nseg = 0;
+    int max_chunks = 0, max_granules = 0;   // per segment
+    std::vector<int> sbase, cbase, gbase, gch, off;
+    std::vector<int> src;                  // segment-major entry -> CSR entry
+    std::vector<unsigned short> xcol;      // segment-major: column within the segment
+    std::vector<unsigned char> delta;      // segment-major: chunk minus its granule's first chunk
+    std::vector<unsigned short> ppos;      // row-block major: CSR position in the block
+};
+void plan_xseg(const int *col, int n, const HostPlan &plan, XsegHost &xs);
+// Distinct 128-byte lines of an FP64 x touched per entry, over windows of
+// window consecutive entries (sampled): near 1 for scattered columns.
+double column_scatter(const int *col, int nnz, int window);
 // Split the row blocks [rb0, rb1) of one system over ngroups workgroups.
 void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan);
 // Vector blocks of elements [e0, e0 + n) of one system.
@@ -249,11 +296,22 @@ int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 hipError_t prepare_spmv(int block_nnz, int npt);
 // FP64 SpMV workgroups of row blocks of block_nnz entries (npt per thread per
 // pass) resident at once on the device (0 when the query fails).
-int spmv_resident_groups(int block_nnz, int npt, int device);
+int spmv_resident_groups(int block_nnz, int npt, int device, bool pre = false);
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, int npt, hipStream_t s);
 hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, int npt, hipStream_t s);
 hipError_t launch_chunks(const ChunkArgs<double> &a, int block_nnz, hipStream_t s);
 hipError_t launch_chunks(const ChunkArgs<float> &a, int block_nnz, hipStream_t s);
+hipError_t launch_xseg(const XsegArgs<double> &a, hipStream_t s);
+hipError_t launch_xseg(const XsegArgs<float> &a, hipStream_t s);
+// Dynamic LDS of k_xseg for values of elem_bytes and the given table sizes.
+size_t xseg_lds_bytes(int elem_bytes, int max_chunks, int max_granules);
+// Allow k_xseg that much LDS (once per plan, outside any stream capture).
+hipError_t prepare_xseg(int max_chunks, int max_granules);
+// Entries per thread per pass of the x-segmented row sums (k_spmv<PRE>): one
+// pass covers a row block of kSegBlockNnz entries.
+constexpr int kPreNpt = kSegBlockNnz / kWideThreads;
+constexpr int kXsegNpt = 8;            // k_xseg entries per thread per pass
+constexpr int kXsegMaxLds = 96 * 1024; // k_xseg LDS cap (x segment + chunk tables); above it no x-segmenting
 hipError_t launch_vec_P(const VecArgs<double> &a, hipStream_t s);
 hipError_t launch_vec_S(const VecArgs<double> &a, hipStream_t s);
 hipError_t launch_vec_XR(const VecArgs<double> &a, hipStream_t s);

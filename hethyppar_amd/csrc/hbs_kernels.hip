@@ -12,7 +12,7 @@
 //               and row sums run.  Short-row blocks sum serially in CSR order
 //               (bit-identical to the reference CSR_spmxv_seq,
 //               hhp_cpu.c:445-455); blocks of longer rows use 2..64 lanes per
-//               row; rows longer than 1024 entries (or than a block) are cut
+//               row; rows longer than 256 entries (or than a block) are cut
 //               into chunks summed by k_chunks and combined here in chunk
 //               order.  Epilogues fuse the dot products BiCGStab needs after
 //               each SpMV.
@@ -74,8 +74,9 @@ __device__ __forceinline__ constexpr bool reads_w()
 }
 
 // One pass over a row block: the entries base + t + TH * i (i < NPT) of the
-// block, as loaded column indices, values and (kind 2) CSR positions.  A wave
-// instruction covers 64 consecutive entries.
+// block, as loaded column indices, values and (kind 2) CSR positions -- or,
+// for x-segmented matrices (PRE), the products and their CSR positions.  A
+// wave instruction covers 64 consecutive entries.
 template <typename T, int NPT>
 struct PassLoad {
     int c[NPT];
@@ -85,12 +86,23 @@ struct PassLoad {
 
 // Issue the stream loads of one pass (entries at or past the block's end
 // load nothing).
-template <typename T, int NPT, int TH>
+template <typename T, int NPT, int TH, bool PRE>
 __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, int base,
                                            PassLoad<T, NPT> &L)
 {
-    const bool sorted = ax.kind == 2;
     const int lim = rb.j1 - rb.j0 - base;
+    if constexpr (PRE) {
+        const T *__restrict__ pr = a.prod + rb.j0 + base;
+        const unsigned short *__restrict__ pp = a.ppos + rb.j0 + base;
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const int k = (int)threadIdx.x + i * TH;
+            L.v[i] = k < lim ? pr[k] : T(0);
+            L.pos[i] = k < lim ? pp[k] : (unsigned short)0;
+        }
+        return;
+    }
+    const bool sorted = ax.kind == 2;
     const int *__restrict__ colp = (sorted ? a.scol : a.col) + rb.j0 + base;
     const T *__restrict__ valp = (sorted ? a.sval : a.val) + rb.j0 + base;
     if (sorted && !a.pos_shift) {
@@ -134,12 +146,30 @@ __device__ __forceinline__ T issue_w0(const SpmvArgs<T> &a, const RowBlock &rb, 
     return T(0);
 }
 
+// Adds lds[j0], lds[j0 + step], ... below e to acc in that order, loading B
+// of them at a time so a long row costs one LDS round trip per batch rather
+// than per product.  Past e the batch reads +0.0, and acc + 0.0 == acc
+// exactly: acc starts at +0.0, so it is never -0.0.
+template <typename T, int B>
+__device__ __forceinline__ T sum_lds(const T *lds, int j0, int e, int step)
+{
+    T acc = T(0);
+    for (int j = j0; j < e; j += B * step) {
+        T v[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) v[u] = j + u * step < e ? lds[j + u * step] : T(0);
+#pragma unroll
+        for (int u = 0; u < B; ++u) acc += v[u];
+    }
+    return acc;
+}
+
 // Sums the rows of a block whose products (lds) and row offsets (lrp) are in
 // LDS and feeds the epilogue.  Serial blocks add each row's products in CSR
 // order from 0 (the reference's order): thread t sums rows t, t + TH, ...,
 // with their epilogue operands loaded together (the first one, w0, was
 // prefetched).  Other blocks use `lanes` lanes per row and an xor butterfly.
-template <typename T, int EPI, int TH>
+template <typename T, int EPI, int TH, int B>
 __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, T w0,
                                          const T *lds, const unsigned short *lrp, Acc<T> &d0, Acc<T> &d1)
 {
@@ -156,9 +186,7 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
             if (t < nr) {
-                const int e = lrp[t + 1];
-                T acc = T(0);
-                for (int j = lrp[t]; j < e; ++j) acc += lds[j];
+                const T acc = sum_lds<T, B>(lds, lrp[t], lrp[t + 1], 1);
                 row_out<T, EPI>(a, rb.r0 + t, acc, wr[q], d0, d1);
             }
         }
@@ -172,7 +200,7 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
             if (t < nr) {
                 if (t0 > 0 && reads_w<EPI>() && l == 0) wr = a.w[rb.r0 + t];
                 const int e = lrp[t + 1];
-                for (int q = lrp[t] + l; q < e; q += lanes) acc += lds[q];
+                acc = sum_lds<T, B>(lds, lrp[t] + l, e, lanes);
             }
             for (int o = lanes >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
             if (t < nr && l == 0) row_out<T, EPI>(a, rb.r0 + t, acc, wr, d0, d1);
@@ -196,8 +224,10 @@ struct Cursor {
 // so they are in flight while pass i's products are stored and, after a
 // block's last pass, while its rows are summed.  A finished system returns
 // before any load; otherwise the first pass's loads are issued before the
-// rest of the state prologue, so they overlap it.
-template <typename T, int EPI, int NPT, int TH>
+// rest of the state prologue, so they overlap it.  PRE (x-segmented
+// matrices): the passes load the products k_xseg formed, with their
+// positions, instead of gathering x.
+template <typename T, int EPI, int NPT, int TH, bool PRE>
 __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_spmv(SpmvArgs<T> a)
 {
     constexpr int NW = TH / 64;
@@ -217,7 +247,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     Cursor cur{g.rb0, 0, a.blocks[g.rb0], a.aux[g.rb0]};
     PassLoad<T, NPT> LA, LB;
-    if (cur.ax.kind != 1) issue_pass<T, NPT, TH>(a, cur.rb, cur.ax, 0, LA);
+    if (cur.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, cur.rb, cur.ax, 0, LA);
     T w0 = issue_w0<T, EPI>(a, cur.rb, cur.ax);
 
     // ||s|| test: the system's first workgroup applies it to the state it
@@ -246,7 +276,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // kind-2 blocks with packed positions: column in the low pos_shift bits
         const bool packed = cur.ax.kind == 2 && a.pos_shift;
         const unsigned cmask = packed ? (1u << a.pos_shift) - 1u : ~0u;
-        if (gather) {
+        if (gather && !PRE) {
             const int lim = nz - cur.base;
 #pragma unroll
             for (int i = 0; i < NPT; ++i) {
@@ -279,17 +309,28 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
             nxt.base = cur.base + PASS;
         }
         const bool more = nxt.b < g.rb1;
-        if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH>(a, nxt.rb, nxt.ax, nxt.base, out);
+        if constexpr (!PRE) {
+            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, nxt.rb, nxt.ax, nxt.base, out);
+        }
         if (gather) {
             const int lim = nz - cur.base;
             const bool sorted = cur.ax.kind == 2;
 #pragma unroll
             for (int i = 0; i < NPT; ++i) {
                 const int k = (int)threadIdx.x + i * TH;
-                const int at = !sorted ? cur.base + k
-                                       : packed ? (int)((unsigned)in.c[i] >> a.pos_shift) : (int)in.pos[i];
-                if (k < lim) lds[at] = in.v[i] * xv[i];
+                if constexpr (PRE) {
+                    if (k < lim) lds[in.pos[i]] = in.v[i];
+                } else {
+                    const int at = !sorted ? cur.base + k
+                                           : packed ? (int)((unsigned)in.c[i] >> a.pos_shift) : (int)in.pos[i];
+                    if (k < lim) lds[at] = in.v[i] * xv[i];
+                }
             }
+        }
+        // PRE: one register set -- the next pass loads into it once this
+        // pass's products are in LDS, still ahead of the row sums
+        if constexpr (PRE) {
+            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, nxt.rb, nxt.ax, nxt.base, out);
         }
         if (stage) {
 #pragma unroll
@@ -300,15 +341,24 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         if (last) {
             if (!gather) {
-                // long row: ordered sum of its chunk sums
-                if (threadIdx.x == 0) {
+                // long row: ordered sum of its chunk sums.  The first wave
+                // loads up to 64 of them at once and every lane adds them
+                // in chunk order from its neighbours' registers, so the
+                // loads are one round trip per 64 chunks, not one per chunk.
+                if (threadIdx.x < 64) {
+                    const T wr = reads_w<EPI>() && threadIdx.x == 0 ? a.w[cur.rb.r0] : T(0);
                     T total = T(0);
-                    for (int c = 0; c < cur.ax.nchunks; ++c) total += (T)a.chunk_sums[cur.ax.first + c];
-                    row_out<T, EPI>(a, cur.rb.r0, total, reads_w<EPI>() ? a.w[cur.rb.r0] : T(0), d0, d1);
+                    for (int c0 = 0; c0 < cur.ax.nchunks; c0 += 64) {
+                        const int c = c0 + (int)threadIdx.x;
+                        const T v = c < cur.ax.nchunks ? (T)a.chunk_sums[cur.ax.first + c] : T(0);
+                        const int lim = min(64, cur.ax.nchunks - c0);
+                        for (int l = 0; l < lim; ++l) total += __shfl(v, l, 64);
+                    }
+                    if (threadIdx.x == 0) row_out<T, EPI>(a, cur.rb.r0, total, wr, d0, d1);
                 }
             } else {
                 __syncthreads();
-                sum_rows<T, EPI, TH>(a, cur.rb, cur.ax, w0, lds, lrp, d0, d1);
+                sum_rows<T, EPI, TH, PRE ? 8 : 2>(a, cur.rb, cur.ax, w0, lds, lrp, d0, d1);
                 __syncthreads();   // LDS is reused by the next row block
             }
             if (more) w0 = issue_w0<T, EPI>(a, nxt.rb, nxt.ax);
@@ -316,9 +366,14 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
         cur = nxt;
         return more;
     };
-    for (;;) {
-        if (!step(LA, LB)) break;
-        if (!step(LB, LA)) break;
+    if constexpr (PRE) {
+        while (step(LA, LA)) {
+        }
+    } else {
+        for (;;) {
+            if (!step(LA, LB)) break;
+            if (!step(LB, LA)) break;
+        }
     }
 
     if constexpr (ND > 0) {
@@ -333,7 +388,8 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
 }
 
-// One workgroup per chunk of a long row: ordered tree sum of its products.
+// One workgroup per chunk of a long row: ordered tree sum of its products
+// (formed here, or by k_xseg for x-segmented matrices).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_chunks(ChunkArgs<T> a)
 {
@@ -341,9 +397,81 @@ __global__ __launch_bounds__(kThreads) void k_chunks(ChunkArgs<T> a)
     const Chunk c = a.chunks[blockIdx.x];
     if (a.st && a.st[c.sys].done) return;
     T acc = T(0);
-    for (int j = c.j0 + (int)threadIdx.x; j < c.j1; j += kThreads) acc += a.val[j] * a.x[a.col[j]];
+    if (a.prod) {
+        for (int j = c.j0 + (int)threadIdx.x; j < c.j1; j += kThreads) acc += a.prod[j];
+    } else {
+        for (int j = c.j0 + (int)threadIdx.x; j < c.j1; j += kThreads) acc += a.val[j] * a.x[a.col[j]];
+    }
     acc = block_sum1<T>(acc, red);
     if (threadIdx.x == 0) a.chunk_sums[blockIdx.x] = (double)acc;
+}
+
+// x-segmented SpMV, product pass (layout in XsegArgs).  For columns spread
+// over all of x, the gathers of the CSR-stream SpMV each miss L2 and are
+// bound by the misses in flight.  Here workgroup s copies its x segment
+// (kSegCols values, coalesced) and its chunk tables into LDS once, then
+// streams its entries: column within the segment, value, and the chunk that
+// places the product in the row-block-major product array, which
+// k_spmv<PRE> / k_chunks then sum in the usual order.  The passes are
+// software-pipelined over two register sets (the next pass's streams load
+// while this pass's products are formed and stored).  The workgroups of
+// neighbouring segments share an XCD, so the chunks one XCD writes into a
+// row block are neighbours.
+template <typename T, int NPT>
+struct XLoad {
+    int c[NPT], d[NPT];
+    T v[NPT];
+};
+
+template <typename T, int NPT>
+__device__ __forceinline__ void xseg_issue(const XsegArgs<T> &a, int g0, int cnt, int base, XLoad<T, NPT> &L)
+{
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int f = base + i * kWideThreads + (int)threadIdx.x;
+        const bool in = f < cnt;
+        L.c[i] = in ? (int)a.xcol[g0 + f] : 0;
+        L.d[i] = in ? (int)a.delta[g0 + f] : 0;
+        L.v[i] = in ? a.xval[g0 + f] : T(0);
+    }
+}
+
+template <typename T, int NPT>
+__global__ __launch_bounds__(kWideThreads) void k_xseg(XsegArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *xs = reinterpret_cast<T *>(smem);
+    int *offl = reinterpret_cast<int *>(smem + (size_t)a.seg_cols * sizeof(T));
+    int *gchl = offl + a.max_chunks;
+    const int s = xcd_remap(blockIdx.x, a.nseg);
+    if (a.st && a.st->done) return;
+    const int g0 = a.sbase[s], cnt = a.sbase[s + 1] - g0;
+    if (cnt <= 0) return;
+    XLoad<T, NPT> LA, LB;
+    xseg_issue<T, NPT>(a, g0, cnt, 0, LA);
+    const int c0 = s * a.seg_cols, nc = min(a.seg_cols, a.n - c0);
+    for (int i = (int)threadIdx.x; i < nc; i += kWideThreads) xs[i] = a.x[c0 + i];
+    const int cb = a.cbase[s], ncs = a.cbase[s + 1] - cb;
+    for (int i = (int)threadIdx.x; i < ncs; i += kWideThreads) offl[i] = a.off[cb + i];
+    const int gb = a.gbase[s], ngs = a.gbase[s + 1] - gb;
+    for (int i = (int)threadIdx.x; i < ngs; i += kWideThreads) gchl[i] = a.gch[gb + i];
+    __syncthreads();
+    T *__restrict__ prod = a.prod + g0;
+    constexpr int PASS = NPT * kWideThreads;
+    auto step = [&](const XLoad<T, NPT> &in, XLoad<T, NPT> &out, int base) -> bool {
+        const bool more = base + PASS < cnt;
+        if (more) xseg_issue<T, NPT>(a, g0, cnt, base + PASS, out);
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const int f = base + i * kWideThreads + (int)threadIdx.x;
+            if (f < cnt) prod[f + offl[gchl[f >> 6] + in.d[i]]] = in.v[i] * xs[in.c[i]];
+        }
+        return more;
+    };
+    for (int base = 0;; base += 2 * PASS) {
+        if (!step(LA, LB, base)) break;
+        if (!step(LB, LA, base + PASS)) break;
+    }
 }
 
 // ---- streaming vector kernels --------------------------------------------
@@ -693,30 +821,31 @@ __global__ void k_state_init(SolverState *st, int nstates, double tol, int maxit
     }
 }
 
-template <typename T, int NPT, int TH>
+template <typename T, int NPT, int TH, bool PRE = false>
 hipError_t spmv_epi(const SpmvArgs<T> &a, int epi, size_t shmem, hipStream_t s)
 {
     switch (epi) {
-    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESIDB: k_spmv<T, EPI_RESIDB, NPT, TH><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESIDB: k_spmv<T, EPI_RESIDB, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 // Allow the dynamic LDS of the wide blocks (above the 64 KiB default).
-template <typename T, int NPT, int TH>
+template <typename T, int NPT, int TH, bool PRE = false>
 hipError_t spmv_allow_lds(size_t shmem)
 {
     const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
     hipError_t e = hipSuccess;
-    const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH>, (const void *)k_spmv<T, EPI_DOT1, NPT, TH>,
-                       (const void *)k_spmv<T, EPI_DOT2, NPT, TH>, (const void *)k_spmv<T, EPI_RESID, NPT, TH>,
-                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH>, (const void *)k_spmv<T, EPI_RESIDB, NPT, TH>};
+    const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_DOT1, NPT, TH, PRE>,
+                       (const void *)k_spmv<T, EPI_DOT2, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_RESID, NPT, TH, PRE>,
+                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH, PRE>,
+                       (const void *)k_spmv<T, EPI_RESIDB, NPT, TH, PRE>};
     for (const void *f : k)
         if (e == hipSuccess) e = hipFuncSetAttribute(f, at, (int)shmem);
     return e;
@@ -731,6 +860,11 @@ hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, int npt, 
     if (a.ngroups <= 0) return hipSuccess;
     if (block_nnz <= 0 || block_nnz > kMaxBlockNnz) return hipErrorInvalidValue;
     const size_t prod = (size_t)block_nnz * sizeof(T);
+    if (a.prod) {
+        // x-segmented: 512-thread workgroups, one pass per row block
+        if (spmv_threads(block_nnz) != kWideThreads || npt != kPreNpt) return hipErrorInvalidValue;
+        return spmv_epi<T, kPreNpt, kWideThreads, true>(a, epi, prod + rowptr_lds_bytes<kWideThreads>(), s);
+    }
     if (block_nnz <= 4 * kThreads) return spmv_epi<T, 4, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
     if (block_nnz <= 8 * kThreads) return spmv_epi<T, 8, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
     const size_t shmem = prod + rowptr_lds_bytes<kWideThreads>();
@@ -774,7 +908,10 @@ hipError_t prepare_spmv(int block_nnz, int npt)
     const size_t rows = rowptr_lds_bytes<kWideThreads>();
     const size_t d = (size_t)block_nnz * sizeof(double) + rows, f = (size_t)block_nnz * sizeof(float) + rows;
     hipError_t e;
-    if (npt == 4) {
+    if (npt == kPreNpt) {
+        e = spmv_allow_lds<double, kPreNpt, kWideThreads, true>(d);
+        if (e == hipSuccess) e = spmv_allow_lds<float, kPreNpt, kWideThreads, true>(f);
+    } else if (npt == 4) {
         e = spmv_allow_lds<double, 4, kWideThreads>(d);
         if (e == hipSuccess) e = spmv_allow_lds<float, 4, kWideThreads>(f);
     } else {
@@ -783,15 +920,16 @@ hipError_t prepare_spmv(int block_nnz, int npt)
     }
     return e;
 }
-int spmv_resident_groups(int block_nnz, int npt, int device)
+int spmv_resident_groups(int block_nnz, int npt, int device, bool pre)
 {
     const int th = spmv_threads(block_nnz);
     const size_t shmem = (size_t)block_nnz * sizeof(double) +
                          (th == kWideThreads ? rowptr_lds_bytes<kWideThreads>() : rowptr_lds_bytes<kThreads>());
-    const void *f = th == kThreads ? (block_nnz <= 4 * kThreads ? (const void *)k_spmv<double, EPI_DOT2, 4, kThreads>
-                                                                 : (const void *)k_spmv<double, EPI_DOT2, 8, kThreads>)
-                                   : (npt == 4 ? (const void *)k_spmv<double, EPI_DOT2, 4, kWideThreads>
-                                               : (const void *)k_spmv<double, EPI_DOT2, 8, kWideThreads>);
+    const void *f = th == kThreads ? (block_nnz <= 4 * kThreads ? (const void *)k_spmv<double, EPI_DOT2, 4, kThreads, false>
+                                                                 : (const void *)k_spmv<double, EPI_DOT2, 8, kThreads, false>)
+                    : pre          ? (const void *)k_spmv<double, EPI_DOT2, kPreNpt, kWideThreads, true>
+                                   : (npt == 4 ? (const void *)k_spmv<double, EPI_DOT2, 4, kWideThreads, false>
+                                               : (const void *)k_spmv<double, EPI_DOT2, 8, kWideThreads, false>);
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, th, shmem) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
@@ -807,6 +945,31 @@ hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, int npt
 }
 hipError_t launch_chunks(const ChunkArgs<double> &a, int, hipStream_t s) { return chunk_dispatch<double>(a, s); }
 hipError_t launch_chunks(const ChunkArgs<float> &a, int, hipStream_t s) { return chunk_dispatch<float>(a, s); }
+size_t xseg_lds_bytes(int elem_bytes, int max_chunks, int max_granules)
+{
+    return (size_t)kSegCols * (size_t)elem_bytes + 4 * ((size_t)max_chunks + (size_t)max_granules);
+}
+hipError_t prepare_xseg(int max_chunks, int max_granules)
+{
+    const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
+    hipError_t e = hipFuncSetAttribute((const void *)k_xseg<double, kXsegNpt>, at,
+                                       (int)xseg_lds_bytes(8, max_chunks, max_granules));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_xseg<float, kXsegNpt>, at,
+                                (int)xseg_lds_bytes(4, max_chunks, max_granules));
+    return e;
+}
+template <typename T>
+hipError_t xseg_dispatch(const XsegArgs<T> &a, hipStream_t s)
+{
+    if (a.nseg <= 0) return hipSuccess;
+    if (a.seg_cols != kSegCols) return hipErrorInvalidValue;
+    const size_t lds = xseg_lds_bytes((int)sizeof(T), a.max_chunks, a.max_granules);
+    k_xseg<T, kXsegNpt><<<a.nseg, kWideThreads, lds, s>>>(a);
+    return hipGetLastError();
+}
+hipError_t launch_xseg(const XsegArgs<double> &a, hipStream_t s) { return xseg_dispatch<double>(a, s); }
+hipError_t launch_xseg(const XsegArgs<float> &a, hipStream_t s) { return xseg_dispatch<float>(a, s); }
 
 hipError_t launch_vec_P(const VecArgs<double> &a, hipStream_t s) { return k_vec_P_dispatch<double>(a, s); }
 hipError_t launch_vec_S(const VecArgs<double> &a, hipStream_t s) { return k_vec_S_dispatch<double>(a, s); }

@@ -95,6 +95,16 @@ struct hbs_csr_s {
     unsigned short *sperm = nullptr;  // position in the block of each sorted entry (pos_shift 0)
     int pos_shift = 0;                // > 0: scol = col | position << pos_shift, no sperm
     int *ssrc = nullptr;              // CSR entry of each sorted position (refreshes sval)
+    // x-segmented SpMV (columns scattered over all of x): k_xseg forms the
+    // products per x segment, k_spmv<PRE> sums them (layout in XsegArgs)
+    int xs_nseg = 0;                  // 0: CSR-stream SpMV
+    int xs_max_chunks = 0, xs_max_granules = 0;
+    int *xs_sbase = nullptr, *xs_cbase = nullptr, *xs_gbase = nullptr, *xs_gch = nullptr, *xs_off = nullptr;
+    int *xs_src = nullptr;            // CSR entry of each segment-major entry (refreshes xs_val)
+    unsigned short *xs_col = nullptr, *xs_pos = nullptr;
+    unsigned char *xs_delta = nullptr;
+    double *xs_val = nullptr, *xs_prod = nullptr;
+    float *xs_val32 = nullptr, *xs_prod32 = nullptr;
     hbs::SpmvPlan plan;
     hbs::SysRange *sys = nullptr;     // [nsys] {g0, g1, 0, 0}
     double *chunk_sums = nullptr;     // [nchunks]
@@ -168,6 +178,11 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
     s.sperm = a->sperm;
     s.pos_shift = a->pos_shift;
     s.chunk_sums = a->chunk_sums;
+    if (a->xs_nseg > 0) {
+        if constexpr (sizeof(T) == 8) s.prod = reinterpret_cast<const T *>(a->xs_prod);
+        else s.prod = reinterpret_cast<const T *>(a->xs_prod32);
+        s.ppos = a->xs_pos;
+    }
     s.part_out = a->part;
     s.stream_nt = a->stream_nt;
     s.alpha = T(1);
@@ -176,10 +191,34 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
     return s;
 }
 
-// Queue one SpMV: the long-row chunk sums (if any) then the row blocks.
+// Queue one SpMV: for x-segmented matrices the product pass, then the
+// long-row chunk sums (if any), then the row blocks.
 template <typename T>
 hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a, int epi, const SolverState *chunk_st, hipStream_t q)
 {
+    if (a->xs_nseg > 0) {
+        XsegArgs<T> x{};
+        x.nseg = a->xs_nseg;
+        x.seg_cols = kSegCols;
+        x.n = a->n;
+        x.max_chunks = a->xs_max_chunks;
+        x.max_granules = a->xs_max_granules;
+        x.sbase = a->xs_sbase;
+        x.cbase = a->xs_cbase;
+        x.gbase = a->xs_gbase;
+        x.gch = a->xs_gch;
+        x.off = a->xs_off;
+        x.xcol = a->xs_col;
+        x.delta = a->xs_delta;
+        if constexpr (sizeof(T) == 8) x.xval = reinterpret_cast<const T *>(a->xs_val);
+        else x.xval = reinterpret_cast<const T *>(a->xs_val32);
+        x.x = s.x;
+        x.prod = const_cast<T *>(s.prod);
+        x.st = chunk_st;
+        if (!x.xval || !x.prod) return hipErrorInvalidValue;
+        hipError_t e = launch_xseg(x, q);
+        if (e != hipSuccess) return e;
+    }
     if (a->plan.nchunks > 0) {
         ChunkArgs<T> c{};
         c.chunks = a->plan.chunks;
@@ -189,6 +228,7 @@ hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a, int epi, const Solver
         c.x = s.x;
         c.chunk_sums = const_cast<double *>(s.chunk_sums);
         c.st = chunk_st;
+        c.prod = s.prod;
         hipError_t e = launch_chunks(c, a->plan.block_nnz, q);
         if (e != hipSuccess) return e;
     }

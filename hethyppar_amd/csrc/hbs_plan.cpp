@@ -115,6 +115,110 @@ int plan_sorted(const int *col, int nnz, int max_line_pct, HostPlan &plan, std::
     return nsorted;
 }
 
+double column_scatter(const int *col, int nnz, int window)
+{
+    if (nnz <= 0 || window <= 0) return 0.0;
+    const long long stride = 16LL * window;   // one window in 16
+    long long lines = 0, ents = 0;
+    std::vector<int> ls;
+    for (long long b = 0; b < nnz; b += stride) {
+        const long long e = std::min<long long>(nnz, b + window);
+        ls.assign(col + b, col + e);
+        for (int &v : ls) v >>= 4;
+        std::sort(ls.begin(), ls.end());
+        lines += std::unique(ls.begin(), ls.end()) - ls.begin();
+        ents += e - b;
+    }
+    return (double)lines / (double)ents;
+}
+
+// Row-block-major order: each block's entries sorted by (segment, CSR
+// position).  Its runs of one segment are the chunks; segment-major order
+// lists, per segment, its chunks in block order.  gch / delta name the chunk
+// of every segment-major entry within its segment (delta < 64: a chunk holds
+// at least one entry, and a granule 64).
+void plan_xseg(const int *col, int n, const HostPlan &plan, XsegHost &xs)
+{
+    const int nseg = std::max(1, (n + kSegCols - 1) / kSegCols);
+    const int nnz = plan.blocks.empty() ? 0 : plan.blocks.back().j1;
+    xs.nseg = nseg;
+    xs.ppos.assign((size_t)nnz, 0);
+    std::vector<int> rbsrc((size_t)nnz);
+    struct Run {
+        int s, p0, cnt;
+    };
+    std::vector<Run> runs;
+    std::vector<long long> seg_cnt((size_t)nseg, 0);
+    std::vector<int> seg_chunks((size_t)nseg, 0);
+    std::vector<unsigned long long> key;
+    for (size_t b = 0; b < plan.blocks.size(); ++b) {
+        const RowBlock &rb = plan.blocks[b];
+        const bool long_row = plan.aux[b].kind == 1;
+        const int nz = rb.j1 - rb.j0;
+        key.resize((size_t)nz);
+        for (int k = 0; k < nz; ++k)
+            key[(size_t)k] = ((unsigned long long)(unsigned)(col[rb.j0 + k] / kSegCols) << 32) | (unsigned)k;
+        std::sort(key.begin(), key.end());
+        for (int k = 0; k < nz;) {
+            const int s = (int)(key[(size_t)k] >> 32);
+            int e = k;
+            for (; e < nz && (int)(key[(size_t)e] >> 32) == s; ++e) {
+                const int pos = (int)(key[(size_t)e] & 0xffffffffu);
+                rbsrc[(size_t)rb.j0 + e] = rb.j0 + pos;
+                xs.ppos[(size_t)rb.j0 + e] = (unsigned short)(long_row ? 0 : pos);
+            }
+            runs.push_back(Run{s, rb.j0 + k, e - k});
+            seg_cnt[(size_t)s] += e - k;
+            seg_chunks[(size_t)s]++;
+            k = e;
+        }
+    }
+    xs.sbase.assign((size_t)nseg + 1, 0);
+    xs.cbase.assign((size_t)nseg + 1, 0);
+    std::vector<int> &cbase = xs.cbase;
+    for (int s = 0; s < nseg; ++s) {
+        xs.sbase[(size_t)s + 1] = xs.sbase[(size_t)s] + (int)seg_cnt[(size_t)s];
+        cbase[(size_t)s + 1] = cbase[(size_t)s] + seg_chunks[(size_t)s];
+        xs.max_chunks = std::max(xs.max_chunks, seg_chunks[(size_t)s]);
+    }
+    const int nchunks = cbase[(size_t)nseg];
+    xs.off.assign((size_t)std::max(nchunks, 1), 0);
+    std::vector<int> cstart((size_t)nchunks + 1, nnz);
+    std::vector<int> next_e(xs.sbase.begin(), xs.sbase.end() - 1), next_c(cbase.begin(), cbase.end() - 1);
+    xs.src.assign((size_t)nnz, 0);
+    xs.xcol.assign((size_t)nnz, 0);
+    for (const Run &r : runs) {
+        const int c = next_c[(size_t)r.s]++, g0 = next_e[(size_t)r.s];
+        next_e[(size_t)r.s] += r.cnt;
+        cstart[(size_t)c] = g0;
+        xs.off[(size_t)c] = r.p0 - g0;
+        for (int k = 0; k < r.cnt; ++k) {
+            const int j = rbsrc[(size_t)r.p0 + k];
+            xs.src[(size_t)g0 + k] = j;
+            xs.xcol[(size_t)g0 + k] = (unsigned short)(col[j] - r.s * kSegCols);
+        }
+    }
+    xs.gbase.assign((size_t)nseg + 1, 0);
+    for (int s = 0; s < nseg; ++s) {
+        const int ng = (int)((seg_cnt[(size_t)s] + 63) / 64);
+        xs.gbase[(size_t)s + 1] = xs.gbase[(size_t)s] + ng;
+        xs.max_granules = std::max(xs.max_granules, ng);
+    }
+    xs.gch.assign((size_t)std::max(xs.gbase[(size_t)nseg], 1), 0);
+    xs.delta.assign((size_t)nnz, 0);
+    for (int s = 0; s < nseg; ++s) {
+        int c = cbase[(size_t)s];
+        const int cnt = xs.sbase[(size_t)s + 1] - xs.sbase[(size_t)s];
+        for (int f = 0; f < cnt; ++f) {
+            const int g = xs.sbase[(size_t)s] + f;
+            while (c + 1 < cbase[(size_t)s + 1] && cstart[(size_t)c + 1] <= g) ++c;
+            int &first = xs.gch[(size_t)xs.gbase[(size_t)s] + f / 64];   // within the segment
+            if (f % 64 == 0) first = c - cbase[(size_t)s];
+            xs.delta[(size_t)g] = (unsigned char)(c - cbase[(size_t)s] - first);
+        }
+    }
+}
+
 // Contiguous, equal-count split of row blocks [rb0, rb1) over ngroups
 // workgroups (row blocks carry about block_nnz entries each).
 void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan)

@@ -142,6 +142,8 @@ _PROTOS = {
     "hbs_set_vec_blocks": (_i, [_i]),
     "hbs_set_spmv_sort": (_i, [_i]),
     "hbs_csr_plan_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
+    "hbs_set_spmv_xseg": (_i, [_i]),
+    "hbs_csr_xseg": (_i, [_vp, _ip]),
     "hbs_shard_pack_host": (_i, [_vp, _vp, _dp]),
     "hbs_shard_spmv2": (_i, [_vp, _vp, _dp, _vp, _d, _d, _vp]),
 }
@@ -542,6 +544,9 @@ class DeviceCSR:
         w = [C.c_int() for _ in range(4)]
         check(lib().hbs_csr_plan_info(self.h, *[C.byref(x) for x in w]))
         d.update(zip(["ngroups", "nchunks", "block_nnz", "max_row"], [x.value for x in w]))
+        ns = C.c_int()
+        check(lib().hbs_csr_xseg(self.h, C.byref(ns)))
+        d["xseg"] = ns.value
         return d
 
     def spmv(self, x, y, alpha=1.0, beta=0.0):
@@ -709,6 +714,12 @@ def set_spmv_plan(block_nnz=0, serial_max_avg=64, max_groups=0, stream_nt=0, npt
     check(lib().hbs_set_spmv_groups(max_groups))
     check(lib().hbs_set_spmv_stream_nt(stream_nt))
     check(lib().hbs_set_spmv_pass(npt))
+
+
+def set_spmv_xseg(mode=2):
+    """Process-wide: x-segmented SpMV for DeviceCSR objects created
+    afterwards (0 never, 1 every one-system matrix, 2 scattered columns)."""
+    check(lib().hbs_set_spmv_xseg(mode))
 
 
 def set_vec_blocks(target=512):

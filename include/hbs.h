@@ -350,6 +350,18 @@ int hbs_set_spmv_pass(int entries_per_thread);
 /* Plan of a device CSR: SpMV workgroups, long-row chunks, nnz per row
  * block, longest row. */
 int hbs_csr_plan_info(hbs_csr a, int *ngroups, int *nchunks, int *block_nnz, int *max_row);
+/* x-segmented SpMV for matrices created afterwards.  Its first kernel
+ * stages x in segments of 8192 columns in LDS and forms every product there;
+ * its second sums them per row, in the same order as the CSR-stream SpMV (so
+ * the serial rows stay bit-identical to the sequential CSR sum).  The cost is
+ * one more pass over the entries (the products), against gathers that each
+ * miss L2 when columns spread over all of x.  mode 0: never; 1: every matrix
+ * of one system; 2 (default): one-system matrices with x above 8 MiB (FP64)
+ * whose columns touch a new 128-byte line of x for more than half the
+ * entries. */
+int hbs_set_spmv_xseg(int mode);
+/* Segments of an x-segmented device CSR (0: CSR-stream SpMV). */
+int hbs_csr_xseg(hbs_csr a, int *nseg);
 
 #ifdef __cplusplus
 }

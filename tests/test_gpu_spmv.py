@@ -103,7 +103,7 @@ def test_spmv_wide_blocks_bit_exact(ctx, block_nnz, which):
     assert info["block_nnz"] == block_nnz
     ref = orc.spmv(orc.CSR(rp, c, v), x)
     k = np.diff(rp)
-    serial = k <= min(block_nnz, 1024)   # longer rows are summed in chunks
+    serial = k <= min(block_nnz, 256)   # longer rows are summed in chunks
     assert np.array_equal(y[serial], ref[serial])
     assert np.all(np.abs(y - ref) <= seq_bound(rp, c, v, x) + 1e-300)
     ref32 = orc.spmv(orc.CSR(rp, c, v.astype(np.float32).astype(np.float64)), x32.astype(np.float64))

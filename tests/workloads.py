@@ -58,14 +58,12 @@ def dense(rp, c, v, n=None):
 def seq_bound(rp, c, v, x):
     """Per-row bound of any summation order vs another: 2*gamma_k*sum|a||x|."""
     m = rp.size - 1
-    out = np.zeros(m)
     u = np.finfo(np.float64).eps / 2
-    for i in range(m):
-        s, e = rp[i], rp[i + 1]
-        k = e - s
-        g = 2 * k * u / (1 - k * u) if k else 0.0
-        out[i] = 2 * g * np.sum(np.abs(v[s:e] * x[c[s:e]]))
-    return out
+    k = np.diff(rp).astype(np.float64)
+    g = 2 * k * u / (1 - k * u)
+    rows = np.repeat(np.arange(m), np.diff(rp))
+    s = np.bincount(rows, weights=np.abs(v * x[c]), minlength=m)
+    return 2 * g * s
 
 
 PARTITIONS = [
