@@ -494,15 +494,22 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
             (rc = dalloc(&a->xs_gbase, xh.gbase.size())) || (rc = upload(a->xs_cbase, xh.cbase)) ||
             (rc = dalloc(&a->xs_gch, xh.gch.size())) || (rc = dalloc(&a->xs_off, xh.off.size())) ||
             (rc = dalloc(&a->xs_src, (size_t)nnz)) || (rc = dalloc(&a->xs_col, (size_t)nnz)) ||
-            (rc = dalloc(&a->xs_pos, (size_t)nnz)) || (rc = dalloc(&a->xs_delta, (size_t)nnz)) ||
-            (rc = dalloc(&a->xs_val, (size_t)nnz)) || (rc = dalloc(&a->xs_prod, (size_t)nnz)) ||
+            (rc = dalloc(&a->xs_pos, padded)) || (rc = dalloc(&a->xs_delta, (size_t)nnz)) ||
+            (rc = dalloc(&a->xs_val, (size_t)nnz)) || (rc = dalloc(&a->xs_prod, padded)) ||
             (rc = upload(a->xs_sbase, xh.sbase)) || (rc = upload(a->xs_gbase, xh.gbase)) ||
             (rc = upload(a->xs_gch, xh.gch)) || (rc = upload(a->xs_off, xh.off)) || (rc = upload(a->xs_src, xh.src)) ||
             (rc = upload(a->xs_col, xh.xcol)) || (rc = upload(a->xs_pos, xh.ppos)) ||
             (rc = upload(a->xs_delta, xh.delta)) || (rc = upload(a->xs_val, xv)))
             return bail(rc);
+        // the passes read a block's products clamped to its last entry, or
+        // to entry nnz (padding) for a block without entries
+        if (hipMemset(a->xs_pos + nnz, 0, sizeof(unsigned short) * kRowPad) != hipSuccess ||
+            hipMemset(a->xs_prod + nnz, 0, sizeof(double) * kRowPad) != hipSuccess)
+            return bail(fail(HBS_ERR_HIP, "x-segment buffers"));
         if (a->val32) {
-            if ((rc = dalloc(&a->xs_val32, (size_t)nnz)) || (rc = dalloc(&a->xs_prod32, (size_t)nnz))) return bail(rc);
+            if ((rc = dalloc(&a->xs_val32, (size_t)nnz)) || (rc = dalloc(&a->xs_prod32, padded))) return bail(rc);
+            if (hipMemset(a->xs_prod32 + nnz, 0, sizeof(float) * kRowPad) != hipSuccess)
+                return bail(fail(HBS_ERR_HIP, "x-segment buffers"));
             if (launch_to_f32(nnz, a->xs_val, a->xs_val32, ctx->stream) != hipSuccess ||
                 hipStreamSynchronize(ctx->stream) != hipSuccess)
                 return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));

@@ -82,23 +82,31 @@ struct PassLoad {
     int c[NPT];
     T v[NPT];
     unsigned short pos[NPT];
+    int rp[4];   // PRE, first pass of a block: its row pointers (kRowsPerThread)
 };
 
-// Issue the stream loads of one pass (entries at or past the block's end
-// load nothing).
+// Issue the stream loads of one pass.  Every lane loads: entries at or past
+// the block's end read its last entry again (its first, padding included,
+// for a block without entries) and are ignored later.  Loads that always
+// issue let the compiler wait for exactly the ones it needs; loads behind a
+// branch make it wait for every load in flight, the next pass's included.
 template <typename T, int NPT, int TH, bool PRE>
 __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, int base,
                                            PassLoad<T, NPT> &L)
 {
-    const int lim = rb.j1 - rb.j0 - base;
+    const int hi = max(rb.j1 - rb.j0 - base - 1, 0);
     if constexpr (PRE) {
         const T *__restrict__ pr = a.prod + rb.j0 + base;
         const unsigned short *__restrict__ pp = a.ppos + rb.j0 + base;
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
-            const int k = (int)threadIdx.x + i * TH;
-            L.v[i] = k < lim ? pr[k] : T(0);
-            L.pos[i] = k < lim ? pp[k] : (unsigned short)0;
+            const int k = min((int)threadIdx.x + i * TH, hi);
+            L.v[i] = pr[k];
+            L.pos[i] = pp[k];
+        }
+        if (base == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) L.rp[q] = a.rowptr[rb.r0 + min((int)threadIdx.x + q * TH, rb.r1 - rb.r0)];
         }
         return;
     }
@@ -108,20 +116,17 @@ __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock 
     if (sorted && !a.pos_shift) {
         const unsigned short *__restrict__ pp = a.sperm + rb.j0 + base;
 #pragma unroll
-        for (int i = 0; i < NPT; ++i) {
-            const int k = (int)threadIdx.x + i * TH;
-            L.pos[i] = k < lim ? pp[k] : (unsigned short)0;
-        }
+        for (int i = 0; i < NPT; ++i) L.pos[i] = pp[min((int)threadIdx.x + i * TH, hi)];
     }
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
-        const int k = (int)threadIdx.x + i * TH;
+        const int k = min((int)threadIdx.x + i * TH, hi);
         if (a.stream_nt) {
-            L.c[i] = k < lim ? __builtin_nontemporal_load(colp + k) : 0;
-            L.v[i] = k < lim ? __builtin_nontemporal_load(valp + k) : T(0);
+            L.c[i] = __builtin_nontemporal_load(colp + k);
+            L.v[i] = __builtin_nontemporal_load(valp + k);
         } else {
-            L.c[i] = k < lim ? colp[k] : 0;
-            L.v[i] = k < lim ? valp[k] : T(0);
+            L.c[i] = colp[k];
+            L.v[i] = valp[k];
         }
     }
 }
@@ -130,6 +135,7 @@ __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock 
 // sums take one round of loads.  The block's row pointers, relative to its
 // first entry, are staged in LDS as 16-bit offsets (block_nnz <= 16384).
 constexpr int kRowsPerThread = 4;
+static_assert(kRowsPerThread == 4, "PassLoad::rp holds 4 row pointers per thread");
 template <int TH>
 constexpr int rowptr_lds_bytes()
 {
@@ -142,7 +148,7 @@ template <typename T, int EPI>
 __device__ __forceinline__ T issue_w0(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax)
 {
     const int first = ax.lanes <= 1 ? (int)threadIdx.x : (int)threadIdx.x / ax.lanes;
-    if (reads_w<EPI>() && first < rb.r1 - rb.r0) return a.w[rb.r0 + first];
+    if (reads_w<EPI>()) return a.w[rb.r0 + min(first, rb.r1 - rb.r0 - 1)];
     return T(0);
 }
 
@@ -169,10 +175,11 @@ __device__ __forceinline__ T sum_lds(const T *lds, int j0, int e, int step)
 // order from 0 (the reference's order): thread t sums rows t, t + TH, ...,
 // with their epilogue operands loaded together (the first one, w0, was
 // prefetched).  Other blocks use `lanes` lanes per row and an xor butterfly.
-template <typename T, int EPI, int TH, int B>
+template <typename T, int EPI, int TH, bool PRE>
 __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, T w0,
                                          const T *lds, const unsigned short *lrp, Acc<T> &d0, Acc<T> &d1)
 {
+    constexpr int B = PRE ? 8 : 2;
     const int nr = rb.r1 - rb.r0;
     const int lanes = ax.lanes;
     if (lanes <= 1) {
@@ -180,15 +187,21 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
-            wr[q] = q == 0 ? w0 : (reads_w<EPI>() && t < nr ? a.w[rb.r0 + t] : T(0));
+            wr[q] = q == 0 ? w0 : (reads_w<EPI>() ? a.w[rb.r0 + min(t, nr - 1)] : T(0));
+        }
+        // all sums first, then the stores: a store between two rows' LDS
+        // reads makes the next row wait for every memory operation in flight
+        // (the next pass's loads among them)
+        T acc[kRowsPerThread];
+#pragma unroll
+        for (int q = 0; q < kRowsPerThread; ++q) {
+            const int t = (int)threadIdx.x + q * TH;
+            acc[q] = t < nr ? sum_lds<T, B>(lds, lrp[t], lrp[t + 1], 1) : T(0);
         }
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
-            if (t < nr) {
-                const T acc = sum_lds<T, B>(lds, lrp[t], lrp[t + 1], 1);
-                row_out<T, EPI>(a, rb.r0 + t, acc, wr[q], d0, d1);
-            }
+            if (t < nr) row_out<T, EPI>(a, rb.r0 + t, acc[q], wr[q], d0, d1);
         }
     } else {
         const int gi = threadIdx.x / lanes, l = threadIdx.x & (lanes - 1);
@@ -277,24 +290,23 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const bool packed = cur.ax.kind == 2 && a.pos_shift;
         const unsigned cmask = packed ? (1u << a.pos_shift) - 1u : ~0u;
         if (gather && !PRE) {
-            const int lim = nz - cur.base;
+            // every lane gathers (past the block's end, at a column of the
+            // block or the padding's column 0); 32-bit byte offset from the
+            // uniform base (n * sizeof(T) < 2^32)
 #pragma unroll
-            for (int i = 0; i < NPT; ++i) {
-                const int k = (int)threadIdx.x + i * TH;
-                // 32-bit byte offset from the uniform base (n * sizeof(T) < 2^32)
-                xv[i] = k < lim ? *reinterpret_cast<const T *>(reinterpret_cast<const char *>(a.x) +
-                                                              ((unsigned)in.c[i] & cmask) * (unsigned)sizeof(T))
-                                : T(0);
-            }
+            for (int i = 0; i < NPT; ++i)
+                xv[i] = *reinterpret_cast<const T *>(reinterpret_cast<const char *>(a.x) +
+                                                     ((unsigned)in.c[i] & cmask) * (unsigned)sizeof(T));
         }
         // first pass of a block: its row pointers, staged in LDS below
+        // (PRE: loaded with the pass, one step earlier)
         int rpv[kRowsPerThread];
         const bool stage = gather && cur.base == 0;
         if (stage) {
 #pragma unroll
             for (int q = 0; q < kRowsPerThread; ++q) {
                 const int t = (int)threadIdx.x + q * TH;
-                rpv[q] = t <= cur.rb.r1 - cur.rb.r0 ? a.rowptr[cur.rb.r0 + t] : 0;
+                rpv[q] = PRE ? in.rp[q] : a.rowptr[cur.rb.r0 + min(t, cur.rb.r1 - cur.rb.r0)];
             }
         }
         Cursor nxt = cur;
@@ -327,17 +339,18 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
             }
         }
-        // PRE: one register set -- the next pass loads into it once this
-        // pass's products are in LDS, still ahead of the row sums
-        if constexpr (PRE) {
-            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, nxt.rb, nxt.ax, nxt.base, out);
-        }
         if (stage) {
 #pragma unroll
             for (int q = 0; q < kRowsPerThread; ++q) {
                 const int t = (int)threadIdx.x + q * TH;
                 if (t <= cur.rb.r1 - cur.rb.r0) lrp[t] = (unsigned short)(rpv[q] - cur.rb.j0);
             }
+        }
+        // PRE: one register set -- the next pass loads into it once this
+        // pass's products and row pointers are in LDS, still ahead of the
+        // row sums
+        if constexpr (PRE) {
+            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, nxt.rb, nxt.ax, nxt.base, out);
         }
         if (last) {
             if (!gather) {
@@ -358,7 +371,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
             } else {
                 __syncthreads();
-                sum_rows<T, EPI, TH, PRE ? 8 : 2>(a, cur.rb, cur.ax, w0, lds, lrp, d0, d1);
+                sum_rows<T, EPI, TH, PRE>(a, cur.rb, cur.ax, w0, lds, lrp, d0, d1);
                 __syncthreads();   // LDS is reused by the next row block
             }
             if (more) w0 = issue_w0<T, EPI>(a, nxt.rb, nxt.ax);
