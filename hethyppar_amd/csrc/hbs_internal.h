@@ -17,6 +17,7 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kWideThreads = 512;      // SpMV workgroup size for row blocks above 2048 entries
 constexpr int kMaxBlockNnz = 16384;    // upper bound of nnz per SpMV row block (128 KiB of FP64 LDS)
 constexpr int kLongRow = 256;          // rows longer than this are summed in chunks (k_chunks)
+constexpr int kChunkLen = 1024;        // entries per chunk of such a row
 constexpr int kRowPad = 8;             // entries of padding past nnz in col / val / val32
 constexpr int kSegCols = 8192;         // x-segmented SpMV: columns per segment (64 KiB of FP64 x in LDS)
 constexpr int kSegBlockNnz = 8192;     // x-segmented SpMV: entries per row block

@@ -21,9 +21,9 @@ int pow2_floor(int v)
 // Greedy row blocking: a block takes consecutive rows while their entries fit
 // in block_nnz (the LDS product buffer).  A row longer than
 // min(block_nnz, kLongRow) becomes one kind-1 block whose value is the
-// ordered sum of its chunk sums, each chunk (<= that many entries) summed by
-// its own workgroup of k_chunks, so no thread sums more than kLongRow
-// products in turn.
+// ordered sum of its chunk sums, each chunk (kChunkLen entries) summed by its
+// own workgroup of k_chunks, so no thread sums more than kLongRow products
+// in turn.
 // Blocks whose average row length is at most serial_max_avg use the
 // order-exact serial row sum; others pow2_floor(threads / rows) lanes per
 // row, threads = spmv_threads(block_nnz).
@@ -39,11 +39,11 @@ void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max
         const int len = rowptr[r + 1] - rowptr[r];
         plan.max_row = std::max(plan.max_row, len);
         if (len > long_row) {
-            const int nch = (len + long_row - 1) / long_row;
+            const int nch = (len + kChunkLen - 1) / kChunkLen;
             const int first = (int)plan.chunks.size();
             for (int c = 0; c < nch; ++c) {
-                const int j0 = rowptr[r] + c * long_row;
-                plan.chunks.push_back(Chunk{j0, std::min(j0 + long_row, rowptr[r + 1]), r, sys});
+                const int j0 = rowptr[r] + c * kChunkLen;
+                plan.chunks.push_back(Chunk{j0, std::min(j0 + kChunkLen, rowptr[r + 1]), r, sys});
             }
             plan.blocks.push_back(RowBlock{r, r + 1, rowptr[r], rowptr[r + 1]});
             plan.aux.push_back(BlockAux{1, first, nch, 1});
