@@ -360,7 +360,7 @@ def extras(hbs, ctx, rp, c, v, n):
     res = {}
     for mode in ("fp64", "mixed"):
         fn = S3.bicgstab if mode == "fp64" else S3.bicgstab_mixed
-        o = hbs.default_opts(tol=1e-8, maxit=5000, use_graph=2, check_every=8)
+        o = hbs.default_opts(tol=1e-8, maxit=5000, use_graph=2, check_every=4)
         dx3.zero()
         fn(db3, dx3, o, raise_on_error=False)
         ts = []
