@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="only the SpMV launches and the copy probe (for counter runs)")
+    ap.add_argument("--circuit", action="store_true",
+                    help="only the circuit-like 3.5M FP64 vs mixed solves and SpMVs (for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -88,6 +90,11 @@ def main():
 
     from hethyppar_amd import hbs
 
+    if args.circuit:
+        ctx = hbs.Context(0)
+        print(json.dumps({"circuit_3p5M": extras_circuit(hbs, ctx)}), flush=True)
+        ctx.close()
+        return
     n = args.n
     log(f"bench: generating Markov-like n={n}")
     rp, c, v = hbs.gen_markov(n, 12, 22, max(1, int(round(0.02 * n))), 1.05, 2)
@@ -329,26 +336,8 @@ def extras_adjoint(hbs, ctx):
     return res
 
 
-def extras(hbs, ctx, rp, c, v, n):
-    out = {}
-    # one rank's share: part 0 of a contiguous 4-way split (labelled substitute
-    # for the closed PaToH partition), local+halo SpMV with host halo
-    log("bench: shard part 0 of 4")
-    pv = (np.arange(n, dtype=np.int64) * 4 // n).astype(np.int32)
-    hs = hbs.shard_build(rp, c, v, pv, 4, 0)
-    sh = hbs.Shard(ctx, hs)
-    Am = sh.csr()
-    m, ns = sh.m_loc, sh.n_shr
-    xe = ctx.to_device(np.random.default_rng(1).uniform(-1, 1, m + ns))
-    ye = ctx.empty(m)
-    ms = Am.bench_spmv(xe, ye, 50)
-    Bs = spmv_bytes(m, m + ns, sh.nnz)
-    out["shard_part0"] = {"m_loc": m, "n_shr": ns, "n_send": sh.n_send, "nnz": sh.nnz,
-                          "spmv_ms": round(ms, 5), "spmv_gbs": round(Bs / (ms * 1e-3) / 1e9, 1)}
-    sh.close()
-    hbs.shard_free(hs)
-
-    # mixed precision vs FP64 on the circuit-like matrix
+def extras_circuit(hbs, ctx):
+    """FP64 vs mixed BiCGStab and the SpMV on the 3.5M-row circuit-like matrix."""
     N3 = 3500000
     log("bench: circuit-like n=3.5M, FP64 vs mixed")
     rp3, c3, v3 = hbs.gen_circuit(N3, 4.2, 2.1, 200, 1000, 50000, 1.1, 3, equilibrate=True)
@@ -382,9 +371,31 @@ def extras(hbs, ctx, rp, c, v, n):
     res["nnz"] = nnz3
     res["spmv_fp64"] = {"ms": round(ms64, 5), "gbs": round(spmv_bytes(N3, N3, nnz3) / (ms64 * 1e-3) / 1e9, 1)}
     res["spmv_fp32"] = {"ms": round(ms32, 5), "gbs": round(spmv_bytes(N3, N3, nnz3, 4) / (ms32 * 1e-3) / 1e9, 1)}
-    out["circuit_3p5M"] = res
     S3.close()
     A3.close()
+    return res
+
+
+def extras(hbs, ctx, rp, c, v, n):
+    out = {}
+    # one rank's share: part 0 of a contiguous 4-way split (labelled substitute
+    # for the closed PaToH partition), local+halo SpMV with host halo
+    log("bench: shard part 0 of 4")
+    pv = (np.arange(n, dtype=np.int64) * 4 // n).astype(np.int32)
+    hs = hbs.shard_build(rp, c, v, pv, 4, 0)
+    sh = hbs.Shard(ctx, hs)
+    Am = sh.csr()
+    m, ns = sh.m_loc, sh.n_shr
+    xe = ctx.to_device(np.random.default_rng(1).uniform(-1, 1, m + ns))
+    ye = ctx.empty(m)
+    ms = Am.bench_spmv(xe, ye, 50)
+    Bs = spmv_bytes(m, m + ns, sh.nnz)
+    out["shard_part0"] = {"m_loc": m, "n_shr": ns, "n_send": sh.n_send, "nnz": sh.nnz,
+                          "spmv_ms": round(ms, 5), "spmv_gbs": round(Bs / (ms * 1e-3) / 1e9, 1)}
+    sh.close()
+    hbs.shard_free(hs)
+
+    out["circuit_3p5M"] = extras_circuit(hbs, ctx)
     return out
 
 
