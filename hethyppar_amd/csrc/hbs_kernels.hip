@@ -85,6 +85,18 @@ struct PassLoad {
     int rp[4];   // PRE, first pass of a block: its row pointers (kRowsPerThread)
 };
 
+// PRE: the block's row pointers, with its first pass's loads (staged in LDS
+// when that pass is consumed).  Later passes load the block's first row
+// pointer in every lane instead (one request per wave), so no branch guards
+// them.
+template <typename T, int NPT, int TH>
+__device__ __forceinline__ void issue_rowptr(const SpmvArgs<T> &a, const RowBlock &rb, int base, PassLoad<T, NPT> &L)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        L.rp[q] = a.rowptr[rb.r0 + (base == 0 ? min((int)threadIdx.x + q * TH, rb.r1 - rb.r0) : 0)];
+}
+
 // Issue the stream loads of one pass.  Every lane loads: entries at or past
 // the block's end read its last entry again (its first, padding included,
 // for a block without entries) and are ignored later.  Loads that always
@@ -104,10 +116,7 @@ __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock 
             L.v[i] = pr[k];
             L.pos[i] = pp[k];
         }
-        if (base == 0) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) L.rp[q] = a.rowptr[rb.r0 + min((int)threadIdx.x + q * TH, rb.r1 - rb.r0)];
-        }
+        issue_rowptr<T, NPT, TH>(a, rb, base, L);
         return;
     }
     const bool sorted = ax.kind == 2;
@@ -298,16 +307,15 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 xv[i] = *reinterpret_cast<const T *>(reinterpret_cast<const char *>(a.x) +
                                                      ((unsigned)in.c[i] & cmask) * (unsigned)sizeof(T));
         }
-        // first pass of a block: its row pointers, staged in LDS below
-        // (PRE: loaded with the pass, one step earlier)
+        // first pass of a block: its row pointers, staged in LDS below (PRE:
+        // loaded with the pass, one step earlier)
         int rpv[kRowsPerThread];
         const bool stage = gather && cur.base == 0;
         if (stage) {
 #pragma unroll
-            for (int q = 0; q < kRowsPerThread; ++q) {
-                const int t = (int)threadIdx.x + q * TH;
-                rpv[q] = PRE ? in.rp[q] : a.rowptr[cur.rb.r0 + min(t, cur.rb.r1 - cur.rb.r0)];
-            }
+            for (int q = 0; q < kRowsPerThread; ++q)
+                rpv[q] = PRE ? in.rp[q]
+                             : a.rowptr[cur.rb.r0 + min((int)threadIdx.x + q * TH, cur.rb.r1 - cur.rb.r0)];
         }
         Cursor nxt = cur;
         if (last) {
