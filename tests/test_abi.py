@@ -54,3 +54,5 @@ def test_plan_knob_validation_without_device():
     assert L.hbs_set_spmv_groups(-1) == -1 and b"max_groups" in L.hbs_last_error()
     assert L.hbs_set_spmv_groups(0) == 0
     assert L.hbs_set_spmv_plan(0, 64) == 0
+    assert L.hbs_set_spmv_xseg(3) == -1 and b"mode" in L.hbs_last_error()
+    assert L.hbs_set_spmv_xseg(2) == 0
