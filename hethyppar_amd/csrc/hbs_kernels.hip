@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "hbs_device.h"
 #include "hbs_internal.h"
@@ -857,18 +859,32 @@ hipError_t spmv_epi(const SpmvArgs<T> &a, int epi, size_t shmem, hipStream_t s)
     return hipGetLastError();
 }
 
-// Allow the dynamic LDS of the wide blocks (above the 64 KiB default).
+// Raise a kernel's dynamic LDS limit to at least `bytes` (above the 64 KiB
+// default).  Limits only grow, so a plan made later with smaller blocks does
+// not lower the limit an earlier plan's launches need.
+hipError_t raise_lds_limit(const void *f, size_t bytes)
+{
+    static std::mutex mu;
+    static std::map<const void *, size_t> limit;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t &have = limit[f];
+    if (bytes <= have) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) have = bytes;
+    return e;
+}
+
+// Allow the dynamic LDS of the wide blocks.
 template <typename T, int NPT, int TH, bool PRE = false>
 hipError_t spmv_allow_lds(size_t shmem)
 {
-    const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
     hipError_t e = hipSuccess;
     const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_DOT1, NPT, TH, PRE>,
                        (const void *)k_spmv<T, EPI_DOT2, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_RESID, NPT, TH, PRE>,
                        (const void *)k_spmv<T, EPI_RESID2, NPT, TH, PRE>,
                        (const void *)k_spmv<T, EPI_RESIDB, NPT, TH, PRE>};
     for (const void *f : k)
-        if (e == hipSuccess) e = hipFuncSetAttribute(f, at, (int)shmem);
+        if (e == hipSuccess) e = raise_lds_limit(f, shmem);
     return e;
 }
 
@@ -972,12 +988,9 @@ size_t xseg_lds_bytes(int elem_bytes, int max_chunks, int max_granules)
 }
 hipError_t prepare_xseg(int max_chunks, int max_granules)
 {
-    const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
-    hipError_t e = hipFuncSetAttribute((const void *)k_xseg<double, kXsegNpt>, at,
-                                       (int)xseg_lds_bytes(8, max_chunks, max_granules));
+    hipError_t e = raise_lds_limit((const void *)k_xseg<double, kXsegNpt>, xseg_lds_bytes(8, max_chunks, max_granules));
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_xseg<float, kXsegNpt>, at,
-                                (int)xseg_lds_bytes(4, max_chunks, max_granules));
+        e = raise_lds_limit((const void *)k_xseg<float, kXsegNpt>, xseg_lds_bytes(4, max_chunks, max_granules));
     return e;
 }
 template <typename T>

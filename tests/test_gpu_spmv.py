@@ -112,6 +112,27 @@ def test_spmv_wide_blocks_bit_exact(ctx, block_nnz, which):
     assert np.all(np.abs(y32 - ref32) <= bound * 1.01 + 1e-30)
 
 
+def test_spmv_lds_limit_kept_for_earlier_plans(ctx):
+    """A plan with 16384-entry blocks (132 KiB of LDS per workgroup), then
+    one with 8192: the first matrix's SpMV still launches (the kernels' LDS
+    limits only grow)."""
+    rp, c, v = markov(50000)
+    n = rp.size - 1
+    x = np.random.default_rng(9).uniform(-1, 1, n)
+    try:
+        hbs.set_spmv_plan(16384, 1 << 20)
+        A1 = hbs.DeviceCSR(ctx, rp, c, v)
+        hbs.set_spmv_plan(8192, 1 << 20)
+        A2 = hbs.DeviceCSR(ctx, rp, c, v)
+    finally:
+        hbs.set_spmv_plan()
+    ref = orc.spmv(orc.CSR(rp, c, v), x)
+    dx, dy = ctx.to_device(x), ctx.empty(n)
+    for A in (A1, A2, A1):
+        A.spmv(dx, dy)
+        assert np.array_equal(dy.download(), ref)
+
+
 def test_spmv_alpha_beta_and_empty_rows(ctx):
     rp, c, v = random_csr(500, 400, 0.01, 9)   # has empty rows
     assert (np.diff(rp) == 0).any()
