@@ -31,6 +31,8 @@
 #include <random>
 #include <vector>
 
+#include <omp.h>
+
 #include "hbs_err.h"
 #include "hbs_internal.h"
 
@@ -50,15 +52,44 @@ struct Hyper {
     std::vector<long long> cw;
     std::vector<int> cost;
 
+    // Cells -> nets, each cell's nets ascending: a counting sort of the pins.
+    // Large hypergraphs split the nets into runs of about equal pins, one per
+    // thread; a cell's slots go to the runs in order, so the lists are the
+    // same for any thread count.
     void build_incidence()
     {
+        const int T = pin.size() < ((size_t)1 << 20) ? 1 : std::max(1, std::min(omp_get_max_threads(), 32));
+        std::vector<int> run((size_t)T + 1, nn);
+        run[0] = 0;
+        for (int t = 1, e = 0; t < T; ++t) {
+            const long long want = (long long)pin.size() * t / T;
+            while (e < nn && xp[(size_t)e] < want) ++e;
+            run[(size_t)t] = e;
+        }
+        std::vector<int> cnt((size_t)T * nc, 0);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+        for (int t = 0; t < T; ++t) {
+            int *ct = cnt.data() + (size_t)t * nc;
+            for (int j = xp[(size_t)run[(size_t)t]]; j < xp[(size_t)run[(size_t)t + 1]]; ++j) ct[pin[(size_t)j]]++;
+        }
         xn.assign((size_t)nc + 1, 0);
-        for (int p : pin) xn[(size_t)p + 1]++;
-        for (int c = 0; c < nc; ++c) xn[(size_t)c + 1] += xn[(size_t)c];
+        for (int c = 0; c < nc; ++c) {
+            int s = xn[(size_t)c];
+            for (int t = 0; t < T; ++t) {
+                int &k = cnt[(size_t)t * nc + c];
+                const int here = k;
+                k = s;
+                s += here;
+            }
+            xn[(size_t)c + 1] = s;
+        }
         net.resize(pin.size());
-        std::vector<int> fill(xn.begin(), xn.end() - 1);
-        for (int e = 0; e < nn; ++e)
-            for (int j = xp[(size_t)e]; j < xp[(size_t)e + 1]; ++j) net[(size_t)fill[(size_t)pin[(size_t)j]]++] = e;
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+        for (int t = 0; t < T; ++t) {
+            int *ct = cnt.data() + (size_t)t * nc;
+            for (int e = run[(size_t)t]; e < run[(size_t)t + 1]; ++e)
+                for (int j = xp[(size_t)e]; j < xp[(size_t)e + 1]; ++j) net[(size_t)ct[pin[(size_t)j]]++] = e;
+        }
     }
     long long total_weight() const { return std::accumulate(cw.begin(), cw.end(), 0LL); }
 };
