@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -303,6 +304,10 @@ bool graphs(const hbs_csr a, int use_graph)
     return use_graph != 0;
 }
 
+// Iterations at least this long (host-measured, between polls) use the
+// optimistic convergence prediction in iterate().
+constexpr double kLongIterUs = 150.0;
+
 // Queue iteration chunks until the device states say every system is done.
 // One chunk is kept queued ahead of the one whose state the host reads, so
 // the GPU never idles on the poll -- until the residual history of a single
@@ -338,6 +343,12 @@ int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use
     bool ahead = true;
     double rr_prev = -1.0;
     int it_prev = 0;
+    // wall time per iteration, from the polls: long iterations make a chunk
+    // queued past convergence cost more than a poll the GPU waits for
+    using clk = std::chrono::steady_clock;
+    clk::time_point t_prev{};
+    int it_time_prev = -1;
+    double us_per_iter = 0.0;
     CHK(queue(next));
     for (;;) {
         if (ahead && nfly < 2 && issued < maxit) CHK(queue((int)std::min<long>(next, maxit - issued)));
@@ -352,9 +363,22 @@ int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use
         if (done || (issued >= maxit && nfly == 0)) break;
         ahead = true;
         next = k;
+        const clk::time_point t_now = clk::now();
+        if (it_time_prev >= 0 && h->iter > it_time_prev)
+            us_per_iter = std::chrono::duration<double, std::micro>(t_now - t_prev).count() / (h->iter - it_time_prev);
+        t_prev = t_now;
+        it_time_prev = h->iter;
         if (s->nsys == 1 && !h->fixed && h->iter > 0 && h->rr > 0.0 && h->thr > 0.0) {
-            if (rr_prev > 0.0 && h->iter > it_prev && h->rr < rr_prev) {
-                const double per = std::log(h->rr / rr_prev) / (h->iter - it_prev);
+            // log decay of ||r||^2 per iteration over the last interval;
+            // for long iterations, the faster of that and the rate since the
+            // start (BiCGStab residuals are irregular, and there a chunk
+            // queued past convergence costs more than a missed poll)
+            double per = 0.0;
+            if (rr_prev > 0.0 && h->iter > it_prev && h->rr < rr_prev)
+                per = std::log(h->rr / rr_prev) / (h->iter - it_prev);
+            if (us_per_iter >= kLongIterUs && h->rr0 > 0.0 && h->rr < h->rr0)
+                per = std::min(per, std::log(h->rr / h->rr0) / h->iter);
+            if (per < 0.0) {
                 const double need = std::log(h->thr * h->thr / h->rr) / per;   // iterations after h
                 const double left = need - (nfly ? fly_len[0] : 0);
                 if (left <= 0.5) ahead = false;
