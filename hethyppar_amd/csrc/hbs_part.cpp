@@ -424,6 +424,7 @@ std::vector<char> bisect(const Hyper &h, double frac1, double eps, std::mt19937 
     const Hyper &hc = levels.back();
     std::vector<char> best;
     long long best_cut = -1;
+    bool best_ok = false;
     const int tries = hc.nc <= 4 * kCoarsestCells && hc.pin.size() <= 200000 ? kInitTries : 1;
     for (int t = -1; t < tries; ++t) {
         std::vector<char> side;
@@ -443,10 +444,12 @@ std::vector<char> bisect(const Hyper &h, double frac1, double eps, std::mt19937 
         const long long cut = fm_refine(hc, side, maxw, kFmPasses);
         long long w1 = 0;
         for (int c = 0; c < hc.nc; ++c) w1 += side[(size_t)c] ? hc.cw[(size_t)c] : 0;
+        // a balanced split beats any unbalanced one; then the lower cut
         const bool ok = w1 <= maxw[1] && W - w1 <= maxw[0];
-        if (best_cut < 0 || (ok && cut < best_cut)) {
+        if (best_cut < 0 || (ok && !best_ok) || (ok == best_ok && cut < best_cut)) {
             best_cut = cut;
             best = side;
+            best_ok = ok;
         }
     }
     // uncoarsen with refinement
