@@ -555,33 +555,20 @@ long long partition_cut(int n, const int *rowptr, const int *col, const int *par
 int partition_rows(int n, const int *rowptr, const int *col, int nparts, const double *target, double final_imbal,
                    int seed, int *partvec)
 {
+    // rows are nets over their columns; single-pin nets (a row whose only
+    // column is its own diagonal) cannot be cut and are dropped
     Hyper h;
     h.nc = n;
-    h.nn = n;
-    h.xp.assign(rowptr, rowptr + n + 1);
-    h.pin.assign(col, col + rowptr[n]);
-    h.cost.assign((size_t)n, 1);
     h.cw.resize((size_t)n);
     for (int r = 0; r < n; ++r) h.cw[(size_t)r] = std::max(1, rowptr[r + 1] - rowptr[r]);
-    // drop single-pin nets (a row whose only column is its own diagonal)
-    {
-        Hyper g;
-        g.nc = n;
-        g.cw = h.cw;
-        g.xp.push_back(0);
-        for (int e = 0; e < n; ++e) {
-            const size_t start = g.pin.size();
-            for (int j = h.xp[(size_t)e]; j < h.xp[(size_t)e + 1]; ++j) g.pin.push_back(h.pin[(size_t)j]);
-            if (g.pin.size() - start < 2) {
-                g.pin.resize(start);
-                continue;
-            }
-            g.cost.push_back(1);
-            g.xp.push_back((int)g.pin.size());
-        }
-        g.nn = (int)g.cost.size();
-        h = std::move(g);
+    h.xp.push_back(0);
+    for (int r = 0; r < n; ++r) {
+        if (rowptr[r + 1] - rowptr[r] < 2) continue;
+        h.pin.insert(h.pin.end(), col + rowptr[r], col + rowptr[r + 1]);
+        h.cost.push_back(1);
+        h.xp.push_back((int)h.pin.size());
     }
+    h.nn = (int)h.cost.size();
     h.build_incidence();
     std::vector<double> t(target, target + nparts);
     std::vector<int> cells((size_t)n);
