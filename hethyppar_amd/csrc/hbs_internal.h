@@ -311,7 +311,10 @@ hipError_t prepare_xseg(int max_chunks, int max_granules);
 // Entries per thread per pass of the x-segmented row sums (k_spmv<PRE>): one
 // pass covers a row block of kSegBlockNnz entries.
 constexpr int kPreNpt = kSegBlockNnz / kWideThreads;
-constexpr int kXsegNpt = 8;            // k_xseg entries per thread per pass
+// k_xseg entries per thread per pass: as many as the registers allow with
+// two workgroups per CU, so each workgroup keeps more bytes in flight
+template <typename T>
+constexpr int xseg_npt() { return sizeof(T) == 8 ? 12 : 16; }
 constexpr int kXsegMaxLds = 96 * 1024; // k_xseg LDS cap (x segment + chunk tables); above it no x-segmenting
 hipError_t launch_vec_P(const VecArgs<double> &a, hipStream_t s);
 hipError_t launch_vec_S(const VecArgs<double> &a, hipStream_t s);

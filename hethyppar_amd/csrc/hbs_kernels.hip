@@ -988,9 +988,9 @@ size_t xseg_lds_bytes(int elem_bytes, int max_chunks, int max_granules)
 }
 hipError_t prepare_xseg(int max_chunks, int max_granules)
 {
-    hipError_t e = raise_lds_limit((const void *)k_xseg<double, kXsegNpt>, xseg_lds_bytes(8, max_chunks, max_granules));
+    hipError_t e = raise_lds_limit((const void *)k_xseg<double, xseg_npt<double>()>, xseg_lds_bytes(8, max_chunks, max_granules));
     if (e == hipSuccess)
-        e = raise_lds_limit((const void *)k_xseg<float, kXsegNpt>, xseg_lds_bytes(4, max_chunks, max_granules));
+        e = raise_lds_limit((const void *)k_xseg<float, xseg_npt<float>()>, xseg_lds_bytes(4, max_chunks, max_granules));
     return e;
 }
 template <typename T>
@@ -999,7 +999,7 @@ hipError_t xseg_dispatch(const XsegArgs<T> &a, hipStream_t s)
     if (a.nseg <= 0) return hipSuccess;
     if (a.seg_cols != kSegCols) return hipErrorInvalidValue;
     const size_t lds = xseg_lds_bytes((int)sizeof(T), a.max_chunks, a.max_granules);
-    k_xseg<T, kXsegNpt><<<a.nseg, kWideThreads, lds, s>>>(a);
+    k_xseg<T, xseg_npt<T>()><<<a.nseg, kWideThreads, lds, s>>>(a);
     return hipGetLastError();
 }
 hipError_t launch_xseg(const XsegArgs<double> &a, hipStream_t s) { return xseg_dispatch<double>(a, s); }
