@@ -34,8 +34,28 @@ def _check_f64(rp, c, v, x, y, serial_max=256):
     assert np.all(np.abs(y - ref) <= seq_bound(rp, c, v, x) + 1e-300)
 
 
+def _segment_counts_csr(counts, m, seed):
+    """m-row CSR whose column segment s (columns [8192 s, 8192 (s + 1)))
+    holds exactly counts[s] entries, at distinct random positions."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for s, k in enumerate(counts):
+        flat = rng.choice(m * 8192, size=k, replace=False)
+        rows.append(flat // 8192)
+        cols.append(s * 8192 + flat % 8192)
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    order = np.lexsort((c, r))
+    r, c = r[order], c[order]
+    rp = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=m))]).astype(np.int32)
+    return rp, c.astype(np.int32), rng.standard_normal(c.size)
+
+
 def _cases():
     out = []
+    # segments of exactly one k_xseg pass (12 x 512 FP64, 16 x 512 FP32
+    # entries), one more, one fewer, two passes, a single entry and none
+    counts = [6144, 8192, 6145, 8191, 12288, 1, 0]
+    out.append(("segment_passes", *_segment_counts_csr(counts, 3000, 17), 8192 * len(counts)))
     out.append(("circuit60k", *hbs.gen_circuit(60000, nheavy=12, heavy_lo=300, heavy_hi=30000, seed=11), None))
     out.append(("markov50k", *markov(50000), None))
     out.append(("random_rect", *random_csr(3000, 20001, 0.002, 5), 20001))   # empty rows, n % 8192 != 0
