@@ -1,7 +1,7 @@
 """BiCGStab on the MI355X (FP64 drop-in and mixed FP32/FP64 refinement)
 against the oracle's restatement of the reference CPU solver.
 
-Bar (north star): iteration count within +-2 of the oracle's, relative
+Bar: iteration count within +-2 of the oracle's, relative
 residual sqrt(||b-Ax||^2/||b||^2) <= 1e-8 (FP64, with 1% slack for the
 stopping iterate), solution within 1e-6 relative of the oracle's.
 """
