@@ -85,7 +85,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
-              "(this build runs one rank's share on one GPU)", file=sys.stderr)
+              "(one rank's share runs on one GPU)", file=sys.stderr)
         sys.exit(2)
 
     from hethyppar_amd import hbs

@@ -5,7 +5,7 @@
  * CPU algorithms (HetHypPar, src/util/hhp_cpu.c, src/util/hhp_matrix.c,
  * src/util/hhp_wrap_bora.c, src/entry/bicgstab_cpu.c, src/entry/bicgstab_gpu.c)
  * used to check the HIP path.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load it, and only as the checker: it is
+ * cpu_baseline leg of bench.py may load it, and only to check results: it is
  * never the thing measured as the GPU product and never a fallback for it.
  *
  * Every function cites the reference file:line it follows.  Arithmetic is

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
 Test infrastructure only: tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py use it as the checker / CPU baseline.  It is
+cpu_baseline leg of bench.py use it to check results / as the CPU baseline.  It is
 never a fallback for the HIP path.
 """
 import ctypes as C
