@@ -361,6 +361,10 @@ namespace {
 int validate_csr(int m, int n, int nnz, const int *rowptr, const int *col, const double *val)
 {
     if (m < 0 || n < 0 || nnz < 0) return fail(HBS_ERR_ARG, "negative dimension");
+    // the SpMV gathers x through 32-bit byte offsets from its base
+    if ((long long)n * (long long)sizeof(double) > (long long)UINT32_MAX)
+        return fail(HBS_ERR_ARG, "n = %d columns exceed the SpMV's 32-bit x offsets (at most %lld)", n,
+                    (long long)(UINT32_MAX / sizeof(double)));
     if (!rowptr || (nnz > 0 && (!col || !val))) return fail(HBS_ERR_ARG, "null CSR array");
     if (rowptr[0] != 0 || rowptr[m] != nnz) return fail(HBS_ERR_ARG, "rowptr[0] must be 0 and rowptr[m] == nnz");
     for (int i = 0; i < m; ++i)

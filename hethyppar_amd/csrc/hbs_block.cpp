@@ -181,7 +181,7 @@ int block_iterate(const BlockArgs &g, BlockWork *w, int c0, int maxit, int check
         HIPCHK(hipMemcpyAsync(w->hst + (size_t)(c & 1) * kBlockCols, w->st, bytes, hipMemcpyDeviceToHost, q));
         HIPCHK(hipEventRecord(w->ev[c & 1], q));
         if (pending >= 0) {
-            HIPCHK(hipEventSynchronize(w->ev[pending & 1]));
+            CHK(wait_event_limited(w->ev[pending & 1]));
             const SolverState *h = w->hst + (size_t)(pending & 1) * kBlockCols;
             bool all = true;
             for (int i = 0; i < g.kc && all; ++i) all = h[i].done != 0;
@@ -189,7 +189,7 @@ int block_iterate(const BlockArgs &g, BlockWork *w, int c0, int maxit, int check
         }
         pending = c++;
         if (issued >= maxit) {
-            HIPCHK(hipEventSynchronize(w->ev[pending & 1]));
+            CHK(wait_event_limited(w->ev[pending & 1]));
             break;
         }
     }

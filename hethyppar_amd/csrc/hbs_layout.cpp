@@ -303,6 +303,61 @@ extern "C" int hbs_vector_write(const char *fname, int size, const double *v)
     return HBS_OK;
 }
 
+// Rank `rank`'s slice of a global vector: the entries of the rows the
+// partition gives it, in ascending row order (reference
+// MPI_vector_read_parted, hhp_cpu.c:252-334: rank 0 orders the vector by
+// part, then by row, and Scatterv hands each rank its run).
+extern "C" int hbs_vector_part(int size, const double *big, const int *partvec, int nparts, int rank, double *out,
+                               int *nloc)
+{
+    if (size < 0 || nparts < 1 || rank < 0 || rank >= nparts || (size > 0 && (!big || !partvec)))
+        return lfail(HBS_ERR_ARG);
+    int k = 0;
+    for (int i = 0; i < size; ++i) {
+        const int p = partvec[i];
+        if (p < 0 || p >= nparts) return lfail(HBS_ERR_PARTITION);
+        if (p == rank) {
+            if (out) out[k] = big[i];
+            ++k;
+        }
+    }
+    if (nloc) *nloc = k;
+    return HBS_OK;
+}
+
+extern "C" int hbs_vector_read_parted(const char *fname, int size, const int *partvec, int nparts, int rank,
+                                      double **out, int *nloc)
+{
+    if (!fname || !out || !nloc || size < 0) return lfail(HBS_ERR_ARG);
+    *out = nullptr;
+    *nloc = 0;
+    std::vector<double> big((size_t)size);
+    int rc = hbs_vector_read(fname, size, big.data());
+    if (rc) return rc;
+    int k = 0;
+    if ((rc = hbs_vector_part(size, big.data(), partvec, nparts, rank, nullptr, &k))) return rc;
+    double *o = static_cast<double *>(std::malloc(sizeof(double) * (size_t)std::max(k, 1)));
+    if (!o) return lfail(HBS_ERR_ALLOC);
+    hbs_vector_part(size, big.data(), partvec, nparts, rank, o, &k);
+    *out = o;
+    *nloc = k;
+    return HBS_OK;
+}
+
+// One rank's part of the gather back to a global vector: big[gind[k]] =
+// loc[k] for its nloc owned entries (reference MPI_vector_gather,
+// hhp_cpu.c:199-250, whose root Gathervs every rank's loc values and gind
+// and scatters them the same way).  Calling it for every rank's share
+// assembles the whole vector.
+extern "C" int hbs_vector_gather(int nloc, const double *loc, const int *gind, int size, double *big)
+{
+    if (nloc < 0 || size < 0 || (nloc > 0 && (!loc || !gind || !big))) return lfail(HBS_ERR_ARG);
+    for (int k = 0; k < nloc; ++k)
+        if (gind[k] < 0 || gind[k] >= size) return lfail(HBS_ERR_ARG);
+    for (int k = 0; k < nloc; ++k) big[gind[k]] = loc[k];
+    return HBS_OK;
+}
+
 // y = A x with each row summed from 0.0 in CSR order (the reference
 // CSR_spmxv_seq order, hhp_cpu.c:445-455).
 extern "C" int hbs_host_csr_spmv(int m, const int *rowptr, const int *col, const double *val,

@@ -159,6 +159,10 @@ struct hbs_solver_s {
 
 namespace hbs {
 
+// Host wait for an event, bounded by the solver wait limit
+// (hbs_set_wait_limit): HBS_ERR_HIP when it passes or the stream fails.
+int wait_event_limited(hipEvent_t ev);
+
 // SpMV launch arguments of matrix a with values val (FP64 or FP32 copy).
 template <typename T>
 SpmvArgs<T> spmv_args(hbs_csr a, const T *val)

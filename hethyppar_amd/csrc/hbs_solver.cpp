@@ -230,16 +230,17 @@ int run_chunk(hbs_solver s, const Work<T> &w, int k, int slot, bool use_graph, i
     hipStream_t q = s->A->ctx->stream;
     const int gi = sizeof(T) == 8 ? 0 : 1;
     *stamp = 0;
-    if (!use_graph) {
+    // Chunks longer than the kept graphs launch their kernels one by one: an
+    // executable graph made for one launch could only be destroyed once that
+    // launch has finished, which the host learns only later.
+    if (!use_graph || k >= hbs_solver_s::kGraphLens) {
         s->seq = s->seq == INT_MAX ? 1 : s->seq + 1;
         *stamp = s->seq;
         HIPCHK(launch_chunk<T>(s, w, k, slot, s->seq));
         return HBS_OK;
     }
-    // one graph per chunk length and poll slot, kept; longer chunks are captured each time
-    const bool keep = k < hbs_solver_s::kGraphLens;
-    hipGraphExec_t scratch_graph = nullptr;
-    hipGraphExec_t &ge = keep ? s->graph[gi][k][slot] : scratch_graph;
+    // one graph per chunk length and poll slot, captured once and kept
+    hipGraphExec_t &ge = s->graph[gi][k][slot];
     if (!ge) {
         hipGraph_t gr = nullptr;
         HIPCHK(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
@@ -255,16 +256,21 @@ int run_chunk(hbs_solver s, const Work<T> &w, int k, int slot, bool use_graph, i
         }
     }
     HIPCHK(hipGraphLaunch(ge, q));
-    if (!keep) HIPCHK(hipGraphExecDestroy(ge));
     return HBS_OK;
 }
 
+// Longest host wait for one chunk's stamp before the solve gives up with
+// HBS_ERR_HIP (hbs_set_wait_limit).
+std::atomic<double> g_wait_limit_s{600.0};
+
 // Wait until every mirrored state of poll slot `slot` carries `stamp`
 // (written by the chunk's last kernel after the states), checking the
-// stream for errors now and then.
+// stream for errors and the time limit now and then.
 int wait_stamp(hbs_solver s, int slot, int stamp)
 {
     const volatile SolverState *h = s->hst + (size_t)slot * s->nsys;
+    const auto t0 = std::chrono::steady_clock::now();
+    const double limit = g_wait_limit_s.load();
     for (long spin = 0;; ++spin) {
         bool all = true;
         for (int i = 0; i < s->nsys && all; ++i) all = h[i].pad == stamp;
@@ -282,10 +288,38 @@ int wait_stamp(hbs_solver s, int slot, int stamp)
                     if (h[i].pad != stamp) return fail(HBS_ERR_HIP, "chunk state stamp %d missing", stamp);
                 return HBS_OK;
             }
+            const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (waited > limit)
+                return fail(HBS_ERR_HIP, "no chunk state after %.3f s (wait limit %.3f s); work is still queued",
+                            waited, limit);
             std::this_thread::yield();
         }
     }
 }
+
+}  // namespace
+
+// Wait for an event recorded after a chunk, within the same time limit as
+// wait_stamp.
+int hbs::wait_event_limited(hipEvent_t ev)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const double limit = g_wait_limit_s.load();
+    for (long spin = 0;; ++spin) {
+        const hipError_t e = hipEventQuery(ev);
+        if (e == hipSuccess) return HBS_OK;
+        if (e != hipErrorNotReady) return fail(HBS_ERR_HIP, "solver stream failed: %s", hipGetErrorString(e));
+        if ((spin & 63) == 63) {
+            const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (waited > limit)
+                return fail(HBS_ERR_HIP, "no chunk state after %.3f s (wait limit %.3f s); work is still queued",
+                            waited, limit);
+            std::this_thread::yield();
+        }
+    }
+}
+
+namespace {
 
 bool all_done(const SolverState *h, int nsys)
 {
@@ -353,7 +387,7 @@ int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use
     for (;;) {
         if (ahead && nfly < 2 && issued < maxit) CHK(queue((int)std::min<long>(next, maxit - issued)));
         if (fly_stamp[0]) CHK(wait_stamp(s, fly_slot[0], fly_stamp[0]));
-        else HIPCHK(hipEventSynchronize(s->ev[fly_slot[0]]));
+        else CHK(wait_event_limited(s->ev[fly_slot[0]]));
         const SolverState *h = s->hst + (size_t)fly_slot[0] * s->nsys;
         const bool done = all_done(h, s->nsys);
         fly_slot[0] = fly_slot[1];
@@ -512,8 +546,12 @@ int solve_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts &o,
         HIPCHK(hipEventRecord(s->e1, q));
         return HBS_OK;
     };
+    // x aliasing b: no tail ahead of the convergence the host has seen, since
+    // its copy of the iterate into x would overwrite the b that a later tail
+    // (after a misprediction) measures the residual against
     bool tail_done = false;
-    CHK(iterate<double>(s, w, std::max(o.maxit, 1), o.check_every, use_graphs, tail, &tail_done));
+    CHK(iterate<double>(s, w, std::max(o.maxit, 1), o.check_every, use_graphs,
+                        x == b ? std::function<int()>{} : std::function<int()>{tail}, &tail_done));
     if (!tail_done) CHK(tail());
     std::vector<SolverState> hs((size_t)s->nsys);
     CHK(read_states(s, kSlotA, hs.data()));
@@ -553,6 +591,13 @@ int worst_system(const std::vector<hbs_solve_info> &infos)
 }
 
 }  // namespace
+
+extern "C" int hbs_set_wait_limit(double seconds)
+{
+    if (!(seconds > 0.0)) return fail(HBS_ERR_ARG, "wait limit must be positive");
+    g_wait_limit_s.store(seconds);
+    return HBS_OK;
+}
 
 extern "C" int hbs_bicgstab_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts *opts,
                                 hbs_solve_info *info)

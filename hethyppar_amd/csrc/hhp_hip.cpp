@@ -228,11 +228,75 @@ extern "C" int device_SHARD_CSC_destroy(Device_SHARD_CSC *A)
     return rc;
 }
 
+namespace {
+
+// The shard built the reference driver's way (bicgstab_mpi_gpu.c:248-257:
+// device_csc_create of A.loc and A.shr, gind/send/recv borrowed, no merged
+// shard): the two SpMVs of MPI_device_SHARD_CSC_mpi_spmxv (hhp_cuda.c:
+// 259-352), Y = 0; Y = alpha*loc*dX + beta*Y; Y = alpha*shr*x_shr + alpha*Y,
+// with the halo placed by recv.J (x_shr[recv.J[p]] = recv.val[p], the CPU
+// semantics of hhp_cpu.c:574-584) and A.send.val = X[send.J] packed on the
+// host from a copy of dX, as the reference does.
+int shard_spmxv_two_pass(const Device_SHARD_CSC &A, Device_Vector dX, Device_Vector dX_shr, Device_Vector Y,
+                         double alpha, double beta, SpMV_Profile *prof)
+{
+    if (!A.loc.desc) return hbs::set_error(HBS_ERR_ARG, "shard has neither a merged form nor a loc block");
+    hbs_ctx ctx = hhp_default_handle();
+    if (!ctx) return HBS_ERR_NODEVICE;
+    const int nloc = A.loc.data.n, m = A.loc.data.m, nshr = A.shr.desc ? A.shr.data.n : 0;
+    if ((int)dX.nvals < nloc || (int)Y.nvals < m || (nshr > 0 && (int)dX_shr.nvals < nshr))
+        return hbs::set_error(HBS_ERR_ARG, "vector shorter than the shard");
+    const int nsend = A.send.num > 0 ? A.send.I[A.send.num] : 0;
+    const int nrecv = A.recv.num > 0 ? A.recv.I[A.recv.num] : 0;
+    if (nrecv != nshr) return hbs::set_error(HBS_ERR_ARG, "recv list has %d entries for %d halo columns", nrecv, nshr);
+    double ms = 0.0;
+    int rc = hbs_dev_zero(ctx, Y.vals, sizeof(double) * (size_t)m);
+    if (rc) return rc;
+    if (nsend > 0) {
+        if (prof) hbs_timer_start(ctx);
+        std::vector<double> xh((size_t)nloc);
+        if ((rc = hbs_d2h(ctx, xh.data(), dX.vals, sizeof(double) * (size_t)nloc))) return rc;
+        for (int i = 0; i < nsend; ++i) {
+            const int k = A.send.J[i];
+            if (k < 0 || k >= nloc) return hbs::set_error(HBS_ERR_ARG, "send.J[%d] = %d out of range", i, k);
+            A.send.val[i] = xh[(size_t)k];
+        }
+        if (prof) {
+            hbs_timer_stop(ctx, &ms);
+            prof->send_fill += ms * 1e-3;
+        }
+    }
+    if (prof) hbs_timer_start(ctx);
+    if ((rc = hbs_csr_spmv_f64(A.loc.desc, alpha, dX.vals, beta, Y.vals))) return rc;
+    if (prof) {
+        hbs_timer_stop(ctx, &ms);
+        prof->local_spmv += ms * 1e-3;
+    }
+    if (nshr > 0) {
+        if (prof) hbs_timer_start(ctx);
+        std::vector<double> xs((size_t)nshr);
+        for (int p = 0; p < nrecv; ++p) {
+            const int k = A.recv.J[p];
+            if (k < 0 || k >= nshr) return hbs::set_error(HBS_ERR_ARG, "recv.J[%d] = %d out of range", p, k);
+            xs[(size_t)k] = A.recv.val[p];
+        }
+        if ((rc = hbs_h2d(ctx, dX_shr.vals, xs.data(), sizeof(double) * (size_t)nshr))) return rc;
+        if ((rc = hbs_csr_spmv_f64(A.shr.desc, alpha, dX_shr.vals, alpha, Y.vals))) return rc;
+        if (prof) {
+            hbs_timer_stop(ctx, &ms);
+            prof->shared_spmv += ms * 1e-3;
+        }
+    }
+    return hbs_ctx_sync(ctx);
+}
+
+}  // namespace
+
 extern "C" int device_SHARD_CSC_spmxv(Device_SHARD_CSC A, Vector, Device_Vector dX, Device_Vector dX_shr,
                                       Device_Vector Y, hhp_handle_t, const double alpha, const double beta,
                                       Device_Buffer_SpMV, Device_Buffer_SpMV, SpMV_Profile *prof)
 {
-    if (!A.dev) return hbs::set_error(HBS_ERR_ARG, "shard not created");
+    if (!A.dev) return shard_spmxv_two_pass(A, dX, dX_shr, Y, alpha, beta, prof);
     hbs_ctx ctx = hhp_default_handle();
     int nsend = A.send.num > 0 ? A.send.I[A.send.num] : 0;
     if (prof) hbs_timer_start(ctx);

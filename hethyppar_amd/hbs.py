@@ -145,6 +145,10 @@ _PROTOS = {
     "hbs_set_spmv_xseg": (_i, [_i]),
     "hbs_csr_xseg": (_i, [_vp, _ip]),
     "hbs_shard_pack_host": (_i, [_vp, _vp, _dp]),
+    "hbs_vector_part": (_i, [_i, _dp, _ip, _i, _i, _dp, _ip]),
+    "hbs_vector_read_parted": (_i, [C.c_char_p, _i, _ip, _i, _i, _dpp, _ip]),
+    "hbs_vector_gather": (_i, [_i, _dp, _ip, _i, _dp]),
+    "hbs_set_wait_limit": (_i, [_d]),
     "hbs_shard_spmv2": (_i, [_vp, _vp, _dp, _vp, _d, _d, _vp]),
 }
 
@@ -310,6 +314,39 @@ def gen_uneven_systems(nsys, lo_exp=3.0, hi_exp=5.0, seed=5):
         rp, c, v = gen_convdiff2d(side, side, float(u[2 * q + 1]))
         out.append((rp, c, v, float(u[2 * q + 1])))
     return out
+
+
+def vector_part(big, partvec, nparts, rank):
+    """Rank `rank`'s slice of a global vector (entries of its rows in
+    ascending row order), as MPI_vector_read_parted hands it out."""
+    big = np.ascontiguousarray(big, dtype=np.float64)
+    pv = np.ascontiguousarray(partvec, dtype=np.int32)
+    k = C.c_int()
+    check(lib().hbs_vector_part(big.size, _dp_of(big), _ip_of(pv), nparts, rank, None, C.byref(k)))
+    out = np.zeros(k.value, dtype=np.float64)
+    check(lib().hbs_vector_part(big.size, _dp_of(big), _ip_of(pv), nparts, rank, _dp_of(out), C.byref(k)))
+    return out
+
+
+def vector_read_parted(path, size, partvec, nparts, rank):
+    pv = np.ascontiguousarray(partvec, dtype=np.int32)
+    out, k = _dp(), C.c_int()
+    check(lib().hbs_vector_read_parted(os.fsencode(path), size, _ip_of(pv), nparts, rank, C.byref(out),
+                                       C.byref(k)))
+    return _take(out, k.value, np.float64)
+
+
+def vector_gather(loc, gind, big):
+    """big[gind[k]] = loc[k] for this rank's entries (in place; returns big)."""
+    loc = np.ascontiguousarray(loc, dtype=np.float64)
+    gind = np.ascontiguousarray(gind, dtype=np.int32)
+    assert big.dtype == np.float64 and big.flags.c_contiguous
+    check(lib().hbs_vector_gather(loc.size, _dp_of(loc), _ip_of(gind[:loc.size]), big.size, _dp_of(big)))
+    return big
+
+
+def set_wait_limit(seconds):
+    check(lib().hbs_set_wait_limit(float(seconds)))
 
 
 def host_spmv(rowptr, col, val, x):

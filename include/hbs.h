@@ -66,8 +66,10 @@ typedef struct hbs_csr_s *hbs_csr;
 /* Deep-copies a host CSR (rowptr[m+1], col[nnz], val[nnz], 0-based int32
  * indices, FP64 values) to HBM and builds the row-block plan of the CSR-stream
  * SpMV.  flags: HBS_CSR_F32 also keeps an FP32 copy of the values (mixed
- * precision); HBS_CSR_TRANSPOSE also keeps A^T as CSR (adjoint).
- * Replaces device_csr_create (reference src/util/hhp_cuda.c:19-51). */
+ * precision); HBS_CSR_TRANSPOSE also keeps A^T as CSR (adjoint).  n is at
+ * most 536,870,911 (x is gathered through 32-bit byte offsets); larger is
+ * HBS_ERR_ARG.  Replaces device_csr_create (reference
+ * src/util/hhp_cuda.c:19-51). */
 #define HBS_CSR_F32       1
 #define HBS_CSR_TRANSPOSE 2
 int hbs_csr_create(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr,
@@ -138,6 +140,10 @@ typedef struct {
 } hbs_solve_info;
 
 void hbs_solve_opts_default(hbs_solve_opts *o);
+/* Longest host wait, in seconds, for one chunk of iterations before a solve
+ * gives up with HBS_ERR_HIP (default 600; process-wide).  Work already
+ * queued then still runs: synchronise the context before reusing buffers. */
+int hbs_set_wait_limit(double seconds);
 
 typedef struct hbs_solver_s *hbs_solver;
 /* Workspace for BiCGStab on A (vectors, device scalars, graphs).  Reusable. */
@@ -145,7 +151,8 @@ int hbs_solver_create(hbs_csr a, hbs_solver *out);
 int hbs_solver_destroy(hbs_solver s);
 /* FP64 BiCGStab, the drop-in for the reference loop (bicgstab_cpu.c:172-255,
  * bicgstab_gpu_dp.c:145-208) with the stop test of bicgstab_gpu.c:148-150,
- * 231-236, 289-293.  b_dev, x_dev: device; x holds x0 on entry.  On a
+ * 231-236, 289-293.  b_dev, x_dev: device; x holds x0 on entry and may
+ * be b itself (x0 = b, b is overwritten by the solution).  On a
  * matrix of several systems every system is solved at once and info gets
  * the most iterations, the largest residuals and the worst status. */
 int hbs_bicgstab_f64(hbs_solver s, const double *b_dev, double *x_dev,
@@ -218,6 +225,22 @@ int hbs_mm_read(const char *fname, int *m, int *n, int *nnz,
 int hbs_vector_read(const char *fname, int size, double *out);
 /* "%lf\n" per value (reference vector_write, hhp_cpu.c:430-442) */
 int hbs_vector_write(const char *fname, int size, const double *v);
+/* Rank `rank`'s slice of a global vector big[size]: out[k] = big[i_k] for
+ * the rows i_0 < i_1 < ... with partvec[i] == rank, which is the order of
+ * the shard's owned columns (gind[0..loc.n)).  Replaces
+ * MPI_vector_read_parted (hhp_cpu.c:252-334) minus the Scatterv.  out may be
+ * NULL to count; *nloc (may be NULL) receives the count. */
+int hbs_vector_part(int size, const double *big, const int *partvec, int nparts, int rank, double *out,
+                    int *nloc);
+/* The same from a vector file (vector_read format); *out is malloc'd
+ * (hbs_host_free). */
+int hbs_vector_read_parted(const char *fname, int size, const int *partvec, int nparts, int rank,
+                           double **out, int *nloc);
+/* One rank's part of the gather back: big[gind[k]] = loc[k], k < nloc.
+ * Replaces MPI_vector_gather (hhp_cpu.c:199-250), whose root Gathervs every
+ * rank's values and gind and scatters them this way; called for every
+ * rank's share it assembles the global vector. */
+int hbs_vector_gather(int nloc, const double *loc, const int *gind, int size, double *big);
 
 /* ---- synthetic workloads (SplitMix64 streams, one per row) ----------- */
 /* 2D 5-point convection-diffusion on nx*ny: diag 4+conv, west -1-conv,

@@ -112,10 +112,21 @@ int device_csc_spmv(hhp_handle_t handle, Device_CSC mat, Device_Vector X, Device
 int device_SHARD_CSC_create(SHARD_CSC A, Device_SHARD_CSC *out);
 int device_SHARD_CSC_destroy(Device_SHARD_CSC *A);
 /* MPI_device_SHARD_CSC_mpi_spmxv (hhp_cuda.c:259-352) without the MPI
- * transport: A.send.val <- dX[send.J] (for the caller to send), then
- * Y = alpha*(loc*dX + shr*x_shr) + beta*Y with x_shr[recv.J[p]] =
- * A.recv.val[p] (the CPU semantics of hhp_cpu.c:574-584).  dX_shr receives
- * x_shr.  X is unused (kept for the reference signature).  prof may be NULL. */
+ * transport: A.send.val <- dX[send.J] (for the caller to send); the halo is
+ * A.recv.val (recv order, as MPI delivered it), placed by x_shr[recv.J[p]] =
+ * A.recv.val[p] (the CPU semantics of hhp_cpu.c:574-584); dX_shr receives
+ * x_shr.  X is unused (kept for the reference signature); prof may be NULL.
+ *   - A from device_SHARD_CSC_create (A.dev set): one pass over the merged
+ *     local+halo rows, Y = alpha*(loc*dX + shr*x_shr) + beta*Y, each row
+ *     summed loc entries first, then shr entries in recv order -- bit for
+ *     bit the CPU MPI_SHARD_CSC_mpi_spmxv_seq (hhp_cpu.c:514-601) when
+ *     alpha = 1 and beta = 0, the coefficients every reference driver
+ *     passes.
+ *   - A built as the reference driver builds it (bicgstab_mpi_gpu.c:248-257:
+ *     device_csc_create of A.loc and, if it has columns, A.shr; A.dev NULL):
+ *     the reference's two SpMVs, Y = 0, Y = alpha*loc*dX + beta*Y, then
+ *     Y = alpha*shr*x_shr + alpha*Y (so alpha reaches the loc part twice,
+ *     as in the reference). */
 int device_SHARD_CSC_spmxv(Device_SHARD_CSC A, Vector X, Device_Vector dX, Device_Vector dX_shr,
                            Device_Vector Y, hhp_handle_t handle, const double alpha,
                            const double beta, Device_Buffer_SpMV locbuf, Device_Buffer_SpMV shrbuf,

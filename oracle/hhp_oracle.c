@@ -588,6 +588,30 @@ void orc_vector_read_parted(const double *big, const int *partvec, int n, int ra
         if (partvec[i] == rank) out[k++] = big[i];
 }
 
+/* src/util/hhp_cpu.c:199-250 (MPI_vector_gather), every rank in one call:
+ * the root's two Gathervs concatenate the ranks' local values and their gind
+ * prefixes in rank order (counts[r] entries each, displacements the running
+ * sums), then big[gmap[i]] = temp_big[i] for i in order. */
+void orc_vector_gather(int nparts, const int *counts, const double *const *locs, const int *const *ginds,
+                       int size, double *big)
+{
+    int total = 0;
+    for (int r = 0; r < nparts; r++) total += counts[r];
+    double *temp_big = malloc(sizeof(double) * (total > 0 ? total : 1));
+    int *gmap = malloc(sizeof(int) * (total > 0 ? total : 1));
+    int d = 0;
+    for (int r = 0; r < nparts; r++) {
+        for (int k = 0; k < counts[r]; k++) {
+            temp_big[d + k] = locs[r][k];
+            gmap[d + k] = ginds[r][k];
+        }
+        d += counts[r];
+    }
+    for (int i = 0; i < total && i < size; i++) big[gmap[i]] = temp_big[i];
+    free(temp_big);
+    free(gmap);
+}
+
 /* src/util/hhp_cpu.c:514-601 (MPI_SHARD_CSC_mpi_spmxv_seq) with the halo
  * already received: y = 0; y += loc*x (column scatter, :563); then every recv
  * block in rank order (the MPI_Waitany arrival order taken as ascending), for

@@ -35,6 +35,11 @@
  *                          in the reference, so halo lists rest on these.
  *   orc_shard_spmv         equals global CSR SpMV rows within rounding; exact
  *                          when a row has no shared columns
+ *   orc_vector_read_parted /
+ *   orc_vector_gather      brute force on small inputs; slice then gather is
+ *                          the identity over every rank of the reference
+ *                          partitions; a planted swap of two gind entries
+ *                          must break the round trip
  */
 #ifndef HHP_ORACLE_H
 #define HHP_ORACLE_H
@@ -126,6 +131,8 @@ void orc_shard_free(orc_shard *s);
 
 void orc_vector_read_parted(const double *big, const int *partvec, int n,
                             int rank, double *out);
+void orc_vector_gather(int nparts, const int *counts, const double *const *locs, const int *const *ginds,
+                       int size, double *big);
 void orc_shard_spmv(const orc_shard *s, const double *x_loc,
                     const double *recv_buf, double *y);
 void orc_shard_pack(const orc_shard *s, const double *x_loc, double *send_buf);

@@ -57,6 +57,8 @@ def lib():
         L.orc_build_shards.argtypes = [P(OrcCSR), _ip, C.c_int, P(OrcShard)]
         L.orc_shard_free.argtypes = [P(OrcShard)]
         L.orc_vector_read_parted.argtypes = [_dp, _ip, C.c_int, C.c_int, _dp]
+        L.orc_vector_gather.argtypes = [C.c_int, _ip, C.POINTER(_dp), C.POINTER(_ip), C.c_int, _dp]
+        L.orc_vector_gather.restype = None
         L.orc_shard_spmv.argtypes = [P(OrcShard), _dp, _dp, _dp]
         L.orc_shard_pack.argtypes = [P(OrcShard), _dp, _dp]
         _lib = L
@@ -262,3 +264,16 @@ def vector_read_parted(big, partvec, rank):
     out = np.zeros(int((pv == rank).sum()), dtype=np.float64)
     lib().orc_vector_read_parted(_dp_of(big), _ip_of(pv), big.size, rank, _dp_of(out))
     return out
+
+
+def vector_gather(locs, ginds, size, big=None):
+    """Every rank's (local values, gind prefix) in rank order -> the global
+    vector (MPI_vector_gather at the root); entries no rank owns keep big's."""
+    big = np.zeros(size) if big is None else np.array(big, dtype=np.float64, copy=True)
+    locs = [np.ascontiguousarray(a, dtype=np.float64) for a in locs]
+    ginds = [np.ascontiguousarray(g, dtype=np.int32) for g in ginds]
+    counts = np.array([a.size for a in locs], dtype=np.int32)
+    lp = (_dp * len(locs))(*[_dp_of(a) for a in locs])
+    gp = (_ip * len(ginds))(*[_ip_of(g) for g in ginds])
+    lib().orc_vector_gather(len(locs), _ip_of(counts), lp, gp, size, _dp_of(big))
+    return big

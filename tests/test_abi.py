@@ -56,3 +56,34 @@ def test_plan_knob_validation_without_device():
     assert L.hbs_set_spmv_plan(0, 64) == 0
     assert L.hbs_set_spmv_xseg(3) == -1 and b"mode" in L.hbs_last_error()
     assert L.hbs_set_spmv_xseg(2) == 0
+
+
+def test_wait_limit_knob_validation_without_device():
+    L = hbs.lib()
+    assert L.hbs_set_wait_limit(ctypes.c_double(0.0)) == -1 and b"wait limit" in L.hbs_last_error()
+    assert L.hbs_set_wait_limit(ctypes.c_double(-1.0)) == -1
+    assert L.hbs_set_wait_limit(ctypes.c_double(600.0)) == 0
+
+
+def test_csr_create_rejects_columns_past_32bit_x_offsets():
+    """The SpMV gathers x through 32-bit byte offsets, so a matrix with
+    n * 8 > 2^32 - 1 columns is refused before anything touches a device.
+    (The context handle is never dereferenced on this path: argument checks
+    come first.)"""
+    L = hbs.lib()
+    rp = (ctypes.c_int * 2)(0, 0)
+    out = ctypes.c_void_p()
+    fake_ctx = ctypes.c_void_p(1)
+    for n, ok in ((1 << 29, False), ((1 << 29) + 5, False)):
+        rc = L.hbs_csr_create(fake_ctx, 1, n, 0, rp, None, None, 0, ctypes.byref(out))
+        assert rc == -1 and b"32-bit" in L.hbs_last_error()
+    rc = L.hbs_csr_create(fake_ctx, -1, 4, 0, rp, None, None, 0, ctypes.byref(out))
+    assert rc == -1
+
+
+def test_plain_c_caller_compiles_against_headers(tmp_path):
+    """A C11 program written against include/hhp_hip.h and hhp_dp_kernels.h
+    (the reference harness' call sequence) compiles warning-free and links."""
+    from c_caller import build_caller
+    exe = build_caller(tmp_path)
+    assert os.path.exists(exe)

@@ -12,34 +12,14 @@ import pytest
 import pyoracle as orc
 from conftest import golden
 from hethyppar_amd import hbs
+from parity import check as _check
 from workloads import sample320
 
+# Most systems or columns must meet the plain bar; at most this fraction may
+# need the order-sensitive one (parity.check), and each that does is printed.
+MAX_ORDER_SENSITIVE = 0.25
+
 pytestmark = pytest.mark.gpu
-
-
-def _check(x, info, xo, ito, what, A=None, b=None, xt=None):
-    print(f"{what}: gpu iters {info.iters} stop {info.stop} relres {info.relres:.3e} | oracle iters {ito}",
-          flush=True)
-    assert info.status == 0
-    assert info.relres <= 1.01e-8
-    dist = np.linalg.norm(x - xo) / np.linalg.norm(xo)
-    if (abs(info.iters - ito) > 2 or dist > 1e-6) and A is not None:
-        # an order-sensitive system: the oracle's own count and stopping
-        # iterate move when its dot sums are re-associated (as every parallel
-        # reduction re-associates them).  The bar: that range of counts +-2,
-        # and a forward error (against the known solution xt) within 4x the
-        # worst of the re-associated oracle runs (all stop at relres 1e-8,
-        # where the forward error is only bounded by cond(A) * 1e-8)
-        _, lo, hi, sols = orc.iteration_spread(A, b, maxit=2000)
-        fe = lambda y: np.linalg.norm(y - xt) / np.linalg.norm(xt)
-        worst = max(fe(y) for y in sols)
-        print(f"{what}: re-associated oracle runs span {lo}..{hi} iterations, forward errors up to "
-              f"{worst:.2e}; gpu {fe(x):.2e} ({dist:.2e} from the oracle)", flush=True)
-        assert lo - 2 <= info.iters <= hi + 2
-        assert fe(x) <= 4 * worst
-    else:
-        assert abs(info.iters - ito) <= 2
-        assert dist <= 1e-6
 
 
 def _systems(nsys, lo, hi, seed):
@@ -60,14 +40,17 @@ def test_uneven_systems_each_match_oracle(ctx):
     dx.zero()
     infos = S.bicgstab_systems(db, dx, tol=1e-8, maxit=2000)
     x = dx.download()
-    its = []
+    its, loose = [], []
     for q, (srp, sc, sv, conv) in enumerate(sysl):
         r0, r1 = rows[q], rows[q + 1]
         Aq = orc.CSR(srp, sc, sv)
         xo, ito, _, _ = orc.bicgstab(Aq, b[r0:r1], tol=1e-8, maxit=2000)
-        _check(x[r0:r1], infos[q], xo, ito, f"system {q} n={r1 - r0} conv={conv:.2f}", Aq, b[r0:r1],
-               xt[r0:r1])
+        if _check(x[r0:r1], infos[q], xo, ito, f"system {q} n={r1 - r0} conv={conv:.2f}", Aq, b[r0:r1],
+                  xt[r0:r1]):
+            loose.append(q)
         its.append(infos[q].iters)
+    print(f"order-sensitive bar used by systems {loose} of {len(sysl)}", flush=True)
+    assert len(loose) <= MAX_ORDER_SENSITIVE * len(sysl)
     assert len(set(its)) > 1                        # systems stopped at their own iteration
 
 
@@ -142,14 +125,18 @@ def test_block_columns_match_oracle(ctx, k):
     dX = ctx.to_device(np.zeros(n * k))
     infos = S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=1000)
     X = dX.download().reshape(n, k)
+    loose = []
     for j in range(k):
         if j == k // 2:
             assert infos[j].iters == 0 and infos[j].stop == 2 and np.all(X[:, j] == 0.0)
             continue
         bj = np.ascontiguousarray(B[:, j])
         xo, ito, _, _ = orc.bicgstab(orc.CSR(rp, c, v), bj, tol=1e-8, maxit=1000)
-        _check(X[:, j], infos[j], xo, ito, f"column {j} of {k}", orc.CSR(rp, c, v), bj,
-               np.ascontiguousarray(Xt[:, j]))
+        if _check(X[:, j], infos[j], xo, ito, f"column {j} of {k}", orc.CSR(rp, c, v), bj,
+                  np.ascontiguousarray(Xt[:, j])):
+            loose.append(j)
+    print(f"order-sensitive bar used by columns {loose} of {k}", flush=True)
+    assert len(loose) <= MAX_ORDER_SENSITIVE * k
 
 
 def test_block_fixed_iterations_golden(ctx):

@@ -40,7 +40,7 @@ def test_convdiff64_fp64(ctx):
     _check(x, info, xo, ito)
 
 
-@pytest.mark.parametrize("graph,every", [(0, 1), (1, 4), (1, 8), (0, 16), (2, 4)])
+@pytest.mark.parametrize("graph,every", [(0, 1), (1, 4), (1, 8), (0, 16), (2, 4), (1, 40), (0, 40)])
 def test_convdiff64_deterministic_any_schedule(ctx, graph, every):
     rp, c, v, xt, b = convdiff64()
     x_ref, i_ref = _solve(ctx, rp, c, v, b, use_graph=1, check_every=8)
@@ -152,3 +152,35 @@ def test_direct_iterate_in_caller_x_matches_graphs(ctx):
         assert np.array_equal(runs[k][1], runs["direct"][1])
     assert runs["alias"][0].iters == runs["x0=b"][0].iters
     assert np.array_equal(runs["alias"][1], runs["x0=b"][1])
+
+
+def test_wait_limit_gives_up_then_solver_recovers(ctx):
+    """A host wait longer than hbs_set_wait_limit returns HBS_ERR_HIP instead
+    of spinning on; the queued chunks still finish normally, and after a
+    context sync the same solver gives the full solve again.  Forced from the
+    host with a 1 us limit against chunks of 32 iterations (~2 ms each) on
+    the 445k-row system, with the stamp poll (kernels launched one by one)
+    and with events (graphs)."""
+    rp, c, v = markov(445000)
+    n = rp.size - 1
+    b = hbs.host_spmv(rp, c, v, hbs.gen_uniform(n, -1.0, 1.0, 12))
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    S = hbs.Solver(A)
+    db, dx = ctx.to_device(b), ctx.empty(n)
+    dx.zero()
+    ref = S.bicgstab(db, dx, tol=1e-8, maxit=1000)
+    x_ref = dx.download()
+    for graph in (0, 1):
+        dx.zero()
+        try:
+            hbs.set_wait_limit(1e-6)
+            with pytest.raises(hbs.HBSError) as e:
+                S.bicgstab(db, dx, tol=1e-8, maxit=1000, use_graph=graph, check_every=32)
+        finally:
+            hbs.set_wait_limit(600.0)
+        print(f"graph {graph}: {e.value}", flush=True)
+        assert e.value.code == -2 and "wait limit" in str(e.value)
+        ctx.sync()
+        dx.zero()
+        info = S.bicgstab(db, dx, tol=1e-8, maxit=1000, use_graph=graph)
+        assert info.iters == ref.iters and np.array_equal(dx.download(), x_ref)

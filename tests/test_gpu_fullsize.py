@@ -1,0 +1,233 @@
+"""The workloads the throughput figures are quoted on, at their full sizes, on
+the MI355X against the oracle.
+
+  * circuit-like, n = 3,500,000 (rows equilibrated: each row divided by its
+    diagonal; unpreconditioned BiCGStab stagnates near 1e-5 on the raw
+    matrix, in the oracle too): the x-segmented SpMV bit for bit on the
+    serial rows, the FP64 solve to the single-system bar, and the mixed
+    solve to the FP64 residual bar with a forward error no worse than the
+    oracle's.
+  * FEM-like, n = 1,450,002, 52.2M entries: the forward solve to the bar;
+    the adjoint's lambda against the oracle's BiCGStab on A^T (iterations
+    +-2, within 1e-6); dL/dA bit for bit as -lambda_i x_j; central
+    differences (h = 1e-6) of L on 32 entries of A and 8 of b.
+  * 128^3 7-point convection-diffusion with 64 right-hand sides: every
+    column's relative residual recomputed with the oracle's SpMV, and two
+    columns against the oracle's solve.
+  * 256 uneven 2-D systems in one matrix: every system against the oracle.
+
+The oracle runs on the host's cores, several solves at a time in threads
+(ctypes releases the GIL during its calls).
+"""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from hethyppar_amd import hbs
+from parity import check
+from workloads import seq_bound
+
+pytestmark = pytest.mark.gpu
+
+# at most this fraction of systems / columns may need the order-sensitive bar
+MAX_ORDER_SENSITIVE = 0.25
+POOL = 8
+
+
+def _pool():
+    return ThreadPoolExecutor(max_workers=POOL)
+
+
+@pytest.mark.timeout(600)
+def test_circuit_3p5M_spmv_fp64_and_mixed(ctx):
+    n = 3500000
+    rp, c, v = hbs.gen_circuit(n, 4.2, 2.1, 200, 1000, 50000, 1.1, 3, equilibrate=True)
+    xt = hbs.gen_uniform(n, -1.0, 1.0, 33)
+    b = hbs.host_spmv(rp, c, v, xt)
+    Ao = orc.CSR(rp, c, v)
+    with _pool() as ex:
+        fut = ex.submit(orc.bicgstab, Ao, b, None, 1e-8, 5000)
+        A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+        assert A.info()["xseg"] > 0, "the circuit matrix takes the x-segmented SpMV"
+        # SpMV: rows of at most 256 entries summed serially in CSR order
+        dxt, dy = ctx.to_device(xt), ctx.empty(n)
+        A.spmv(dxt, dy)
+        y = dy.download()
+        ref = orc.spmv(Ao, xt)
+        k = np.diff(rp)
+        serial = k <= 256
+        print(f"circuit: {c.size} entries, {int((~serial).sum())} rows above 256 entries", flush=True)
+        assert np.array_equal(y[serial], ref[serial])
+        assert np.all(np.abs(y - ref) <= seq_bound(rp, c, v, xt) + 1e-300)
+        S = hbs.Solver(A)
+        db, dx = ctx.to_device(b), ctx.empty(n)
+        dx.zero()
+        info = S.bicgstab(db, dx, tol=1e-8, maxit=5000)
+        x = dx.download()
+        dx.zero()
+        im = S.bicgstab_mixed(db, dx, tol=1e-8, maxit=5000)
+        xm = dx.download()
+        xo, ito, rro, _ = fut.result()
+    check(x, info, xo, ito, "circuit 3.5M FP64")
+    e_m = np.linalg.norm(xm - xt) / np.linalg.norm(xt)
+    e_o = np.linalg.norm(xo - xt) / np.linalg.norm(xt)
+    d = np.linalg.norm(xm - xo) / np.linalg.norm(xo)
+    print(f"circuit 3.5M mixed: {im.iters} iterations in {im.sweeps} sweeps, relres {im.relres:.3e}; "
+          f"forward error {e_m:.2e} (oracle {e_o:.2e}); relative error vs the FP64 oracle {d:.2e}", flush=True)
+    assert im.status == 0 and im.relres <= 1.01e-8
+    assert e_m <= 2 * max(e_o, 1e-7)
+    # the residual recomputed on the host from the mixed iterate
+    r = b - orc.spmv(Ao, xm)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-8
+
+
+@pytest.mark.timeout(600)
+def test_fem_1p45M_forward_adjoint_and_central_differences(ctx):
+    rp, c, v = hbs.gen_fem(483334, 12, 1.1, 6)
+    n, nnz = rp.size - 1, int(c.size)
+    b = hbs.host_spmv(rp, c, v, hbs.gen_uniform(n, -1.0, 1.0, 61))
+    xt = hbs.gen_uniform(n, -1.0, 1.0, 62)
+    Ao = orc.CSR(rp, c, v)
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_TRANSPOSE)
+    S = hbs.Solver(A)
+    db, dx = ctx.to_device(b), ctx.empty(n)
+    with _pool() as ex:
+        fwd = ex.submit(orc.bicgstab, Ao, b, None, 1e-8, 2000)
+        tp, tc, tv = orc.csr_to_csc(Ao)          # CSC of A == CSR of A^T
+        dx.zero()
+        info = S.bicgstab(db, dx, tol=1e-8, maxit=2000)
+        x = dx.download()
+        g = x - xt
+        dg, dl, dgv = ctx.to_device(g), ctx.empty(n), ctx.empty(nnz)
+        ainfo = S.adjoint(dg, dx, dl, dgv, tol=1e-8, maxit=2000)
+        lam, gv = dl.download(), dgv.download()
+        xo, ito, _, _ = fwd.result()
+        lo, ilo, _, _ = orc.bicgstab(orc.CSR(tp, tc, tv), g, tol=1e-8, maxit=2000)
+    check(x, info, xo, ito, "FEM 1.45M forward")
+    print(f"FEM adjoint: gpu iters {ainfo.iters} relres {ainfo.relres:.3e} | oracle on A^T iters {ilo}", flush=True)
+    assert ainfo.status == 0 and abs(ainfo.iters - ilo) <= 2 and ainfo.relres <= 1.01e-8
+    assert np.linalg.norm(lam - lo) / np.linalg.norm(lo) <= 1e-6
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    assert np.array_equal(gv, -(lam[rows] * x[c]))
+
+    # central differences.  x_base solved to 1e-13; each perturbed system is
+    # solved from x_base to 1e-10 of its own initial residual (which is the
+    # perturbation, ~h), so x+ - x- is accurate far below the O(h^2)
+    # truncation.  L+ - L- = (x+ - x-) . (x+ + x- - 2 xt) / 2.
+    dx.zero()
+    S.bicgstab(db, dx, tol=1e-13, maxit=2000)
+    x_base = dx.download()
+    dg.upload(x_base - xt)
+    S.adjoint(dg, dx, dl, dgv, tol=1e-13, maxit=2000)
+    lam, gv = dl.download(), dgv.download()
+    dval = ctx.to_device(v)
+    dxp = ctx.empty(n)
+    h = 1e-6
+
+    def solve_from_base(dbb):
+        dxp.upload(x_base)
+        S.bicgstab(dbb, dxp, tol=1e-10, maxit=2000)
+        return dxp.download()
+
+    def fd(xp, xm):
+        return float(np.dot(xp - xm, (xp + xm) - 2.0 * xt)) / (4.0 * h)
+
+    rng = np.random.default_rng(6)
+    gmax = np.max(np.abs(gv))
+    worst = 0.0
+    for j in rng.choice(nnz, 32, replace=False):
+        sols = []
+        for s in (h, -h):
+            vv = np.array([v[j] + s])
+            hbs.check(hbs.lib().hbs_h2d(ctx.h, dval.offset(int(j)), vv.ctypes.data, 8))
+            A.set_values(dval)
+            sols.append(solve_from_base(db))
+        hbs.check(hbs.lib().hbs_h2d(ctx.h, dval.offset(int(j)), v[j:j + 1].ctypes.data, 8))
+        A.set_values(dval)
+        est = fd(*sols)
+        err = abs(est - gv[j]) / max(abs(gv[j]), 1e-3 * gmax)
+        worst = max(worst, err)
+        assert err <= 1e-5, f"dL/dA entry {j}: fd {est:.9e} adjoint {gv[j]:.9e}"
+    lmax = np.max(np.abs(lam))
+    for i in rng.choice(n, 8, replace=False):
+        sols = []
+        for s in (h, -h):
+            bb = b.copy()
+            bb[i] += s
+            sols.append(solve_from_base(ctx.to_device(bb)))
+        est = fd(*sols)
+        err = abs(est - lam[i]) / max(abs(lam[i]), 1e-3 * lmax)
+        worst = max(worst, err)
+        assert err <= 1e-5, f"dL/db entry {i}: fd {est:.9e} adjoint {lam[i]:.9e}"
+    print(f"FEM central differences (32 A entries, 8 b entries): worst relative gap {worst:.2e}", flush=True)
+
+
+@pytest.mark.timeout(600)
+def test_block_128cube_64_rhs(ctx):
+    rp, c, v = hbs.gen_convdiff3d(128, 128, 128, 0.5)
+    n, k = rp.size - 1, 64
+    Xt = hbs.gen_uniform(n * k, -1.0, 1.0, 4).reshape(n, k)
+    Ao = orc.CSR(rp, c, v)
+    cols = [np.ascontiguousarray(Xt[:, j]) for j in range(k)]
+    with _pool() as ex:
+        B = np.stack(list(ex.map(lambda xj: orc.spmv(Ao, xj), cols)), axis=1)
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    S = hbs.Solver(A)
+    dB, dX = ctx.to_device(np.ascontiguousarray(B).ravel()), ctx.empty(n * k)
+    dX.zero()
+    sample = (0, 37)
+    with _pool() as ex:
+        futs = {j: ex.submit(orc.bicgstab, Ao, np.ascontiguousarray(B[:, j]), None, 1e-8, 2000) for j in sample}
+        infos = S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=2000, use_graph=1, check_every=8)
+        X = dX.download().reshape(n, k)
+        xcols = [np.ascontiguousarray(X[:, j]) for j in range(k)]
+        res = list(ex.map(lambda j: np.linalg.norm(B[:, j] - orc.spmv(Ao, xcols[j])) / np.linalg.norm(B[:, j]),
+                          range(k)))
+        oracle = {j: f.result() for j, f in futs.items()}
+    its = [i.iters for i in infos]
+    print(f"128^3 x 64: iterations {min(its)}..{max(its)}, host-recomputed relres up to {max(res):.3e}", flush=True)
+    for j in range(k):
+        assert infos[j].status == 0 and infos[j].relres <= 1.01e-8
+        assert res[j] <= 1.01e-8, f"column {j}: relres recomputed on the host {res[j]:.3e}"
+    loose = []
+    for j in sample:
+        xo, ito, _, _ = oracle[j]
+        if check(xcols[j], infos[j], xo, ito, f"128^3 column {j}", Ao, np.ascontiguousarray(B[:, j]), cols[j]):
+            loose.append(j)
+    assert len(loose) <= 1
+
+
+@pytest.mark.timeout(600)
+def test_uneven_256_systems_each_against_oracle(ctx):
+    sysl = hbs.gen_uneven_systems(256)
+    rp, c, v, rows = hbs.concat_systems([(a, b, d) for a, b, d, _ in sysl])
+    n = int(rows[-1])
+    xt = hbs.gen_uniform(n, -1.0, 1.0, 5)
+    b = hbs.host_spmv(rp, c, v, xt)
+    A = hbs.DeviceCSR(ctx, rp, c, v, sys_rows=rows)
+    S = hbs.Solver(A)
+    db, dx = ctx.to_device(b), ctx.empty(n)
+
+    def oracle(q):
+        srp, sc, sv, _ = sysl[q]
+        return orc.bicgstab(orc.CSR(srp, sc, sv), b[rows[q]:rows[q + 1]], None, 1e-8, 5000)
+
+    order = sorted(range(256), key=lambda q: -int(sysl[q][1].size))   # big systems first
+    with _pool() as ex:
+        futs = {q: ex.submit(oracle, q) for q in order}
+        dx.zero()
+        infos = S.bicgstab_systems(db, dx, tol=1e-8, maxit=5000, use_graph=1, check_every=8)
+        x = dx.download()
+        res = {q: f.result() for q, f in futs.items()}
+    loose = []
+    for q in range(256):
+        srp, sc, sv, conv = sysl[q]
+        r0, r1 = rows[q], rows[q + 1]
+        xo, ito, _, _ = res[q]
+        if check(x[r0:r1], infos[q], xo, ito, f"system {q} n={r1 - r0} conv={conv:.2f}", orc.CSR(srp, sc, sv),
+                 b[r0:r1], xt[r0:r1], maxit=5000):
+            loose.append(q)
+    print(f"256 systems ({n} rows): order-sensitive bar used by {len(loose)}: {loose}", flush=True)
+    assert len(loose) <= MAX_ORDER_SENSITIVE * 256

@@ -167,3 +167,190 @@ def test_device_shard_spmxv_host_halo():
             assert np.array_equal(down(lib, dXs)[:ns], x[hs["gind"][m:]])
         assert lib.device_SHARD_CSC_destroy(C.byref(dA)) == 0
         hbs.shard_free(hs)
+
+
+def _csc_struct(d):
+    """hbs.HostCSR view of a SHARD-convention CSC dict (arrays kept alive in d)."""
+    ip = C.POINTER(C.c_int)
+    for k in ("I", "J", "val"):
+        d[k] = np.ascontiguousarray(d[k])
+    return hbs.HostCSR(d["I"].ctypes.data_as(ip), d["J"].ctypes.data_as(ip), d["val"].ctypes.data_as(_dp),
+                       d["nnz"], d["m"], d["n"])
+
+
+def _L2():
+    lib = L()
+    lib.device_csc_create.argtypes = [hbs.HostCSR, C.POINTER(DeviceCSR)]
+    lib.device_csc_destroy.argtypes = [C.POINTER(DeviceCSR)]
+    lib.device_csc_spmv.argtypes = [C.c_void_p, DeviceCSR, DeviceVector, DeviceVector, C.c_double, C.c_double,
+                                    C.c_void_p]
+    lib.device_csr_spmv_async.argtypes = lib.device_csc_spmv.argtypes
+    lib.device_vector_zero.argtypes = [DeviceVector]
+    lib.device_vector_GPUtoGPU.argtypes = [DeviceVector, DeviceVector]
+    return lib
+
+
+@pytest.mark.parametrize("rank", [0, 7, 15])
+def test_device_csc_blocks_and_reference_built_shard(rank):
+    """device_csc_create / device_csc_spmv on the loc and shr blocks of a
+    SHARD_CSC, and MPI_device_SHARD_CSC_mpi_spmxv on a shard built the way
+    bicgstab_mpi_gpu.c:248-257 builds it (gind/send/recv borrowed, loc and
+    shr from device_csc_create).  Each block's SpMV sums a row over its
+    columns in ascending order (SHARD_loc_CSC_to_CSR, hhp_matrix.c:
+    1441-1474), so it equals the oracle's CSR SpMV of that block bit for bit;
+    the shard SpMV follows hhp_cuda.c:259-352: Y = 0, Y = alpha*loc*X +
+    beta*Y, Y = alpha*shr*x_shr + alpha*Y."""
+    lib = _L2()
+    _, _, rp, c, v = hbs.mm_read(golden("sample320", "A.mtx"))
+    pv = partition("sample320", "row_part_16")
+    x = np.random.default_rng(4 + rank).uniform(-1, 1, 320)
+    hs = hbs.shard_build(rp, c, v, pv, 16, rank)
+    raw = hs["raw"]
+    m, ns = hs["loc"]["m"], hs["shr"]["n"]
+    x_loc = x[hs["gind"][:m]]
+    x_shr = x[hs["gind"][m:]]
+    recv_buf = x_shr[hs["recv"]["J"]]
+    dloc = DeviceCSR()
+    assert lib.device_csc_create(raw.loc, C.byref(dloc)) == 0
+    assert (dloc.data.m, dloc.data.n, dloc.data.nnz) == (m, hs["loc"]["n"], hs["loc"]["nnz"])
+    loc_csr = orc.CSR(*orc.shard_csc_to_csr(hs["loc"]["I"], hs["loc"]["J"], hs["loc"]["val"], m, m))
+    dX, dY = dvec(lib, x_loc), dvec(lib, np.full(m, 3.0))
+    assert lib.device_csc_spmv(None, dloc, dX, dY, 1.0, 0.0, None) == 0
+    y_loc = orc.spmv(loc_csr, x_loc)
+    assert np.array_equal(down(lib, dY), y_loc)
+    dshr = DeviceCSR()
+    if ns:
+        assert lib.device_csc_create(raw.shr, C.byref(dshr)) == 0
+        shr_csr = orc.CSR(*orc.shard_csc_to_csr(hs["shr"]["I"], hs["shr"]["J"], hs["shr"]["val"], m, ns), ncols=ns)
+        dXs = dvec(lib, x_shr)
+        dYs = dvec(lib, np.zeros(m))
+        assert lib.device_csc_spmv(None, dshr, dXs, dYs, 1.0, 0.0, None) == 0
+        y_shr = orc.spmv(shr_csr, x_shr)
+        assert np.array_equal(down(lib, dYs), y_shr)
+    else:
+        y_shr = np.zeros(m)
+        dXs = dvec(lib, np.zeros(1))
+    # the shard as the reference driver builds it: no merged form (dev NULL)
+    dA = DeviceShard()
+    dA.loc, dA.shr, dA.gind, dA.send, dA.recv, dA.dev = dloc, dshr, raw.gind, raw.send, raw.recv, None
+    for k in range(ns):
+        raw.recv.val[k] = recv_buf[k]
+    for alpha, beta in ((1.0, 0.0), (2.0, 0.5)):
+        dZ = dvec(lib, np.full(m, 7.0))           # zeroed by the call, as in the reference
+        dXs2 = dvec(lib, np.zeros(max(ns, 1)))
+        assert lib.device_SHARD_CSC_spmxv(dA, Vector(), dX, dXs2, dZ, None, alpha, beta, None, None, None) == 0
+        want = alpha * y_shr + alpha * (alpha * y_loc)
+        assert np.array_equal(down(lib, dZ), want)
+        if ns:
+            assert np.array_equal(down(lib, dXs2)[:ns], x_shr)
+        nsend = int(hs["send"]["I"][-1]) if hs["send"]["num"] else 0
+        assert np.array_equal(np.array([raw.send.val[k] for k in range(nsend)]), orc.shard_pack(hs, x_loc))
+        for vv in (dZ, dXs2):
+            assert lib.device_vector_destroy(C.byref(vv)) == 0
+    # with alpha = 1 and beta = 0 the two-SpMV form is within rounding of the
+    # CPU halo SpMV (loc part and shr part summed separately, then added)
+    ys = orc.shard_spmv(hs, x_loc, recv_buf)
+    assert np.allclose(y_shr + y_loc, ys, rtol=1e-14, atol=1e-14)
+    assert lib.device_csc_destroy(C.byref(dloc)) == 0
+    if ns:
+        assert lib.device_csc_destroy(C.byref(dshr)) == 0
+    hbs.shard_free(hs)
+
+
+def test_merged_shard_spmxv_alpha_beta():
+    """The merged shard (device_SHARD_CSC_create): Y = alpha*(loc*X +
+    shr*x_shr) + beta*Y on the caller's Y, the row sum bit-identical to the
+    CPU halo SpMV (hhp_cpu.c:514-601)."""
+    lib = L()
+    _, _, rp, c, v = hbs.mm_read(golden("sample320", "A.mtx"))
+    pv = partition("sample320", "row_part_16")
+    x = np.random.default_rng(9).uniform(-1, 1, 320)
+    hs = hbs.shard_build(rp, c, v, pv, 16, 3)
+    raw = hs["raw"]
+    m, ns = hs["loc"]["m"], hs["shr"]["n"]
+    x_loc = x[hs["gind"][:m]]
+    recv_buf = x[hs["gind"][m:][hs["recv"]["J"]]]
+    for k in range(ns):
+        raw.recv.val[k] = recv_buf[k]
+    dA = DeviceShard()
+    assert lib.device_SHARD_CSC_create(raw, C.byref(dA)) == 0
+    y0 = np.random.default_rng(10).standard_normal(m)
+    dX, dXs, dY = dvec(lib, x_loc), dvec(lib, np.zeros(max(ns, 1))), dvec(lib, y0)
+    assert lib.device_SHARD_CSC_spmxv(dA, Vector(), dX, dXs, dY, None, 2.0, 0.5, None, None, None) == 0
+    ys = orc.shard_spmv(hs, x_loc, recv_buf)
+    assert np.array_equal(down(lib, dY), 2.0 * ys + 0.5 * y0)
+    assert lib.device_SHARD_CSC_destroy(C.byref(dA)) == 0
+    hbs.shard_free(hs)
+
+
+def test_device_vector_zero_copy_and_async_spmv():
+    lib = _L2()
+    rp, c, v, xt, b = convdiff64()
+    n = rp.size - 1
+    csr = hbs.HostCSR(rp.ctypes.data_as(C.POINTER(C.c_int)), c.ctypes.data_as(C.POINTER(C.c_int)),
+                      v.ctypes.data_as(_dp), c.size, n, n)
+    dA = DeviceCSR()
+    assert lib.device_csr_create(csr, C.byref(dA)) == 0
+    y0 = np.random.default_rng(3).standard_normal(n)
+    dx, dy, dz = dvec(lib, xt), dvec(lib, y0), dvec(lib, np.ones(n))
+    assert lib.device_vector_GPUtoGPU(dy, dz) == 0
+    assert np.array_equal(down(lib, dz), y0)
+    # asynchronous: queued on the handle's stream, the caller synchronises
+    assert lib.device_csr_spmv_async(None, dA, dx, dy, -1.5, 0.25, None) == 0
+    assert hbs.lib().hbs_ctx_sync(lib.hhp_default_handle()) == 0
+    ax = orc.spmv(orc.CSR(rp, c, v), xt)
+    assert np.array_equal(down(lib, dy), -1.5 * ax + 0.25 * y0)
+    assert lib.device_vector_zero(dz) == 0
+    assert np.all(down(lib, dz) == 0.0)
+    for vv in (dx, dy, dz):
+        assert lib.device_vector_destroy(C.byref(vv)) == 0
+    assert lib.device_csr_destroy(C.byref(dA)) == 0
+
+
+def test_dp_update_omega_and_add():
+    lib = L()
+    st = C.c_void_p(hbs.lib().hbs_ctx_stream(lib.hhp_default_handle()))
+    s = np.array([0.0, 0.0, 3.7, -1.1, 0.0, 2.5, 0.375])
+    d = dvec(lib, s)
+    q = lambda k: C.cast(C.c_void_p(C.cast(d.vals, C.c_void_p).value + 8 * k), _dp)
+    lib.hhp_dp_update_omega(q(0), q(1), q(2), q(3), st)   # omega = ts / tt ; neg_omega = -omega
+    lib.hhp_dp_add(q(4), q(5), q(6), st)                    # total = a + b
+    out = down(lib, d)
+    assert out[0] == 3.7 / -1.1 and out[1] == -(3.7 / -1.1)
+    assert out[4] == 2.5 + 0.375
+    assert lib.device_vector_destroy(C.byref(d)) == 0
+
+
+@pytest.mark.parametrize("name,x", [("sample", "X_mul.txt"), ("sample320", None)])
+def test_plain_c_caller_runs_reference_harness_sequence(tmp_path, name, x):
+    """The plain-C caller (tests/c/spmv_caller.c), compiled against the
+    headers, follows spmv_gpu.c:90-136 and checks A*x = [1..m] (sample:
+    x = X_mul.txt; sample320: x = X_solution.txt, since A*[1..320] = B)."""
+    import subprocess
+    from c_caller import build_caller
+    exe = build_caller(tmp_path)
+    m, n, rp, c, v = hbs.mm_read(golden(name, "A.mtx"))
+    if x is None:
+        # sample320's B is A*[1..320], so feed A x = [1..m] by solving for x:
+        # take x = A^-1 [1..m] from a dense solve, written with full digits
+        A = np.zeros((m, n))
+        for i in range(m):
+            for j in range(rp[i], rp[i + 1]):
+                A[i, c[j]] += v[j]
+        xv = np.linalg.solve(A, np.arange(1, m + 1, dtype=float))
+        xp = tmp_path / "x.txt"
+        xp.write_text("".join(f"{float(t)!r}\n" for t in xv))
+        xpath = str(xp)
+    else:
+        xpath = golden(name, x)
+    p = subprocess.run([exe, golden(name, "A.mtx"), xpath, "10"], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    print(p.stdout, flush=True)
+    assert p.returncode == 0
+    kv = dict(l.split(" : ", 1) for l in p.stdout.splitlines() if " : " in l)
+    assert int(kv["n_iters"].strip()) == 10
+    assert float(kv["max_abs_err_vs_1ton"]) <= (5.3e-15 if x else 1e-9)
+    assert float(kv["async_err"]) == 0.0
+    assert float(kv["blas_err"]) <= 1e-12 * m
+    assert float(kv["dot_rel_err"]) <= 1e-14
+    assert float(kv["dp_add"]) == 0.75

@@ -58,12 +58,18 @@ def test_shard_reference_partitions(ctx, name, fname, nparts):
 
 
 def test_shard_markov_part0_of_4(ctx):
+    """The 445k-row system split 4 ways by the library's own partitioner
+    (hbs_partition, the CalcPartVec model of hhp_matrix.c:225-300), and by a
+    contiguous split."""
     rp, c, v = markov(445000)
     n = rp.size - 1
-    pv = (np.arange(n) * 4 // n).astype(np.int32)
+    pv, cut = hbs.partition(rp, c, 4)
+    print(f"hbs_partition: 4 parts of {np.bincount(pv, minlength=4).tolist()} rows, cut {cut}", flush=True)
     x = hbs.gen_uniform(n, -1.0, 1.0, 7)
     for r in (0, 3):
         _rank_check(ctx, rp, c, v, pv, 4, r, x)
+    pv = (np.arange(n) * 4 // n).astype(np.int32)
+    _rank_check(ctx, rp, c, v, pv, 4, 0, x)
 
 
 def test_shard_random_owner_order(ctx):

@@ -56,6 +56,46 @@ def test_vector_io(tmp_path):
 
 
 @pytest.mark.parametrize("name,fname,nparts", PARTITIONS)
+def test_vector_part_and_gather_bit_exact(name, fname, nparts, tmp_path):
+    """Per-rank slice (MPI_vector_read_parted) and the gather back through
+    gind (MPI_vector_gather) against the oracle, on every rank of the
+    reference partitions; the slice from a vector file too."""
+    _, _, rp, c, v = hbs.mm_read(golden(name, "A.mtx"))
+    n = rp.size - 1
+    pv = partition(name, fname)
+    x = np.random.default_rng(5).standard_normal(n)
+    f = tmp_path / "x.txt"
+    f.write_text("".join(f"{float(t)!r}\n" for t in x))
+    big = np.full(n, np.nan)
+    locs, ginds = [], []
+    for r in range(nparts):
+        sl = hbs.vector_part(x, pv, nparts, r)
+        assert np.array_equal(sl, orc.vector_read_parted(x, pv, r))
+        assert np.array_equal(hbs.vector_read_parted(str(f), n, pv, nparts, r), sl)
+        hs = hbs.shard_build(rp, c, v, pv, nparts, r)
+        g = hs["gind"][:hs["loc"]["n"]]
+        assert g.size == sl.size
+        hbs.vector_gather(sl, g, big)
+        locs.append(sl)
+        ginds.append(g)
+        hbs.shard_free(hs)
+    assert np.array_equal(big, x)
+    assert np.array_equal(big, orc.vector_gather(locs, ginds, n))
+
+
+def test_vector_part_errors():
+    x = np.arange(4.0)
+    with pytest.raises(hbs.HBSError) as e:
+        hbs.vector_part(x, np.array([0, 1, 2, 0], np.int32), 2, 0)
+    assert e.value.code == -6
+    with pytest.raises(hbs.HBSError):
+        hbs.vector_part(x, np.array([0, 1, 1, 0], np.int32), 2, 2)
+    with pytest.raises(hbs.HBSError):
+        hbs.vector_gather(np.ones(2), np.array([0, 4], np.int32), np.zeros(4))
+    assert hbs.vector_part(x, np.array([0, 1, 1, 0], np.int32), 2, 1).tolist() == [1.0, 2.0]
+
+
+@pytest.mark.parametrize("name,fname,nparts", PARTITIONS)
 def test_shard_build_bit_exact_reference_partitions(name, fname, nparts):
     _, _, rp, c, v = hbs.mm_read(golden(name, "A.mtx"))
     pv = partition(name, fname)

@@ -222,6 +222,29 @@ def test_vector_read_parted():
     assert orc.vector_read_parted(x, pv, 1).tolist() == [0, 2, 3, 7]
 
 
+def test_vector_gather_brute_force_and_round_trip():
+    """MPI_vector_gather (hhp_cpu.c:199-250) restated: brute force on a small
+    case, then slice -> gather is the identity over every rank of the
+    reference partitions (gind's owned prefix is the slice's row order), and
+    a planted swap of two gind entries breaks the round trip."""
+    big = orc.vector_gather([np.array([5.0, 6.0]), np.array([7.0])], [np.array([2, 0]), np.array([1])], 4,
+                            big=np.full(4, -1.0))
+    assert big.tolist() == [6.0, 7.0, 5.0, -1.0]
+    for name, fname, nparts in PARTITIONS:
+        _, _, rp, c, v = orc.mm_read(golden(name, "A.mtx"))
+        n = rp.size - 1
+        pv = partition(name, fname)
+        x = np.random.default_rng(3).standard_normal(n)
+        shards = orc.build_shards(orc.CSR(rp, c, v), pv, nparts)
+        locs = [orc.vector_read_parted(x, pv, r) for r in range(nparts)]
+        ginds = [sh["gind"][:sh["loc"]["n"]] for sh in shards]
+        assert np.array_equal(orc.vector_gather(locs, ginds, n), x)
+    bad = [g.copy() for g in ginds]
+    k = next(r for r in range(nparts) if bad[r].size >= 2)
+    bad[k][[0, 1]] = bad[k][[1, 0]]
+    assert not np.array_equal(orc.vector_gather(locs, bad, n), x)
+
+
 def test_blocked_dots_variant_reduces_to_reference():
     """The re-associated variant (order-sensitivity probe only) is the
     reference loop when one block holds the whole vector or blk <= 0."""
