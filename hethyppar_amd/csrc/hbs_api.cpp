@@ -25,6 +25,7 @@ int g_stream_nt = 0;
 int g_spmv_npt = 0;       // entries per thread per pass of 512-thread SpMV blocks (0: 4)
 int g_sort_pct = 70;      // kind-2 blocks: sorted gathers touch <= 70% as many lines as entries
 int g_xseg = 2;           // x-segmented SpMV: 0 never, 1 every one-system matrix, 2 scattered columns
+int g_xseg_split = 1;     // k_xseg workgroups per x segment
 }  // namespace
 
 #define fail ::hbs::set_error
@@ -44,9 +45,11 @@ int hbs::DevVecPlan::build(int nn)
 {
     release();
     std::vector<VecBlock> vb;
-    plan_vec(0, nn, vec_elems_per_block(nn), 0, vb);
+    const int e = vec_elems_per_block(nn);
+    plan_vec(0, nn, e, 0, vb);
     nvb = (int)vb.size();
     n = nn;
+    epb = e;
     CHK(dalloc(&blocks, vb.size()));
     CHK(upload(blocks, vb));
     SysRange sr{0, 0, 0, nvb};
@@ -188,6 +191,27 @@ extern "C" int hbs_d2d(hbs_ctx ctx, void *dst, const void *src, size_t bytes)
     return HBS_OK;
 }
 
+extern "C" int hbs_pinned_alloc(hbs_ctx ctx, size_t bytes, void **host)
+{
+    if (!ctx || !host) return fail(HBS_ERR_ARG, "null argument");
+    *host = nullptr;
+    DevGuard g(ctx->device);
+    if (hipHostMalloc(host, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        *host = nullptr;
+        return fail(HBS_ERR_ALLOC, "pinned host allocation of %zu bytes failed", bytes);
+    }
+    return HBS_OK;
+}
+
+extern "C" int hbs_pinned_free(hbs_ctx ctx, void *host)
+{
+    if (!ctx) return fail(HBS_ERR_ARG, "null ctx");
+    if (!host) return HBS_OK;
+    DevGuard g(ctx->device);
+    HIPCHK(hipHostFree(host));
+    return HBS_OK;
+}
+
 extern "C" int hbs_dev_zero(hbs_ctx ctx, void *dev, size_t bytes)
 {
     if (!ctx || (bytes && !dev)) return fail(HBS_ERR_ARG, "null argument");
@@ -227,7 +251,7 @@ extern "C" int hbs_dot_f64(hbs_ctx ctx, int n, const double *x, const double *y,
         return HBS_OK;
     }
     DevGuard g(ctx->device);
-    if (ctx->vp.n != n) CHK(ctx->vp.build(n));
+    if (ctx->vp.n != n || ctx->vp.epb != vec_elems_per_block(n)) CHK(ctx->vp.build(n));
     VecArgs<double> a{};
     a.blocks = ctx->vp.blocks;
     a.sys = ctx->vp.sys;
@@ -302,6 +326,13 @@ extern "C" int hbs_set_spmv_pass(int entries_per_thread)
     if (entries_per_thread != 0 && entries_per_thread != 4 && entries_per_thread != 8)
         return fail(HBS_ERR_ARG, "entries_per_thread must be 0, 4 or 8");
     g_spmv_npt = entries_per_thread;
+    return HBS_OK;
+}
+
+extern "C" int hbs_set_xseg_split(int split)
+{
+    if (split < 1 || split > 16) return fail(HBS_ERR_ARG, "split must be in [1, 16]");
+    g_xseg_split = split;
     return HBS_OK;
 }
 
@@ -490,6 +521,7 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     }
     if (xseg) {
         a->xs_nseg = xh.nseg;
+        a->xs_split = g_xseg_split;
         a->xs_max_chunks = xh.max_chunks;
         a->xs_max_granules = xh.max_granules;
         std::vector<double> xv((size_t)nnz);

@@ -157,6 +157,7 @@ struct SpmvArgs {
 template <typename T>
 struct XsegArgs {
     int nseg, seg_cols, n;
+    int split;                   // workgroups per segment, each a contiguous share of its entries
     int max_chunks, max_granules;
     const int *sbase;            // [nseg + 1] first entry of each segment
     const int *cbase;            // [nseg + 1] first chunk of each segment

@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <map>
 #include <mutex>
 
@@ -447,12 +448,12 @@ struct XLoad {
 };
 
 template <typename T, int NPT>
-__device__ __forceinline__ void xseg_issue(const XsegArgs<T> &a, int g0, int cnt, int base, XLoad<T, NPT> &L)
+__device__ __forceinline__ void xseg_issue(const XsegArgs<T> &a, int g0, int lim, int base, XLoad<T, NPT> &L)
 {
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
         const int f = base + i * kWideThreads + (int)threadIdx.x;
-        const bool in = f < cnt;
+        const bool in = f < lim;
         L.c[i] = in ? (int)a.xcol[g0 + f] : 0;
         L.d[i] = in ? (int)a.delta[g0 + f] : 0;
         L.v[i] = in ? a.xval[g0 + f] : T(0);
@@ -466,12 +467,17 @@ __global__ __launch_bounds__(kWideThreads) void k_xseg(XsegArgs<T> a)
     T *xs = reinterpret_cast<T *>(smem);
     int *offl = reinterpret_cast<int *>(smem + (size_t)a.seg_cols * sizeof(T));
     int *gchl = offl + a.max_chunks;
-    const int s = xcd_remap(blockIdx.x, a.nseg);
+    // workgroup w takes share w % split of segment w / split (the shares of
+    // one segment and neighbouring segments on one XCD); entries f in
+    // [f0, f1) of the segment, numbered from its first entry
+    const int w = xcd_remap(blockIdx.x, a.nseg * a.split);
+    const int s = w / a.split, part = w - s * a.split;
     if (a.st && a.st->done) return;
     const int g0 = a.sbase[s], cnt = a.sbase[s + 1] - g0;
-    if (cnt <= 0) return;
+    const int f0 = (int)((long long)cnt * part / a.split), f1 = (int)((long long)cnt * (part + 1) / a.split);
+    if (f1 <= f0) return;
     XLoad<T, NPT> LA, LB;
-    xseg_issue<T, NPT>(a, g0, cnt, 0, LA);
+    xseg_issue<T, NPT>(a, g0, f1, f0, LA);
     const int c0 = s * a.seg_cols, nc = min(a.seg_cols, a.n - c0);
     for (int i = (int)threadIdx.x; i < nc; i += kWideThreads) xs[i] = a.x[c0 + i];
     const int cb = a.cbase[s], ncs = a.cbase[s + 1] - cb;
@@ -482,16 +488,16 @@ __global__ __launch_bounds__(kWideThreads) void k_xseg(XsegArgs<T> a)
     T *__restrict__ prod = a.prod + g0;
     constexpr int PASS = NPT * kWideThreads;
     auto step = [&](const XLoad<T, NPT> &in, XLoad<T, NPT> &out, int base) -> bool {
-        const bool more = base + PASS < cnt;
-        if (more) xseg_issue<T, NPT>(a, g0, cnt, base + PASS, out);
+        const bool more = base + PASS < f1;
+        if (more) xseg_issue<T, NPT>(a, g0, f1, base + PASS, out);
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
             const int f = base + i * kWideThreads + (int)threadIdx.x;
-            if (f < cnt) prod[f + offl[gchl[f >> 6] + in.d[i]]] = in.v[i] * xs[in.c[i]];
+            if (f < f1) prod[f + offl[gchl[f >> 6] + in.d[i]]] = in.v[i] * xs[in.c[i]];
         }
         return more;
     };
-    for (int base = 0;; base += 2 * PASS) {
+    for (int base = f0;; base += 2 * PASS) {
         if (!step(LA, LB, base)) break;
         if (!step(LB, LA, base + PASS)) break;
     }
@@ -997,9 +1003,9 @@ template <typename T>
 hipError_t xseg_dispatch(const XsegArgs<T> &a, hipStream_t s)
 {
     if (a.nseg <= 0) return hipSuccess;
-    if (a.seg_cols != kSegCols) return hipErrorInvalidValue;
+    if (a.seg_cols != kSegCols || a.split < 1 || (long long)a.nseg * a.split > INT_MAX) return hipErrorInvalidValue;
     const size_t lds = xseg_lds_bytes((int)sizeof(T), a.max_chunks, a.max_granules);
-    k_xseg<T, xseg_npt<T>()><<<a.nseg, kWideThreads, lds, s>>>(a);
+    k_xseg<T, xseg_npt<T>()><<<a.nseg * a.split, kWideThreads, lds, s>>>(a);
     return hipGetLastError();
 }
 hipError_t launch_xseg(const XsegArgs<double> &a, hipStream_t s) { return xseg_dispatch<double>(a, s); }

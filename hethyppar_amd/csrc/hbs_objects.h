@@ -63,6 +63,7 @@ struct DevGuard {
 // Vector workgroups of n elements (one system) on the device.
 struct DevVecPlan {
     int n = -1;
+    int epb = -1;               // elements per vector workgroup
     int nvb = 0;
     VecBlock *blocks = nullptr;
     SysRange *sys = nullptr;    // {0, 0, 0, nvb}
@@ -98,6 +99,7 @@ struct hbs_csr_s {
     // x-segmented SpMV (columns scattered over all of x): k_xseg forms the
     // products per x segment, k_spmv<PRE> sums them (layout in XsegArgs)
     int xs_nseg = 0;                  // 0: CSR-stream SpMV
+    int xs_split = 1;                 // k_xseg workgroups per segment
     int xs_max_chunks = 0, xs_max_granules = 0;
     int *xs_sbase = nullptr, *xs_cbase = nullptr, *xs_gbase = nullptr, *xs_gch = nullptr, *xs_off = nullptr;
     int *xs_src = nullptr;            // CSR entry of each segment-major entry (refreshes xs_val)
@@ -203,6 +205,7 @@ hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a, int epi, const Solver
     if (a->xs_nseg > 0) {
         XsegArgs<T> x{};
         x.nseg = a->xs_nseg;
+        x.split = a->xs_split;
         x.seg_cols = kSegCols;
         x.n = a->n;
         x.max_chunks = a->xs_max_chunks;

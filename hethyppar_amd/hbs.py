@@ -85,6 +85,8 @@ _PROTOS = {
     "hbs_d2h": (_i, [_vp, _vp, _vp, C.c_size_t]),
     "hbs_d2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
     "hbs_dev_zero": (_i, [_vp, _vp, C.c_size_t]),
+    "hbs_pinned_alloc": (_i, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "hbs_pinned_free": (_i, [_vp, _vp]),
     "hbs_csr_create": (_i, [_vp, _i, _i, _i, _ip, _ip, _dp, _i, C.POINTER(_vp)]),
     "hbs_csr_create_systems": (_i, [_vp, _i, _ip, _i, _ip, _ip, _dp, _i, C.POINTER(_vp)]),
     "hbs_csr_systems": (_i, [_vp, _ip]),
@@ -143,6 +145,7 @@ _PROTOS = {
     "hbs_set_spmv_sort": (_i, [_i]),
     "hbs_csr_plan_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "hbs_set_spmv_xseg": (_i, [_i]),
+    "hbs_set_xseg_split": (_i, [_i]),
     "hbs_csr_xseg": (_i, [_vp, _ip]),
     "hbs_shard_pack_host": (_i, [_vp, _vp, _dp]),
     "hbs_vector_part": (_i, [_i, _dp, _ip, _i, _i, _dp, _ip]),
@@ -482,6 +485,32 @@ class Context:
         return ms.value
 
 
+class PinnedArray:
+    """Page-locked host array (hbs_pinned_alloc) seen as a numpy array."""
+
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        p = _vp()
+        check(lib().hbs_pinned_alloc(ctx.h, max(int(n), 1) * self.dtype.itemsize, C.byref(p)))
+        self.ptr = p
+        buf = (C.c_char * (max(int(n), 1) * self.dtype.itemsize)).from_address(p.value)
+        self.a = np.frombuffer(buf, dtype=self.dtype, count=int(n))
+        ctx._adopt(self)
+
+    def close(self):
+        if self.ptr and self.ctx.h:
+            lib().hbs_pinned_free(self.ctx.h, self.ptr)
+        self.ptr = None
+        self.a = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceArray:
     def __init__(self, ctx, n, dtype=np.float64):
         self.ctx = ctx
@@ -751,6 +780,10 @@ def set_spmv_plan(block_nnz=0, serial_max_avg=64, max_groups=0, stream_nt=0, npt
     check(lib().hbs_set_spmv_groups(max_groups))
     check(lib().hbs_set_spmv_stream_nt(stream_nt))
     check(lib().hbs_set_spmv_pass(npt))
+
+
+def set_xseg_split(split=1):
+    check(lib().hbs_set_xseg_split(split))
 
 
 def set_spmv_xseg(mode=2):

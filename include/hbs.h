@@ -59,6 +59,10 @@ int hbs_h2d(hbs_ctx ctx, void *dev, const void *host, size_t bytes);   /* synchr
 int hbs_d2h(hbs_ctx ctx, void *host, const void *dev, size_t bytes);   /* synchronous */
 int hbs_d2d(hbs_ctx ctx, void *dst, const void *src, size_t bytes);    /* async */
 int hbs_dev_zero(hbs_ctx ctx, void *dev, size_t bytes);                /* async, fill kernel */
+/* page-locked host memory (hipHostMalloc), for halo buffers whose copies to
+ * the device should run asynchronously at full PCIe rate */
+int hbs_pinned_alloc(hbs_ctx ctx, size_t bytes, void **host);
+int hbs_pinned_free(hbs_ctx ctx, void *host);
 
 /* ---- CSR matrix on the device ---------------------------------------- */
 typedef struct hbs_csr_s *hbs_csr;
@@ -383,6 +387,11 @@ int hbs_csr_plan_info(hbs_csr a, int *ngroups, int *nchunks, int *block_nnz, int
  * whose columns touch a new 128-byte line of x for more than half the
  * entries. */
 int hbs_set_spmv_xseg(int mode);
+/* Workgroups per x segment in the product pass of x-segmented matrices
+ * created afterwards (1..16, default 1): each stages the segment's x and
+ * forms the products of a contiguous share of its entries, so more
+ * workgroups share the GPU at the cost of staging x once per share. */
+int hbs_set_xseg_split(int split);
 /* Segments of an x-segmented device CSR (0: CSR-stream SpMV). */
 int hbs_csr_xseg(hbs_csr a, int *nseg);
 

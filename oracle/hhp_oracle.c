@@ -206,11 +206,24 @@ void orc_vector_sub_seq(const double *a, const double *b, double *out, int n)
  * alpha*p (:219-236), or when ||r|| < threshold after the omega step
  * (:289-293).  Norms are sqrt of the sequential dot. */
 /* Dot product as sums of consecutive blocks of blk elements (each block
- * sequential), the block sums added in order; blk <= 0 is the reference's
- * sequential dot. */
+ * sequential), the block sums added in order; blk < 0: -blk interleaved
+ * running sums; blk == 0 is the reference's sequential dot. */
 static double dot_blocked(const double *x, const double *y, int n, int blk)
 {
-    if (blk <= 0) return orc_vector_dot_seq(x, y, n);
+    if (blk < 0) {
+        /* interleaved: -blk running sums, entry i into sum i mod -blk (the
+         * way lanes of a parallel reduction stride through a vector), the
+         * sums then added in order */
+        const int k = -blk;
+        double *acc = calloc((size_t)k, sizeof(double));
+        if (!acc) return orc_vector_dot_seq(x, y, n);
+        for (int i = 0; i < n; i++) acc[i % k] += x[i] * y[i];
+        double total = 0.0;
+        for (int j = 0; j < k; j++) total += acc[j];
+        free(acc);
+        return total;
+    }
+    if (blk == 0) return orc_vector_dot_seq(x, y, n);
     double total = 0.0;
     for (int i0 = 0; i0 < n; i0 += blk) {
         const int len = n - i0 < blk ? n - i0 : blk;

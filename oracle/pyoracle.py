@@ -151,17 +151,24 @@ def bicgstab_blocked(A, b, blk, x0=None, tol=1e-8, maxit=1000, fixed=False):
     return x, it.value, rr.value, st.value
 
 
-def iteration_spread(A, b, tol=1e-8, maxit=1000, blocks=(512, 64)):
-    """How far re-associating the dot sums moves the oracle's solve:
-    (sequential count, min count, max count, [solutions: sequential first,
-    then one per block size])."""
-    xs, its, _, _ = bicgstab(A, b, tol=tol, maxit=maxit)
-    counts, sols = [its], [xs]
-    for k in blocks:
-        xk, ik, _, _ = bicgstab_blocked(A, b, k, tol=tol, maxit=maxit)
-        counts.append(ik)
-        sols.append(xk)
-    return its, min(counts), max(counts), sols
+def iteration_spread(A, b, tol=1e-8, maxit=1000, blocks=(512, 64, -256, -4096), perturb=4, threads=8,
+                     seed0=0, sequential=True):
+    """How far rounding-level changes move the oracle's solve: the reference
+    loop with its dot sums re-associated (blocks > 0: consecutive blocks;
+    < 0: interleaved running sums), and the reference loop on b perturbed by
+    1e-15 relative (seeds seed0..seed0+perturb-1).  Returns (counts,
+    solutions), the sequential run first when `sequential`.  Runs in threads
+    (the oracle's calls release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    jobs = [lambda: bicgstab(A, b, tol=tol, maxit=maxit)] if sequential else []
+    jobs += [(lambda k=k: bicgstab_blocked(A, b, k, tol=tol, maxit=maxit)) for k in blocks]
+    for seed in range(seed0, seed0 + perturb):
+        bb = b * (1.0 + 1e-15 * np.random.default_rng(seed).standard_normal(b.size))
+        jobs.append(lambda bb=bb: bicgstab(A, bb, tol=tol, maxit=maxit))
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        res = list(ex.map(lambda f: f(), jobs))
+    return [r[1] for r in res], [r[0] for r in res]
 
 
 def csr_to_csc(A):

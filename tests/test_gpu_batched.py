@@ -46,7 +46,7 @@ def test_uneven_systems_each_match_oracle(ctx):
         Aq = orc.CSR(srp, sc, sv)
         xo, ito, _, _ = orc.bicgstab(Aq, b[r0:r1], tol=1e-8, maxit=2000)
         if _check(x[r0:r1], infos[q], xo, ito, f"system {q} n={r1 - r0} conv={conv:.2f}", Aq, b[r0:r1],
-                  xt[r0:r1]):
+                  xt[r0:r1]) != "plain":
             loose.append(q)
         its.append(infos[q].iters)
     print(f"order-sensitive bar used by systems {loose} of {len(sysl)}", flush=True)
@@ -133,7 +133,7 @@ def test_block_columns_match_oracle(ctx, k):
         bj = np.ascontiguousarray(B[:, j])
         xo, ito, _, _ = orc.bicgstab(orc.CSR(rp, c, v), bj, tol=1e-8, maxit=1000)
         if _check(X[:, j], infos[j], xo, ito, f"column {j} of {k}", orc.CSR(rp, c, v), bj,
-                  np.ascontiguousarray(Xt[:, j])):
+                  np.ascontiguousarray(Xt[:, j])) != "plain":
             loose.append(j)
     print(f"order-sensitive bar used by columns {loose} of {k}", flush=True)
     assert len(loose) <= MAX_ORDER_SENSITIVE * k

@@ -18,12 +18,14 @@ from workloads import markov, random_csr, sample320, seq_bound
 pytestmark = pytest.mark.gpu
 
 
-def _forced(ctx, rp, c, v, ncols=None, flags=0):
+def _forced(ctx, rp, c, v, ncols=None, flags=0, split=1):
     hbs.set_spmv_xseg(1)
+    hbs.set_xseg_split(split)
     try:
         return hbs.DeviceCSR(ctx, rp, c, v, ncols=ncols, flags=flags)
     finally:
         hbs.set_spmv_xseg()
+        hbs.set_xseg_split()
 
 
 def _check_f64(rp, c, v, x, y, serial_max=256):
@@ -63,13 +65,16 @@ def _cases():
     return out
 
 
+@pytest.mark.parametrize("split", [1, 3], ids=lambda s: f"split{s}")
 @pytest.mark.parametrize("case", _cases(), ids=lambda c: c[0])
-def test_xseg_spmv_bit_exact(ctx, case):
+def test_xseg_spmv_bit_exact(ctx, case, split):
+    """split: workgroups per segment in k_xseg (shares of a segment's
+    entries that are not whole passes, a share without entries)."""
     name, rp, c, v, ncols = case
     n = ncols or rp.size - 1
     m = rp.size - 1
     x = np.random.default_rng(3).uniform(-1, 1, n)
-    A = _forced(ctx, rp, c, v, ncols=ncols, flags=hbs.HBS_CSR_F32)
+    A = _forced(ctx, rp, c, v, ncols=ncols, flags=hbs.HBS_CSR_F32, split=split)
     info = A.info()
     print(name, info, flush=True)
     assert info["xseg"] == (n + 8191) // 8192

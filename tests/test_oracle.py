@@ -247,11 +247,12 @@ def test_vector_gather_brute_force_and_round_trip():
 
 def test_blocked_dots_variant_reduces_to_reference():
     """The re-associated variant (order-sensitivity probe only) is the
-    reference loop when one block holds the whole vector or blk <= 0."""
+    reference loop when one block (or one interleaved running sum) holds the
+    whole vector, or blk == 0."""
     rp, c, v, xt, b = convdiff64()
     A = orc.CSR(rp, c, v)
     x0, i0, r0, s0 = orc.bicgstab(A, b, tol=1e-8, maxit=1000)
-    for blk in (0, rp.size - 1, 10 ** 6):
+    for blk in (0, rp.size - 1, 10 ** 6, -1):
         x1, i1, r1, s1 = orc.bicgstab_blocked(A, b, blk, tol=1e-8, maxit=1000)
         assert (i1, s1) == (i0, s0) and np.array_equal(x1, x0) and r1 == r0
     x2, i2, _, _ = orc.bicgstab_blocked(A, b, 64, tol=1e-8, maxit=1000)
