@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("HBS_ARCH", "gfx950")
 
-KERNELS = ["hbs_kernels.hip", "hbs_aux.hip", "hbs_block.hip"]
+KERNELS = ["hbs_kernels.hip", "hbs_aux.hip", "hbs_block.hip", "hbs_noalloc.hip"]
 HOST = ["hbs_api.cpp", "hbs_solver.cpp", "hbs_block.cpp", "hbs_shard.cpp", "hbs_plan.cpp", "hbs_layout.cpp", "hbs_gen.cpp",
         "hhp_hip.cpp", "hbs_part.cpp"]
 

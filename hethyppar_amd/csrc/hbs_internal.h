@@ -287,6 +287,8 @@ double column_scatter(const int *col, int nnz, int window);
 void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan);
 // Vector blocks of elements [e0, e0 + n) of one system.
 void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock> &blocks);
+// The same blocks into out[cap] (out may be null to count); -1 if more than cap.
+int plan_vec_into(int e0, int n, int elems_per_block, int sys, VecBlock *out, int cap);
 int vec_elems_per_block(int n);
 extern int g_vec_target;      // vector workgroups a plan aims at (hbs_set_vec_blocks)
 int vec_packs_per_thread(int elems_per_block, int elem_bytes);

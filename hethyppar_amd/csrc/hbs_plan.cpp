@@ -248,6 +248,20 @@ void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock>
     }
 }
 
+int plan_vec_into(int e0, int n, int elems_per_block, int sys, VecBlock *out, int cap)
+{
+    const int end = e0 + n;
+    int k = 0;
+    for (int e = e0; e < end; ++k) {
+        const int base = e & ~3;
+        const int e1 = std::min(base + elems_per_block, end);
+        if (k >= cap) return -1;
+        if (out) out[k] = VecBlock{sys, e, e1, base};
+        e = e1;
+    }
+    return k;
+}
+
 int g_vec_target = 512;
 
 // Elements per vector workgroup: 512 * u with u in {1, 2, 4, 8}, the

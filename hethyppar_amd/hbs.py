@@ -100,6 +100,7 @@ _PROTOS = {
     "hbs_csr_spmm_f64": (_i, [_vp, _i, _vp, _vp, _i]),
     "hbs_csr_spmm_f32": (_i, [_vp, _i, _vp, _vp, _i]),
     "hbs_dot_f64": (_i, [_vp, _i, _vp, _vp, _dp]),
+    "hbs_dot_f64_noalloc": (_i, [_vp, _i, _vp, _vp, _dp]),
     "hbs_axpy_f64": (_i, [_vp, _i, _d, _vp, _vp]),
     "hbs_scal_f64": (_i, [_vp, _i, _d, _vp]),
     "hbs_solve_opts_default": (None, [C.POINTER(SolveOpts)]),

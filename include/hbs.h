@@ -118,6 +118,16 @@ int hbs_csr_spmm_f32(hbs_csr a, int k, const float *X_dev, float *Y_dev, int til
 int hbs_dot_f64(hbs_ctx ctx, int n, const double *x_dev, const double *y_dev, double *out_host);
 int hbs_axpy_f64(hbs_ctx ctx, int n, double a, const double *x_dev, double *y_dev); /* y += a*x */
 int hbs_scal_f64(hbs_ctx ctx, int n, double a, double *x_dev);                       /* x *= a   */
+/* hbs_dot_f64 without any device allocation, for callers that cannot
+ * allocate (tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same kernels, plan and summation order as
+ * hbs_dot_f64, so the same result bit for bit; its plan, partials and
+ * result live in one file-scope device array of the library (513 KiB per
+ * GPU, held until the process ends).  n is at most 2^26 (larger is
+ * HBS_ERR_ARG).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (from two host threads, or in flight on two streams)
+ * race. */
+int hbs_dot_f64_noalloc(hbs_ctx ctx, int n, const double *x_dev, const double *y_dev, double *out_host);
 
 /* ---- BiCGStab -------------------------------------------------------- */
 typedef struct {
