@@ -130,6 +130,8 @@ def main():
                     help="only the SpMV launches and a copy of known size (for counter runs)")
     ap.add_argument("--circuit", action="store_true",
                     help="only the circuit solves and SpMVs (for kernel traces)")
+    ap.add_argument("--markov", action="store_true",
+                    help="only the 445k-row Markov-like solves, SpMV and shard (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -143,6 +145,11 @@ def main():
         return
     if args.circuit:
         args.no_extras = args.no_cpu = True
+    if args.markov:
+        ctx = hbs.Context(0)
+        print(json.dumps({"markov_445k_fp64_cache_resident": extras_markov(hbs, ctx, args)}), flush=True)
+        ctx.close()
+        return
 
     log(f"bench: generating circuit-like n={N3} (rows equilibrated)")
     rp, c, v, xt, b = circuit(hbs)
