@@ -365,6 +365,7 @@ extern "C" int hbs_csr_destroy(hbs_csr a)
     dfree(a->xs_src);
     dfree(a->xs_col);
     dfree(a->xs_pos);
+    dfree(a->rperm);
     dfree(a->xs_delta);
     dfree(a->xs_val);
     dfree(a->xs_prod);
@@ -537,6 +538,11 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
             (rc = upload(a->xs_col, xh.xcol)) || (rc = upload(a->xs_pos, xh.ppos)) ||
             (rc = upload(a->xs_delta, xh.delta)) || (rc = upload(a->xs_val, xv)))
             return bail(rc);
+        // serial row sums of power-law blocks in descending row length
+        std::vector<unsigned short> rp16;
+        if (plan_row_order(rowptr, hp, m, rp16) > 0) {
+            if ((rc = dalloc(&a->rperm, rp16.size())) || (rc = upload(a->rperm, rp16))) return bail(rc);
+        }
         // the passes read a block's products clamped to its last entry, or
         // to entry nnz (padding) for a block without entries
         if (hipMemset(a->xs_pos + nnz, 0, sizeof(unsigned short) * kRowPad) != hipSuccess ||

@@ -144,6 +144,7 @@ struct SpmvArgs {
     SolverState *st_mark;       // start of a solve: set pending in this slot
     const T *prod;              // x-segmented matrices: the products of k_xseg, row-block major
     const unsigned short *ppos; //   and each one's position in its row block (CSR order)
+    const unsigned short *rperm;    // x-segmented matrices: serial row-sum order (plan_row_order)
 };
 
 // x-segmented SpMV, product pass (k_xseg): workgroup s stages the x segment
@@ -255,6 +256,16 @@ struct HostPlan {
     int serial_rows = 0;
     int max_row = 0;
 };
+// Order of the serial row sums of each block of 512-thread workgroups:
+// rperm[r0 + k] = the block's k-th longest row (local index, ties in row
+// order), so the rows one wave sums together have similar lengths and no
+// lane waits for the longest row of a mixed set.  A block takes that order
+// (aux.first = 1) when its estimated cost -- per wave, the sum over its row
+// slots of the batches of 8 products the longest row of the slot needs;
+// the block's cost is its slowest wave's -- falls below 80% of the row
+// order's.  Returns the number of such blocks; rperm covers every row
+// (identity elsewhere).
+int plan_row_order(const int *rowptr, HostPlan &plan, int m, std::vector<unsigned short> &rperm);
 // Row blocks of rows [row0, row0 + m) of system sys, appended to the plan.
 void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max_avg, int sys,
                HostPlan &plan);

@@ -86,6 +86,7 @@ struct PassLoad {
     T v[NPT];
     unsigned short pos[NPT];
     int rp[4];   // PRE, first pass of a block: its row pointers (kRowsPerThread)
+    int rq[4];   // PRE, blocks summed in rperm order: the rows of this thread's slots
 };
 
 // PRE: the block's row pointers, with its first pass's loads (staged in LDS
@@ -98,6 +99,11 @@ __device__ __forceinline__ void issue_rowptr(const SpmvArgs<T> &a, const RowBloc
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         L.rp[q] = a.rowptr[rb.r0 + (base == 0 ? min((int)threadIdx.x + q * TH, rb.r1 - rb.r0) : 0)];
+    if (a.rperm) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            L.rq[q] = a.rperm[rb.r0 + (base == 0 ? min((int)threadIdx.x + q * TH, max(rb.r1 - rb.r0 - 1, 0)) : 0)];
+    }
 }
 
 // Issue the stream loads of one pass.  Every lane loads: entries at or past
@@ -189,11 +195,32 @@ __device__ __forceinline__ T sum_lds(const T *lds, int j0, int e, int step)
 // prefetched).  Other blocks use `lanes` lanes per row and an xor butterfly.
 template <typename T, int EPI, int TH, bool PRE>
 __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, T w0,
-                                         const T *lds, const unsigned short *lrp, Acc<T> &d0, Acc<T> &d1)
+                                         const T *lds, const unsigned short *lrp, const int (&rq)[4], Acc<T> &d0,
+                                         Acc<T> &d1)
 {
     constexpr int B = PRE ? 8 : 2;
     const int nr = rb.r1 - rb.r0;
     const int lanes = ax.lanes;
+    if (PRE && lanes <= 1 && ax.first) {
+        // rows in the block's rperm order (longest first): slot t + q * TH
+        // sums row rq[q], in CSR order as ever; its epilogue operand and
+        // output sit at that row
+        T wr[kRowsPerThread], acc[kRowsPerThread];
+#pragma unroll
+        for (int q = 0; q < kRowsPerThread; ++q)
+            wr[q] = reads_w<EPI>() ? a.w[rb.r0 + min(rq[q], nr - 1)] : T(0);
+#pragma unroll
+        for (int q = 0; q < kRowsPerThread; ++q) {
+            const int t = (int)threadIdx.x + q * TH;
+            acc[q] = t < nr ? sum_lds<T, B>(lds, lrp[rq[q]], lrp[rq[q] + 1], 1) : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kRowsPerThread; ++q) {
+            const int t = (int)threadIdx.x + q * TH;
+            if (t < nr) row_out<T, EPI>(a, rb.r0 + rq[q], acc[q], wr[q], d0, d1);
+        }
+        return;
+    }
     if (lanes <= 1) {
         T wr[kRowsPerThread];
 #pragma unroll
@@ -313,12 +340,19 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // first pass of a block: its row pointers, staged in LDS below (PRE:
         // loaded with the pass, one step earlier)
         int rpv[kRowsPerThread];
+        int rqv[kRowsPerThread] = {0, 0, 0, 0};
         const bool stage = gather && cur.base == 0;
         if (stage) {
 #pragma unroll
             for (int q = 0; q < kRowsPerThread; ++q)
                 rpv[q] = PRE ? in.rp[q]
                              : a.rowptr[cur.rb.r0 + min((int)threadIdx.x + q * TH, cur.rb.r1 - cur.rb.r0)];
+            if constexpr (PRE) {
+                if (a.rperm) {
+#pragma unroll
+                    for (int q = 0; q < kRowsPerThread; ++q) rqv[q] = in.rq[q];
+                }
+            }
         }
         Cursor nxt = cur;
         if (last) {
@@ -382,7 +416,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
             } else {
                 __syncthreads();
-                sum_rows<T, EPI, TH, PRE>(a, cur.rb, cur.ax, w0, lds, lrp, d0, d1);
+                sum_rows<T, EPI, TH, PRE>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
                 __syncthreads();   // LDS is reused by the next row block
             }
             if (more) w0 = issue_w0<T, EPI>(a, nxt.rb, nxt.ax);

@@ -104,6 +104,7 @@ struct hbs_csr_s {
     int *xs_sbase = nullptr, *xs_cbase = nullptr, *xs_gbase = nullptr, *xs_gch = nullptr, *xs_off = nullptr;
     int *xs_src = nullptr;            // CSR entry of each segment-major entry (refreshes xs_val)
     unsigned short *xs_col = nullptr, *xs_pos = nullptr;
+    unsigned short *rperm = nullptr;   // serial row-sum order of x-segmented matrices (plan_row_order)
     unsigned char *xs_delta = nullptr;
     double *xs_val = nullptr, *xs_prod = nullptr;
     float *xs_val32 = nullptr, *xs_prod32 = nullptr;
@@ -188,6 +189,7 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
         if constexpr (sizeof(T) == 8) s.prod = reinterpret_cast<const T *>(a->xs_prod);
         else s.prod = reinterpret_cast<const T *>(a->xs_prod32);
         s.ppos = a->xs_pos;
+        s.rperm = a->rperm;
     }
     s.part_out = a->part;
     s.stream_nt = a->stream_nt;

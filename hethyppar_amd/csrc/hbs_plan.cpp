@@ -70,6 +70,45 @@ void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max
     }
 }
 
+int plan_row_order(const int *rowptr, HostPlan &plan, int m, std::vector<unsigned short> &rperm)
+{
+    rperm.assign((size_t)std::max(m, 1), 0);
+    const int th = kWideThreads, waves = kWideThreads / 64;
+    int nperm = 0;
+    std::vector<int> ord;
+    std::vector<long long> cost((size_t)waves);
+    for (size_t b = 0; b < plan.blocks.size(); ++b) {
+        const RowBlock &rb = plan.blocks[b];
+        BlockAux &ax = plan.aux[b];
+        const int nr = rb.r1 - rb.r0;
+        for (int k = 0; k < nr; ++k) rperm[(size_t)rb.r0 + k] = (unsigned short)k;
+        if (ax.kind == 1 || ax.lanes > 1 || nr < 2) continue;
+        auto len = [&](int k) { return rowptr[rb.r0 + k + 1] - rowptr[rb.r0 + k]; };
+        // slot s of the row sums belongs to wave (s % th) / 64; its rows are
+        // summed in batches of 8 products, so a slot of 64 rows costs the
+        // batches of its longest row
+        auto estimate = [&](const std::vector<int> &o) {
+            std::fill(cost.begin(), cost.end(), 0);
+            for (int s0 = 0; s0 < nr; s0 += 64) {
+                int mx = 0;
+                for (int s = s0; s < std::min(nr, s0 + 64); ++s) mx = std::max(mx, len(o[(size_t)s]));
+                cost[(size_t)((s0 % th) / 64)] += (mx + 7) / 8;
+            }
+            return *std::max_element(cost.begin(), cost.end());
+        };
+        ord.resize((size_t)nr);
+        for (int k = 0; k < nr; ++k) ord[(size_t)k] = k;
+        const long long c_row = estimate(ord);
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return len(x) > len(y); });
+        const long long c_sorted = estimate(ord);
+        if (c_sorted * 5 >= c_row * 4) continue;
+        ax.first = 1;
+        ++nperm;
+        for (int k = 0; k < nr; ++k) rperm[(size_t)rb.r0 + k] = (unsigned short)ord[(size_t)k];
+    }
+    return nperm;
+}
+
 int plan_sorted(const int *col, int nnz, int max_line_pct, HostPlan &plan, std::vector<int> &src,
                 std::vector<unsigned short> &pos)
 {
