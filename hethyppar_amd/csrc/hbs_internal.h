@@ -353,7 +353,12 @@ hipError_t launch_init_vectors(const VecArgs<double> &a, hipStream_t s);
 hipError_t launch_ir_start(const VecArgs<float> &a, const double *r64, const SolverState *outer,
                            hipStream_t s);
 // Mixed sweep end: x64 += scale * (double)x32 (scale from st_in).
-hipError_t launch_ir_update(const VecArgs<float> &a, double *x64, hipStream_t s);
+// x_out = x_in + X32 * scale of the sweep (x_out may be x_in).
+hipError_t launch_ir_update(const VecArgs<float> &a, const double *x_in, double *x_out, hipStream_t s);
+// Copies n states of a (then n of b, if b is not null) into the host-mapped
+// mirror, then (stamp != 0) stamps every copy after a system-scope fence.
+hipError_t launch_mirror(const SolverState *a, const SolverState *b, int n, SolverState *mirror, int stamp,
+                         hipStream_t s);
 // d0 = (x, y) partials into part_out.
 hipError_t launch_dot(const VecArgs<double> &a, const double *x, const double *y, hipStream_t s);
 hipError_t launch_state_init(SolverState *st, int nstates, double tol, int maxit, int fixed, hipStream_t s);

@@ -832,11 +832,24 @@ __global__ __launch_bounds__(kThreads) void k_ir_start(VecArgs<float> a, const d
     write_parts<2, float>(a, d0, d1, red);
 }
 
-__global__ __launch_bounds__(kThreads) void k_ir_update(VecArgs<float> a, double *x64)
+__global__ __launch_bounds__(kThreads) void k_ir_update(VecArgs<float> a, const double *x_in, double *x_out)
 {
     const VecBlock vb = a.blocks[blockIdx.x];
     const double sc = a.st_in[vb.sys].scale;
-    for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) x64[i] = x64[i] + (double)a.X[i] * sc;
+    for (int i = vb.e0 + threadIdx.x; i < vb.e1; i += kThreads) x_out[i] = x_in[i] + (double)a.X[i] * sc;
+}
+
+// States into the host-mapped mirror the host spins on: the copies first,
+// then, after a system-scope fence, the stamp in each copy.
+__global__ __launch_bounds__(kThreads) void k_mirror(const SolverState *a, const SolverState *b, int n,
+                                                    SolverState *mirror, int stamp)
+{
+    const int tot = b ? 2 * n : n;
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) mirror[q] = q < n ? a[q] : b[q - n];
+    if (!stamp) return;
+    __syncthreads();
+    __threadfence_system();
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) *reinterpret_cast<volatile int *>(&mirror[q].pad) = stamp;
 }
 
 __global__ __launch_bounds__(kThreads) void k_dot(VecArgs<double> a, const double *x, const double *y)
@@ -1082,10 +1095,18 @@ hipError_t launch_ir_start(const VecArgs<float> &a, const double *r64, const Sol
     return hipGetLastError();
 }
 
-hipError_t launch_ir_update(const VecArgs<float> &a, double *x64, hipStream_t s)
+hipError_t launch_ir_update(const VecArgs<float> &a, const double *x_in, double *x_out, hipStream_t s)
 {
     if (a.nvb <= 0) return hipSuccess;
-    k_ir_update<<<a.nvb, kThreads, 0, s>>>(a, x64);
+    k_ir_update<<<a.nvb, kThreads, 0, s>>>(a, x_in, x_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mirror(const SolverState *a, const SolverState *b, int n, SolverState *mirror, int stamp,
+                         hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_mirror<<<1, kThreads, 0, s>>>(a, b, n, mirror, stamp);
     return hipGetLastError();
 }
 

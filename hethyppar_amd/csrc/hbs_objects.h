@@ -125,6 +125,7 @@ struct hbs_csr_s {
 
 namespace hbs {
 constexpr int kSlotA = 0, kSlotB = 1, kSlotOuter = 2, kNumSlots = 4;
+constexpr int kMirrorSlots = 5;   // host mirror: two chunk polls, then states read after a solve or sweep
 struct BlockWork;                    // k right-hand-side workspace (hbs_block.cpp)
 void block_work_free(BlockWork *w);
 }  // namespace hbs
@@ -134,10 +135,11 @@ struct hbs_solver_s {
     int n = 0;
     int nsys = 1;                 // independent systems of A (diagonal blocks)
     double *X = nullptr, *R = nullptr, *R0 = nullptr, *P = nullptr, *V = nullptr, *S = nullptr, *T = nullptr;
+    double *Xb = nullptr;              // mixed: the second FP64 iterate (sweeps alternate X and Xb)
     float *X32 = nullptr, *R32 = nullptr, *R032 = nullptr, *P32 = nullptr, *V32 = nullptr, *S32 = nullptr,
           *T32 = nullptr;
     hbs::SolverState *st = nullptr;    // device [kNumSlots][nsys]
-    hbs::SolverState *hst = nullptr;   // pinned host [3][nsys]: two poll buffers, one read buffer
+    hbs::SolverState *hst = nullptr;   // pinned host [kMirrorSlots][nsys]: poll buffers 0, 1; read buffers 2..4
     hbs::SolverState *hst_dev = nullptr;   // device view of hst (host-mapped, coherent)
     hbs::VecBlock *vblocks = nullptr;
     hbs::SysRange *sys = nullptr;      // [nsys] {g0, g1, v0, v1}

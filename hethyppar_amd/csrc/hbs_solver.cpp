@@ -44,7 +44,7 @@ extern "C" int hbs_solver_destroy(hbs_solver s)
     if (!s) return HBS_OK;
     DevGuard g(s->A->ctx->device);
     (void)hipStreamSynchronize(s->A->ctx->stream);
-    for (double **p : {&s->X, &s->R, &s->R0, &s->P, &s->V, &s->S, &s->T}) dfree(*p);
+    for (double **p : {&s->X, &s->R, &s->R0, &s->P, &s->V, &s->S, &s->T, &s->Xb}) dfree(*p);
     for (float **p : {&s->X32, &s->R32, &s->R032, &s->P32, &s->V32, &s->S32, &s->T32}) dfree(*p);
     for (double **p : {&s->partA, &s->partB, &s->partC, &s->partD, &s->partE, &s->partF, &s->chunk_sums})
         dfree(*p);
@@ -86,11 +86,11 @@ extern "C" int hbs_solver_create(hbs_csr a, hbs_solver *out)
         if ((rc = dalloc(p, s->n))) return bail(rc);
     if ((rc = dalloc(&s->st, (size_t)kNumSlots * s->nsys))) return bail(rc);
     // host-mapped and coherent: k_apply_xr writes the polled states straight into it
-    if (hipHostMalloc((void **)&s->hst, 3 * sizeof(SolverState) * (size_t)s->nsys,
+    if (hipHostMalloc((void **)&s->hst, kMirrorSlots * sizeof(SolverState) * (size_t)s->nsys,
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void **)&s->hst_dev, s->hst, 0) != hipSuccess)
         return bail(fail(HBS_ERR_ALLOC, "pinned host allocation failed"));
-    std::memset((void *)s->hst, 0, 3 * sizeof(SolverState) * (size_t)s->nsys);   // no stamp yet
+    std::memset((void *)s->hst, 0, kMirrorSlots * sizeof(SolverState) * (size_t)s->nsys);   // no stamp yet
     // vector workgroups system by system; one block size for the launch
     std::vector<VecBlock> vb;
     std::vector<SysRange> ranges = a->sys_host;
@@ -266,14 +266,14 @@ std::atomic<double> g_wait_limit_s{600.0};
 // Wait until every mirrored state of poll slot `slot` carries `stamp`
 // (written by the chunk's last kernel after the states), checking the
 // stream for errors and the time limit now and then.
-int wait_stamp(hbs_solver s, int slot, int stamp)
+int wait_mirror(hbs_solver s, size_t first, int count, int stamp)
 {
-    const volatile SolverState *h = s->hst + (size_t)slot * s->nsys;
+    const volatile SolverState *h = s->hst + first;
     const auto t0 = std::chrono::steady_clock::now();
     const double limit = g_wait_limit_s.load();
     for (long spin = 0;; ++spin) {
         bool all = true;
-        for (int i = 0; i < s->nsys && all; ++i) all = h[i].pad == stamp;
+        for (int i = 0; i < count && all; ++i) all = h[i].pad == stamp;
         if (all) {
             std::atomic_thread_fence(std::memory_order_acquire);
             return HBS_OK;
@@ -284,7 +284,7 @@ int wait_stamp(hbs_solver s, int slot, int stamp)
                 return fail(HBS_ERR_HIP, "solver stream failed: %s", hipGetErrorString(e));
             if (e == hipSuccess) {
                 // the stream drained: the stamp must be visible now
-                for (int i = 0; i < s->nsys; ++i)
+                for (int i = 0; i < count; ++i)
                     if (h[i].pad != stamp) return fail(HBS_ERR_HIP, "chunk state stamp %d missing", stamp);
                 return HBS_OK;
             }
@@ -295,6 +295,17 @@ int wait_stamp(hbs_solver s, int slot, int stamp)
             std::this_thread::yield();
         }
     }
+}
+
+int wait_stamp(hbs_solver s, int slot, int stamp)
+{
+    return wait_mirror(s, (size_t)slot * s->nsys, s->nsys, stamp);
+}
+
+int next_stamp(hbs_solver s)
+{
+    s->seq = s->seq == INT_MAX ? 1 : s->seq + 1;
+    return s->seq;
 }
 
 }  // namespace
@@ -431,23 +442,6 @@ int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use
     return HBS_OK;
 }
 
-// Copies slot k's states of every system to out[nsys].
-int read_states(hbs_solver s, int k, SolverState *out)
-{
-    hipStream_t q = s->A->ctx->stream;
-    SolverState *h = s->hst + 2 * (size_t)s->nsys;
-    HIPCHK(hipMemcpyAsync(h, s->slot(k), sizeof(SolverState) * (size_t)s->nsys, hipMemcpyDeviceToHost, q));
-    HIPCHK(hipStreamSynchronize(q));
-    std::memcpy(out, h, sizeof(SolverState) * (size_t)s->nsys);
-    return HBS_OK;
-}
-
-int read_state(hbs_solver s, int k, SolverState *out)
-{
-    if (s->nsys != 1) return fail(HBS_ERR_ARG, "single-system call on a matrix of %d systems", s->nsys);
-    return read_states(s, k, out);
-}
-
 int status_of(const SolverState &st, bool fixed)
 {
     if (st.stop == 4) return HBS_ERR_BREAKDOWN;
@@ -540,8 +534,11 @@ int solve_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts &o,
     // final relative residual sqrt(||B - AX||^2 / ||B||^2) (bicgstab_cpu.c:251-255)
     // into S (dead between iterations), ||B||^2, x = X; queued by iterate
     // behind the chunk predicted to be the last
+    // The final states go to the host-mapped mirror (read buffer 2) before
+    // e1, so the host reads them once e1 has completed.
     auto tail = [&]() -> int {
         HIPCHK(residual_b(s, w.X, b, s->S, kSlotA));
+        HIPCHK(launch_mirror(s->slot(kSlotA), nullptr, s->nsys, s->hst_dev + 2 * (size_t)s->nsys, 0, q));
         if (!direct) HIPCHK(hipMemcpyAsync(x, w.X, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
         HIPCHK(hipEventRecord(s->e1, q));
         return HBS_OK;
@@ -553,8 +550,9 @@ int solve_f64(hbs_solver s, const double *b, double *x, const hbs_solve_opts &o,
     CHK(iterate<double>(s, w, std::max(o.maxit, 1), o.check_every, use_graphs,
                         x == b ? std::function<int()>{} : std::function<int()>{tail}, &tail_done));
     if (!tail_done) CHK(tail());
+    CHK(wait_event_limited(s->e1));
     std::vector<SolverState> hs((size_t)s->nsys);
-    CHK(read_states(s, kSlotA, hs.data()));
+    std::memcpy(hs.data(), (const void *)(s->hst + 2 * (size_t)s->nsys), sizeof(SolverState) * (size_t)s->nsys);
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
     for (int i = 0; i < s->nsys; ++i) fill_info(hs[(size_t)i], o.fixed_iters != 0, ms, &infos[i]);
@@ -650,18 +648,31 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
     DevGuard g(A->ctx->device);
     hipStream_t q = A->ctx->stream;
     const int n = s->n;
-    if (!s->X32)
+    if (!s->X32) {
         for (float **p : {&s->X32, &s->R32, &s->R032, &s->P32, &s->V32, &s->S32, &s->T32}) CHK(dalloc(p, n));
+        CHK(dalloc(&s->Xb, n));
+    }
     Work<float> w32 = work32(s);
+    SolverState *outer = s->slot(kSlotOuter);
+    // The host reads states from the host-mapped mirror: read buffer 3 the
+    // sweep's inner state, 4 the outer one, each stamped after a fence.
+    SolverState *const m_inner = s->hst_dev + 3, *const m_outer = s->hst_dev + 4;
+    const volatile SolverState *const h_inner = s->hst + 3, *const h_outer = s->hst + 4;
+    // the iterate alternates between X and Xb: a sweep reads one and writes
+    // the other, so its update can be queued again after a misprediction
+    double *xc = s->X, *xn = s->Xb;
 
     HIPCHK(hipEventRecord(s->e0, q));
-    HIPCHK(hipMemcpyAsync(s->X, x, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
-    HIPCHK(launch_state_init(s->slot(kSlotOuter), 1, o.tol, 0, 0, q));
+    HIPCHK(hipMemcpyAsync(xc, x, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
+    HIPCHK(launch_state_init(outer, 1, o.tol, 0, 0, q));
     // FP64 outer residual r = b - A x in R, ||b||^2
-    HIPCHK(residual(s, s->X, b, s->R, kSlotOuter));
+    HIPCHK(residual(s, xc, b, s->R, kSlotOuter));
     HIPCHK(bnorm2(s, b, kSlotOuter));
+    int stamp = next_stamp(s);
+    HIPCHK(launch_mirror(outer, nullptr, 1, m_outer, stamp, q));
+    CHK(wait_mirror(s, 4, 1, stamp));
     SolverState hs;
-    CHK(read_state(s, kSlotOuter, &hs));
+    std::memcpy(&hs, (const void *)h_outer, sizeof hs);
     const double r0 = std::sqrt(hs.resid2);
     const double bnorm2v = hs.out;
     const double thr = o.tol * r0;
@@ -678,14 +689,26 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
         VecArgs<float> va = vec_args<float>(s, w32);
         va.st_out = s->slot(kSlotA);
         va.part_out = s->partD;
-        HIPCHK(launch_ir_start(va, s->R, s->slot(kSlotOuter), q));
-        CHK(iterate<float>(s, w32, imax, o.check_every, graphs(s->A, o.use_graph)));
+        HIPCHK(launch_ir_start(va, s->R, outer, q));
         va.st_in = s->slot(kSlotA);
-        HIPCHK(launch_ir_update(va, s->X, q));
-        HIPCHK(residual(s, s->X, b, s->R, kSlotOuter));
+        // the sweep's tail: x_next = x + scale * d, r = b - A x_next (FP64),
+        // then both states to the mirror.  Queued behind the chunk predicted
+        // to be the sweep's last, and again if more chunks follow it.
+        auto tail = [&]() -> int {
+            HIPCHK(launch_ir_update(va, xc, xn, q));
+            HIPCHK(residual(s, xn, b, s->R, kSlotOuter));
+            stamp = next_stamp(s);
+            HIPCHK(launch_mirror(s->slot(kSlotA), outer, 1, m_inner, stamp, q));
+            return HBS_OK;
+        };
+        bool tail_done = false;
+        CHK(iterate<float>(s, w32, imax, o.check_every, graphs(s->A, o.use_graph), tail, &tail_done));
+        if (!tail_done) CHK(tail());
+        CHK(wait_mirror(s, 3, 2, stamp));
         SolverState hi;
-        CHK(read_state(s, kSlotA, &hi));
-        CHK(read_state(s, kSlotOuter, &hs));
+        std::memcpy(&hi, (const void *)h_inner, sizeof hi);
+        std::memcpy(&hs, (const void *)h_outer, sizeof hs);
+        std::swap(xc, xn);
         total_iters += std::max(hi.iter, 0);
         ++sweeps;
         rnorm = std::sqrt(hs.resid2);
@@ -697,9 +720,9 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
             break;
         }
     }
-    HIPCHK(hipMemcpyAsync(x, s->X, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
+    HIPCHK(hipMemcpyAsync(x, xc, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
     HIPCHK(hipEventRecord(s->e1, q));
-    HIPCHK(hipEventSynchronize(s->e1));
+    CHK(wait_event_limited(s->e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
     const int status = converged ? HBS_OK : (breakdown ? HBS_ERR_BREAKDOWN : HBS_ERR_NOCONV);
