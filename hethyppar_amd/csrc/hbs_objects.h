@@ -144,6 +144,12 @@ struct hbs_solver_s {
     hbs::VecBlock *vblocks = nullptr;
     hbs::SysRange *sys = nullptr;      // [nsys] {g0, g1, v0, v1}
     int nvb = 0;
+    // FP32 sweeps (one system): vector workgroups of twice the elements, so
+    // each moves as many bytes as an FP64 one and the fixed per-workgroup
+    // work (partial re-sums, state) is spread over as many bytes
+    hbs::VecBlock *vblocks32 = nullptr;
+    hbs::SysRange *sys32 = nullptr;    // {g0, g1, 0, nvb32}
+    int nvb32 = 0, epb32 = 0;
     // partials, one buffer per producer so a consumer never sees them overwritten
     double *partA = nullptr;      // SpMV1: (r0, v)
     double *partB = nullptr;      // k_vec_S: ||s||^2

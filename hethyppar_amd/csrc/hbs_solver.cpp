@@ -52,6 +52,8 @@ extern "C" int hbs_solver_destroy(hbs_solver s)
     if (s->hst) (void)hipHostFree(s->hst);
     dfree(s->vblocks);
     dfree(s->sys);
+    dfree(s->vblocks32);
+    dfree(s->sys32);
     for (int i = 0; i < 2; ++i)
         for (auto &bylen : s->graph[i])
             for (hipGraphExec_t g : bylen)
@@ -126,14 +128,25 @@ struct Work {
 Work<double> work64(hbs_solver s) { return {s->X, s->R, s->R0, s->P, s->V, s->S, s->T, s->A->val}; }
 Work<float> work32(hbs_solver s) { return {s->X32, s->R32, s->R032, s->P32, s->V32, s->S32, s->T32, s->A->val32}; }
 
+// The vector workgroups (and system ranges naming them) of T's iterations:
+// FP32 sweeps have their own plan of larger workgroups.
+template <typename T>
+const VecBlock *vblocks_of(hbs_solver s) { return sizeof(T) == 4 && s->sys32 ? s->vblocks32 : s->vblocks; }
+template <typename T>
+const SysRange *sys_of(hbs_solver s) { return sizeof(T) == 4 && s->sys32 ? s->sys32 : s->sys; }
+template <typename T>
+int nvb_of(hbs_solver s) { return sizeof(T) == 4 && s->sys32 ? s->nvb32 : s->nvb; }
+template <typename T>
+int epb_of(hbs_solver s) { return sizeof(T) == 4 && s->sys32 ? s->epb32 : vec_elems_per_block(s->n); }
+
 template <typename T>
 VecArgs<T> vec_args(hbs_solver s, const Work<T> &w)
 {
     VecArgs<T> a{};
-    a.blocks = s->vblocks;
-    a.sys = s->sys;
-    a.nvb = s->nvb;
-    a.upb = vec_packs_per_thread(vec_elems_per_block(s->n), (int)sizeof(T));
+    a.blocks = vblocks_of<T>(s);
+    a.sys = sys_of<T>(s);
+    a.nvb = nvb_of<T>(s);
+    a.upb = vec_packs_per_thread(epb_of<T>(s), (int)sizeof(T));
     a.X = w.X;
     a.R = w.R;
     a.P = w.P;
@@ -148,7 +161,7 @@ template <typename T>
 SpmvArgs<T> solver_spmv(hbs_solver s, const T *val)
 {
     SpmvArgs<T> sa = spmv_args<T>(s->A, val);
-    sa.sys = s->sys;
+    sa.sys = sys_of<T>(s);
     sa.chunk_sums = s->chunk_sums;
     return sa;
 }
@@ -166,7 +179,7 @@ hipError_t launch_iteration(hbs_solver s, const Work<T> &w)
     va.st_in = A;
     va.st_out = B;
     va.part_in = s->partD;
-    va.nparts_in = s->nvb;
+    va.nparts_in = nvb_of<T>(s);
     va.in_spmv = 0;
     if ((e = launch_vec_P(va, q)) != hipSuccess) return e;
     // V = A P ; (r0, v)
@@ -194,7 +207,7 @@ hipError_t launch_iteration(hbs_solver s, const Work<T> &w)
     sa.st_out = B;
     sa.fin_in = FIN_S;
     sa.part_in = s->partB;
-    sa.nparts_in = s->nvb;
+    sa.nparts_in = nvb_of<T>(s);
     sa.part_out = s->partC;
     if ((e = run_spmv(sa, s->A, EPI_DOT2, A, q)) != hipSuccess) return e;
     // omega ; X, R ; (r, r0), ||r||^2
@@ -216,7 +229,7 @@ hipError_t launch_chunk(hbs_solver s, const Work<T> &w, int k, int slot, int sta
     hipError_t e = hipSuccess;
     for (int i = 0; i < k && e == hipSuccess; ++i) e = launch_iteration<T>(s, w);
     if (e != hipSuccess) return e;
-    return launch_apply_xr(s->slot(kSlotA), s->sys, s->nsys, s->partD, s->nvb,
+    return launch_apply_xr(s->slot(kSlotA), sys_of<T>(s), s->nsys, s->partD, nvb_of<T>(s),
                            s->hst_dev + (size_t)slot * s->nsys, stamp, s->A->ctx->stream);
 }
 
@@ -651,6 +664,31 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
     if (!s->X32) {
         for (float **p : {&s->X32, &s->R32, &s->R032, &s->P32, &s->V32, &s->S32, &s->T32}) CHK(dalloc(p, n));
         CHK(dalloc(&s->Xb, n));
+        // FP32 vector workgroups: twice the FP64 elements (at most 8192),
+        // never more workgroups than the FP64 plan, whose partial buffers
+        // they share
+        std::vector<VecBlock> vb;
+        const int epb = std::min(2 * vec_elems_per_block(n), 8192);
+        plan_vec(0, n, epb, 0, vb);
+        std::vector<SysRange> r = A->sys_host;
+        r[0].v0 = 0;
+        r[0].v1 = (int)vb.size();
+        if ((int)vb.size() > s->nvb) return fail(HBS_ERR_ARG, "FP32 vector plan larger than the FP64 one");
+        // sys32 is set last: the FP32 iterations use the plan only once it
+        // is whole
+        VecBlock *blocks = nullptr;
+        SysRange *sys = nullptr;
+        CHK(dalloc(&blocks, vb.size()));
+        s->vblocks32 = blocks;
+        CHK(upload(blocks, vb));
+        CHK(dalloc(&sys, 1));
+        if (int rc = upload(sys, r)) {
+            dfree(sys);
+            return rc;
+        }
+        s->nvb32 = (int)vb.size();
+        s->epb32 = epb;
+        s->sys32 = sys;
     }
     Work<float> w32 = work32(s);
     SolverState *outer = s->slot(kSlotOuter);
