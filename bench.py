@@ -126,7 +126,7 @@ def main():
     ap.add_argument("--check-every", type=int, default=4, help="iterations per convergence poll")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--probe", choices=["circuit", "markov"],
+    ap.add_argument("--probe", choices=["circuit", "markov", "fem", "block"],
                     help="only the SpMV launches and a copy of known size (for counter runs)")
     ap.add_argument("--circuit", action="store_true",
                     help="only the circuit solves and SpMVs (for kernel traces)")
@@ -265,8 +265,15 @@ def probe(hbs, which):
     WRITE_SIZE."""
     import ctypes as C
     ctx = hbs.Context(0)
+    if which == "block":
+        probe_block(hbs, ctx)
+        ctx.close()
+        return
     if which == "circuit":
         rp, c, v, xt, _ = circuit(hbs)
+    elif which == "fem":
+        rp, c, v = hbs.gen_fem(483334, 12, 1.1, 6)
+        xt = hbs.gen_uniform(rp.size - 1, -1.0, 1.0, 62)
     else:
         rp, c, v = hbs.gen_markov(445000, 12, 22, 8900, 1.05, 2)
         xt = hbs.gen_uniform(445000, -1.0, 1.0, 12)
@@ -286,6 +293,36 @@ def probe(hbs, which):
                       "spmv_fp64_ms": ms64, "spmv_fp64_bytes": spmv_bytes(n, n, nnz, 8),
                       "copy_doubles": cn, "copy_ms": cms.value}), flush=True)
     ctx.close()
+
+
+def probe_block(hbs, ctx):
+    """The 64-column SpMM of the 128^3 operator: 1 + 10 FP64 and 1 + 10 FP32
+    launches (one 64-column tile), then the copy of 2^25 doubles 1 + 3
+    times."""
+    import ctypes as C
+    rp, c, v = hbs.gen_convdiff3d(128, 128, 128, 0.5)
+    n, nnz, k = rp.size - 1, int(c.size), 64
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    X = hbs.gen_uniform(n * k, -1.0, 1.0, 4)
+    res = {"probe": "block", "n": n, "nnz": nnz, "k": k, "spmm_runs": A.info()["spmm_runs"]}
+    for name, dt, eb in (("fp64", np.float64, 8), ("fp32", np.float32, 4)):
+        dX, dY = ctx.to_device(X.astype(dt)), ctx.empty(n * k, dt)
+        A.spmm(dX, dY, k, 64)
+        ctx.timer_start()
+        for _ in range(10):
+            A.spmm(dX, dY, k, 64)
+        ms = ctx.timer_stop() / 10
+        res[f"spmm_{name}_ms"] = ms
+        res[f"spmm_{name}_bytes"] = nnz * (4 + eb) + 4 * (n + 1) + 2 * k * eb * n
+        dX.free()
+        dY.free()
+    cn = 1 << 25
+    s, d = ctx.empty(cn), ctx.empty(cn)
+    s.zero()
+    cms = C.c_double()
+    hbs.check(hbs.lib().hbs_bench_copy(ctx.h, s.ptr, d.ptr, cn, 3, C.byref(cms)))
+    res.update({"copy_doubles": cn, "copy_ms": cms.value})
+    print(json.dumps(res), flush=True)
 
 
 def timed(ctx, fn, reps=1):

@@ -26,6 +26,8 @@ int g_spmv_npt = 0;       // entries per thread per pass of 512-thread SpMV bloc
 int g_sort_pct = 70;      // kind-2 blocks: sorted gathers touch <= 70% as many lines as entries
 int g_xseg = 2;           // x-segmented SpMV: 0 never, 1 every one-system matrix, 2 scattered columns
 int g_xseg_split = 1;     // k_xseg workgroups per x segment
+int g_spmm_order = 1;     // multi-column SpMM on a 3-D grid: 1 y-slab runs (plan_grid_runs), 0 rows in order
+int g_spmm_dynamic = 1;   // 64-column SpMM sweeps: 1 steps from per-XCD counters, 0 fixed stride
 }  // namespace
 
 #define fail ::hbs::set_error
@@ -329,6 +331,50 @@ extern "C" int hbs_set_spmv_pass(int entries_per_thread)
     return HBS_OK;
 }
 
+extern "C" int hbs_set_spmm_order(int mode)
+{
+    if (mode != 0 && mode != 1) return fail(HBS_ERR_ARG, "mode must be 0 (rows in order) or 1 (grid runs)");
+    g_spmm_order = mode;
+    return HBS_OK;
+}
+
+extern "C" int hbs_set_spmm_tuning(int slab_lines, int groups, int dynamic, int wide_loads)
+{
+    if (slab_lines < 0 || groups < 0 || (groups > 0 && (groups < 8 || groups % 8 != 0)) || dynamic < 0 ||
+        dynamic > kCtrPerXcd || (wide_loads != 0 && wide_loads != 1))
+        return fail(HBS_ERR_ARG, "slab_lines >= 0; groups 0 or a multiple of 8; dynamic 0..%d; wide_loads 0 or 1",
+                    kCtrPerXcd);
+    g_grid_slab = slab_lines;
+    g_spmm_groups = groups;
+    g_spmm_dynamic = dynamic > 0;
+    if (dynamic > 0) g_spmm_counters = dynamic;
+    g_spmm_v16 = wide_loads;
+    return HBS_OK;
+}
+
+extern "C" int hbs_plan_grid_runs(int m, const int *rowptr, const int *col, int *runs, int cap, int *nruns,
+                                  int *len)
+{
+    if (m < 0 || !rowptr || !col || !nruns || !len || cap < 0 || (cap > 0 && !runs))
+        return fail(HBS_ERR_ARG, "bad argument");
+    std::vector<int> r;
+    int l = 0;
+    const int nr = plan_grid_runs(m, rowptr, col, r, l);
+    if (nr > cap) return fail(HBS_ERR_ARG, "%d runs do not fit the %d given", nr, cap);
+    if (nr > 0) std::memcpy(runs, r.data(), sizeof(int) * (size_t)nr);
+    *nruns = nr;
+    *len = l;
+    return HBS_OK;
+}
+
+extern "C" int hbs_csr_spmm_runs(hbs_csr a, int *nruns, int *len)
+{
+    if (!a || !nruns || !len) return fail(HBS_ERR_ARG, "bad argument");
+    *nruns = a->rr_nruns;
+    *len = a->rr_len;
+    return HBS_OK;
+}
+
 extern "C" int hbs_set_xseg_split(int split)
 {
     if (split < 1 || split > 16) return fail(HBS_ERR_ARG, "split must be in [1, 16]");
@@ -366,6 +412,8 @@ extern "C" int hbs_csr_destroy(hbs_csr a)
     dfree(a->xs_col);
     dfree(a->xs_pos);
     dfree(a->rperm);
+    dfree(a->rr_run);
+    dfree(a->spmm_ctr);
     dfree(a->xs_delta);
     dfree(a->xs_val);
     dfree(a->xs_prod);
@@ -507,6 +555,24 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
         ranges.push_back(SysRange{g0, (int)hp.groups.size(), 0, 0});
     }
     a->sys_host = ranges;
+    // step counters of the 64-column SpMM sweeps, zero until a sweep uses
+    // them (each launch leaves them at zero)
+    if (g_spmm_dynamic) {
+        const size_t nctr = (size_t)kSpmmCtrTiles * kXcds * kCtrSet * kCtrStride;
+        if ((rc = dalloc(&a->spmm_ctr, nctr))) return bail(rc);
+        if (hipMemset(a->spmm_ctr, 0, sizeof(int) * nctr) != hipSuccess)
+            return bail(fail(HBS_ERR_HIP, "SpMM counter initialisation failed"));
+    }
+    // multi-column SpMM traversal of a 3-D grid operator
+    if (g_spmm_order == 1 && nsys == 1 && m == n) {
+        std::vector<int> runs;
+        int len = 0;
+        if (plan_grid_runs(m, rowptr, col, runs, len) > 0) {
+            if ((rc = dalloc(&a->rr_run, runs.size())) || (rc = upload(a->rr_run, runs))) return bail(rc);
+            a->rr_len = len;
+            a->rr_nruns = (int)runs.size();
+        }
+    }
     XsegHost xh;
     if (xseg) {
         // the segment's chunk tables must fit k_xseg's LDS beside x
@@ -797,9 +863,9 @@ int spmm_launch(hbs_csr a, int k, const T *X, T *Y, int tile)
 {
     hipError_t e;
     if constexpr (sizeof(T) == 8)
-        e = launch_spmm_f64(a->m, a->rowptr, a->col, a->val, k, X, Y, tile, a->ctx->stream);
+        e = launch_spmm_f64(a->m, a->rowptr, a->col, a->val, k, X, Y, tile, a->row_runs(), a->ctx->stream);
     else
-        e = launch_spmm_f32(a->m, a->rowptr, a->col, a->val32, k, X, Y, tile, a->ctx->stream);
+        e = launch_spmm_f32(a->m, a->rowptr, a->col, a->val32, k, X, Y, tile, a->row_runs(), a->ctx->stream);
     return e == hipSuccess ? HBS_OK : fail(HBS_ERR_HIP, "spmm launch failed: %s", hipGetErrorString(e));
 }
 

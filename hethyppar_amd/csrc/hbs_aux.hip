@@ -70,14 +70,16 @@ __global__ __launch_bounds__(kThreads) void k_grad_pattern(int m, const int *row
 // Y[r, c] = sum_j val[j] * X[col[j], c], row-major with ld = k.  `tile`
 // lanes share one row (one RHS column each) and sum it in CSR order, so
 // every column equals the single-RHS order-exact sum.  Each XCD takes a
-// contiguous share of the rows, dealt in tiles round robin over its
-// workgroups (gridDim.x, a multiple of 8), so that neighbouring rows -- which
-// gather the same X rows -- are worked on together under one L2.
+// contiguous share of the traversal rr (the rows in order, or runs of
+// consecutive rows in y-slab order on a 3-D grid), dealt in tiles round
+// robin over its workgroups (gridDim.x, a multiple of 8), so that rows
+// which gather the same X rows are worked on together under one L2.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_spmm(int m, const int *__restrict__ rowptr,
                                                   const int *__restrict__ col,
                                                   const T *__restrict__ val, int k, int ld,
-                                                  const T *__restrict__ X, T *__restrict__ Y, int tile)
+                                                  const T *__restrict__ X, T *__restrict__ Y, int tile,
+                                                  RowRuns rr)
 {
     constexpr int RB = 2;   // rows per lane in flight together
     const int lane = threadIdx.x & (tile - 1);
@@ -85,39 +87,60 @@ __global__ __launch_bounds__(kThreads) void k_spmm(int m, const int *__restrict_
     const int c = blockIdx.y * tile + lane;
     if (c >= k) return;
     const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, per = gridDim.x / kXcds;
-    const int x0 = (int)((long long)m * xcd / kXcds), x1 = (int)((long long)m * (xcd + 1) / kXcds);
+    int x0, x1;
+    xcd_positions(rr, m, xcd, x0, x1);
     for (int t0 = x0 + wi * RB * rstep; t0 < x1; t0 += per * RB * rstep) {
-        const int r = t0 + (int)threadIdx.x / tile;
+        const int p = t0 + (int)threadIdx.x / tile;
         T acc[RB];
-        rows_dot<T, RB>(rowptr, col, val, X + c, ld, r, rstep, x1, acc);
+        rows_dot<T, RB>(rowptr, col, val, X + c, ld, rr, p, rstep, x1, acc);
 #pragma unroll
         for (int q = 0; q < RB; ++q)
-            if (r + q * rstep < x1) Y[(size_t)(r + q * rstep) * ld + c] = acc[q];
+            if (p + q * rstep < x1) Y[(size_t)pos_row(rr, p + q * rstep) * ld + c] = acc[q];
     }
 }
 
-// Tiles of 64 columns: one wave per row step (wave_rows64), so the row's
-// indices and values are read once per wave through the scalar cache and
-// every gather is a contiguous 64-column stretch of an X row.  Rows are
-// dealt per XCD as in k_spmm, in steps of RB rows per wave.
-template <typename T>
+// Tiles of 64 columns: one wave per row step (wave_sweep64), so each entry
+// is read once per wave and every gather is a contiguous 64-column stretch
+// of an X row.  Each XCD sweeps its share of the positions in steps of RB
+// consecutive rows (a run's length is even, so a step never leaves its
+// run), taken from the tile's step counter where it has one.
+template <typename T, bool V16>
 __global__ __launch_bounds__(kThreads) void k_spmm64(int m, const int *__restrict__ rowptr,
                                                     const int *__restrict__ col, const T *__restrict__ val,
-                                                    int k, int ld, const T *__restrict__ X, T *__restrict__ Y)
+                                                    int k, int ld, const T *__restrict__ X, T *__restrict__ Y,
+                                                    RowRuns rr)
 {
     constexpr int RB = 2;
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.y * 64 + lane;
     const bool ok = c < k;
     const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, per = gridDim.x / kXcds;
-    const int x0 = (int)((long long)m * xcd / kXcds), x1 = (int)((long long)m * (xcd + 1) / kXcds);
-    const int wv = wi * kWaves + (int)(threadIdx.x >> 6), nw = per * kWaves;
-    for (int r = x0 + wv * RB; r < x1; r += nw * RB) {
-        const int ru = __builtin_amdgcn_readfirstlane(r);
-        wave_rows64<T, RB>(rowptr, col, val, X, ld, c, ru, x1, ok, [&](int q, T acc) {
-            if (ok && ru + q < x1) Y[(size_t)(ru + q) * ld + c] = acc;
-        });
+    int x0, x1;
+    xcd_positions(rr, m, xcd, x0, x1);
+    const int wv = __builtin_amdgcn_readfirstlane(wi * kWaves + (int)(threadIdx.x >> 6)), nw = per * kWaves;
+    const SweepCtr sc = sweep_ctr(
+        rr.ctr && (int)blockIdx.y < rr.ctr_tiles ? rr.ctr + ((int)blockIdx.y * kXcds + xcd) * kCtrSet * kCtrStride
+                                                 : nullptr,
+        wv, nw, rr.ctr_per_xcd);
+    int p0, ps;
+    if constexpr (V16) {
+        // 16-byte gathers: a step of G rows, columns in 16-byte pieces
+        constexpr int G = 64 / (64 / Cols16<T>::N);
+        sweep_span(sc, wv, nw, x0, G, p0, ps);
+        const int cbase = (int)blockIdx.y * 64;
+        wave_sweep64v<T>(rowptr, col, val, X + cbase, ld, min(64, k - cbase), rr, p0, x1, ps, sc.mine,
+                         [&](int, int row, const Cols16<T> &acc, int cc) {
+                             if (cbase + cc < k)
+                                 *reinterpret_cast<Cols16<T> *>(Y + (size_t)row * ld + cbase + cc) = acc;
+                         });
+    } else {
+        sweep_span(sc, wv, nw, x0, RB, p0, ps);
+        wave_sweep64<T, RB>(rowptr, col, val, X, ld, c, ok, rr, p0, x1, ps, sc.mine, [](int, int) {},
+                            [&](int, int row, T acc) {
+                                if (ok) Y[(size_t)row * ld + c] = acc;
+                            });
     }
+    sweep_done(sc, nw);
 }
 
 // ---- reference-shaped kernels (hhp_dp_kernels.cu:16-74) -------------------
@@ -227,29 +250,35 @@ hipError_t launch_grad_pattern(int m, const int *rowptr, const int *col, const d
 // About 2048 workgroups in all, the row workgroups a multiple of 8.
 template <typename T>
 hipError_t spmm_dispatch(int m, const int *rowptr, const int *col, const T *val, int k, const T *X, T *Y,
-                         int tile, hipStream_t s)
+                         int tile, const RowRuns &rr, hipStream_t s)
 {
     if (m <= 0 || k <= 0) return hipSuccess;
     if (tile < 1 || tile > 64 || (tile & (tile - 1))) return hipErrorInvalidValue;
     const int rstep = kThreads / tile;
     const int ytiles = (k + tile - 1) / tile;
     const long long passes = ((long long)m + 2LL * rstep - 1) / (2LL * rstep);
-    long long gx = std::min(std::max(1LL, 2048LL / ytiles), passes);
+    const long long target = g_spmm_groups > 0 ? g_spmm_groups : 2048;
+    long long gx = std::min(std::max(1LL, target / ytiles), passes);
     gx = std::max(8LL, (gx + 7) / 8 * 8);
     dim3 grid((unsigned)gx, (unsigned)ytiles);
-    if (tile == 64) k_spmm64<T><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y);
-    else k_spmm<T><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, tile);
+    if (rr.run && ((long long)rr.nruns * rr.len != m || rr.len % 2 != 0)) return hipErrorInvalidValue;
+    // 16-byte gathers where every lane's 16 bytes lie in a row
+    const bool v16 = g_spmm_v16 && ((size_t)k * sizeof(T)) % 16 == 0 && (uintptr_t)X % 16 == 0 &&
+                     (uintptr_t)Y % 16 == 0;
+    if (tile == 64 && v16) k_spmm64<T, true><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, rr);
+    else if (tile == 64) k_spmm64<T, false><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, rr);
+    else k_spmm<T><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, tile, rr);
     return hipGetLastError();
 }
 hipError_t launch_spmm_f64(int m, const int *rowptr, const int *col, const double *val, int k,
-                           const double *X, double *Y, int tile, hipStream_t s)
+                           const double *X, double *Y, int tile, const RowRuns &rr, hipStream_t s)
 {
-    return spmm_dispatch<double>(m, rowptr, col, val, k, X, Y, tile, s);
+    return spmm_dispatch<double>(m, rowptr, col, val, k, X, Y, tile, rr, s);
 }
 hipError_t launch_spmm_f32(int m, const int *rowptr, const int *col, const float *val, int k,
-                           const float *X, float *Y, int tile, hipStream_t s)
+                           const float *X, float *Y, int tile, const RowRuns &rr, hipStream_t s)
 {
-    return spmm_dispatch<float>(m, rowptr, col, val, k, X, Y, tile, s);
+    return spmm_dispatch<float>(m, rowptr, col, val, k, X, Y, tile, rr, s);
 }
 
 hipError_t launch_dp_vecop_P(double *P, const double *V, const double *R, const double *omega,

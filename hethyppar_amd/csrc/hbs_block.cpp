@@ -116,6 +116,7 @@ BlockArgs group_args(hbs_solver s, BlockWork *w, int c0, int kc)
     a.st = w->st;
     a.skip_done = 1;
     a.part = w->part;
+    a.rr = s->A->row_runs();
     return a;
 }
 

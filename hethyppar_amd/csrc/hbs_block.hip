@@ -35,19 +35,22 @@ __device__ __forceinline__ LaneMap lane_map(const BlockArgs &a)
     return m;
 }
 
-// Rows of this workgroup: its XCD's contiguous share of the rows, dealt in
-// tiles of `tile` rows round robin over the XCD's workgroups, so that the
-// workgroups of one XCD work on neighbouring rows at the same time and the x
-// rows they gather stay in that XCD's L2.  ngroups is a multiple of 8.
+// Positions of this workgroup in the traversal rr: its XCD's contiguous
+// share, dealt in tiles of `tile` positions round robin over the XCD's
+// workgroups, so that the workgroups of one XCD work on neighbouring rows
+// at the same time and the x rows they gather stay in that XCD's L2.
+// ngroups is a multiple of 8.  The streaming kernels take the rows in order
+// (rr without runs).
 struct RowSweep {
     int first, end, stride;
 };
-__device__ __forceinline__ RowSweep row_sweep(int n, int ngroups, int tile)
+__device__ __forceinline__ RowSweep row_sweep(const RowRuns &rr, int n, int ngroups, int tile)
 {
     const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds;
     const int per = ngroups / kXcds;
-    const int r0 = (int)((long long)n * xcd / kXcds), r1 = (int)((long long)n * (xcd + 1) / kXcds);
-    return RowSweep{r0 + wi * tile, r1, per * tile};
+    int p0, p1;
+    xcd_positions(rr, n, xcd, p0, p1);
+    return RowSweep{p0 + wi * tile, p1, per * tile};
 }
 
 // Column sums of ND per-lane values -> part[d][c][p], fixed order: xor
@@ -118,38 +121,53 @@ __global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a, const int *__re
     };
     const bool any = __syncthreads_or(act);
     if constexpr (WIDE) {
-        // one row per wave step, RB rows per step: indices and values through
-        // the scalar cache, one contiguous 64-column gather per entry
+        // RB consecutive rows per wave step (wave_sweep64), one contiguous
+        // 64-column gather per entry; every wave arrives at the XCD's step
+        // counter once (sweep_done), also when no column is active
         constexpr int RB = 2;
+        const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, per = a.ngroups / kXcds;
+        const int nw = per * kWaves;
+        const int wv = __builtin_amdgcn_readfirstlane(wi * kWaves + (int)(threadIdx.x >> 6));
+        const SweepCtr sc =
+            sweep_ctr(a.rr.ctr ? a.rr.ctr + xcd * kCtrSet * kCtrStride : nullptr, wv, nw, a.rr.ctr_per_xcd);
         if (!any) {
+            sweep_done(sc, nw);
             column_partials<ND>(a, v, p);
             return;
         }
-        const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, per = a.ngroups / kXcds;
-        const int x0 = (int)((long long)a.n * xcd / kXcds), x1 = (int)((long long)a.n * (xcd + 1) / kXcds);
-        const int wv = wi * kWaves + (int)(threadIdx.x >> 6), nw = per * kWaves;
-        for (int r = x0 + wv * RB; r < x1; r += nw * RB) {
-            const int ru = __builtin_amdgcn_readfirstlane(r);
-            double wr[RB];
+        // positions of a step are consecutive rows of one run (runs have an
+        // even length and each XCD's share starts at a run); the epilogue
+        // operands load with the step's gathers (every lane, in bounds)
+        int x0, x1;
+        xcd_positions(a.rr, a.n, xcd, x0, x1);
+        const int wc = m.col_ok ? m.c : 0;
+        double wr[RB];
+        int p0, ps;
+        sweep_span(sc, wv, nw, x0, RB, p0, ps);
+        wave_sweep64<double, RB>(
+            rowptr, col, val, a.x, a.ld, m.c, act, a.rr, p0, x1, ps, sc.mine,
+            [&](int ru, int nr) {
 #pragma unroll
-            for (int q = 0; q < RB; ++q) wr[q] = act && ru + q < x1 ? a.w[(size_t)(ru + q) * a.ld + m.c] : 0.0;
-            wave_rows64<double, RB>(rowptr, col, val, a.x, a.ld, m.c, ru, x1, act, [&](int q, double acc) {
-                if (act && ru + q < x1) emit(ru + q, acc, wr[q]);
+                for (int q = 0; q < RB; ++q) wr[q] = a.w[(size_t)(ru + min(q, max(nr - 1, 0))) * a.ld + wc];
+            },
+            [&](int q, int row, double acc) {
+                if (act) emit(row, acc, wr[q]);
             });
-        }
+        sweep_done(sc, nw);
     } else if (any) {
         constexpr int RB = 2;   // rows per lane in flight together
-        const RowSweep sw = row_sweep(a.n, a.ngroups, RB * m.rstep);
+        const RowSweep sw = row_sweep(a.rr, a.n, a.ngroups, RB * m.rstep);
         const double *__restrict__ x = a.x + m.c;
         for (int t0 = sw.first; t0 < sw.end; t0 += sw.stride) {
             if (!act) continue;
-            const int r = t0 + m.ro;
+            const int p = t0 + m.ro;
             double accs[RB];
-            rows_dot<double, RB>(a.rowptr, a.col, a.val, x, a.ld, r, m.rstep, sw.end, accs);
+            rows_dot<double, RB>(a.rowptr, a.col, a.val, x, a.ld, a.rr, p, m.rstep, sw.end, accs);
 #pragma unroll
             for (int q = 0; q < RB; ++q) {
-                const int rq = r + q * m.rstep;
-                if (rq >= sw.end) break;
+                const int pq = p + q * m.rstep;
+                if (pq >= sw.end) break;
+                const int rq = pos_row(a.rr, pq);
                 emit(rq, accs[q], a.w[(size_t)rq * a.ld + m.c]);
             }
         }
@@ -175,7 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_bvec(BlockArgs a)
             be = s.beta;
             sconv = s.s_conv;
         }
-        const RowSweep sw = row_sweep(a.n, a.ngroups, m.rstep);
+        const RowSweep sw = row_sweep(RowRuns{}, a.n, a.ngroups, m.rstep);
         for (int t0 = sw.first; t0 < sw.end; t0 += sw.stride) {
             const int r = t0 + m.ro;
             if (!act || r >= sw.end) continue;
@@ -237,6 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_bfin(SolverState *st, const double
 hipError_t launch_block_spmm(const BlockArgs &a, int epi, hipStream_t s)
 {
     if (a.ngroups <= 0) return hipSuccess;
+    if (a.rr.run && ((long long)a.rr.nruns * a.rr.len != a.n || a.rr.len % 2 != 0)) return hipErrorInvalidValue;
     switch (epi) {
     case BEPI_DOT1:
         if (a.kp == 64) k_bspmm<BEPI_DOT1, true><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);

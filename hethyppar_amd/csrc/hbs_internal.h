@@ -18,6 +18,7 @@ constexpr int kWideThreads = 512;      // SpMV workgroup size for row blocks abo
 constexpr int kMaxBlockNnz = 16384;    // upper bound of nnz per SpMV row block (128 KiB of FP64 LDS)
 constexpr int kLongRow = 256;          // rows longer than this are summed in chunks (k_chunks)
 constexpr int kChunkLen = 1024;        // entries per chunk of such a row
+constexpr int kXcds = 8;                // MI355X: 8 XCDs, workgroups dispatched round robin
 constexpr int kRowPad = 8;             // entries of padding past nnz in col / val / val32
 constexpr int kSegCols = 8192;         // x-segmented SpMV: columns per segment (64 KiB of FP64 x in LDS)
 constexpr int kSegBlockNnz = 8192;     // x-segmented SpMV: entries per row block
@@ -208,6 +209,26 @@ struct VecArgs {
 // columns (kp = kc rounded up to a power of two): lane l works on column
 // l % kp of row offset l / kp, so one row's columns are contiguous in HBM.
 constexpr int kBlockCols = 64;
+// Row traversal of the multi-column SpMM kernels: runs of `len` consecutive
+// rows (len even), run k starting at row run[k], the runs covering every
+// row once; each XCD takes an equal contiguous share of the runs.  run ==
+// nullptr: the rows in order, each XCD an equal contiguous share of them.
+// ctr: step counters, zero between launches, for ctr_tiles column tiles:
+// per tile and XCD a set of kCtrSet counters, kCtrStride ints apart -- the
+// kCtrPerXcd step counters, then the arrival count.  The waves of a tile
+// with counters take their steps from them, the others in a fixed stride.
+struct RowRuns {
+    const int *run = nullptr;
+    int len = 0, nruns = 0;
+    int *ctr = nullptr;
+    int ctr_tiles = 0;
+    int ctr_per_xcd = 0;   // step counters in use per XCD (at most kCtrPerXcd)
+};
+constexpr int kSpmmCtrTiles = 16;   // column tiles with step counters per matrix
+constexpr int kCtrPerXcd = 16;      // step counters per XCD (steps dealt round robin over them)
+constexpr int kCtrSet = kCtrPerXcd + 1;
+constexpr int kCtrStride = 64;      // ints between two counters (256 bytes)
+
 struct BlockArgs {
     int n, ld, kc, kp;
     int ngroups;                  // workgroups (a multiple of 8) = partials per column
@@ -221,6 +242,7 @@ struct BlockArgs {
     const SolverState *st;        // [kc], read-only in the streaming kernels
     int skip_done;                // 1: columns whose state is done are skipped
     double *part;                 // [kMaxDots][kBlockCols][ngroups]
+    RowRuns rr;                   // SpMM row traversal
 };
 enum BlockEpi : int {
     BEPI_DOT1 = 0,    // y = A x ; d0 = (w, y)
@@ -300,8 +322,17 @@ void plan_groups(int rb0, int rb1, int ngroups, int sys, HostPlan &plan);
 void plan_vec(int e0, int n, int elems_per_block, int sys, std::vector<VecBlock> &blocks);
 // The same blocks into out[cap] (out may be null to count); -1 if more than cap.
 int plan_vec_into(int e0, int n, int elems_per_block, int sys, VecBlock *out, int cap);
+// Runs of the multi-column SpMM for a square matrix that is a 7-point
+// operator on a 3-D grid too large for an XCD's L2 (y-slab order): the run
+// starts and len = nx; returns the number of runs, 0 (runs empty) when the
+// matrix is not such a grid.
+int plan_grid_runs(int m, const int *rowptr, const int *col, std::vector<int> &runs, int &len);
 int vec_elems_per_block(int n);
 extern int g_vec_target;      // vector workgroups a plan aims at (hbs_set_vec_blocks)
+extern int g_grid_slab;       // y-lines per slab of plan_grid_runs (0: from the L2 size)
+extern int g_spmm_groups;     // workgroups of the multi-column SpMM (0: 2048)
+extern int g_spmm_v16;        // 64-column SpMM: 16-byte gathers where aligned (1) or 8/4-byte (0)
+extern int g_spmm_counters;   // step counters per XCD of the dealt 64-column sweeps
 int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 
 // ---- launchers (hbs_kernels.hip) ----
@@ -383,9 +414,9 @@ hipError_t launch_grad_pattern(int m, const int *rowptr, const int *col, const d
 
 // SpMM (row-major blocks, ld = k).
 hipError_t launch_spmm_f64(int m, const int *rowptr, const int *col, const double *val, int k,
-                           const double *X, double *Y, int tile, hipStream_t s);
+                           const double *X, double *Y, int tile, const RowRuns &rr, hipStream_t s);
 hipError_t launch_spmm_f32(int m, const int *rowptr, const int *col, const float *val, int k,
-                           const float *X, float *Y, int tile, hipStream_t s);
+                           const float *X, float *Y, int tile, const RowRuns &rr, hipStream_t s);
 
 // Reference-shaped fused kernels (hhp_dp_kernels.h).
 hipError_t launch_dp_vecop_P(double *P, const double *V, const double *R, const double *omega,

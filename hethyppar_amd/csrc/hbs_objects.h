@@ -115,6 +115,19 @@ struct hbs_csr_s {
     hbs_csr_s *tr = nullptr;
     int *tr_perm = nullptr;           // [nnz] position of entry j of A in A^T
     std::map<std::pair<int, int>, int> spmm_tile;   // (k, dtype) -> tile
+    // multi-column SpMM traversal: runs of rr_len consecutive rows (a 3-D
+    // grid's x-lines in y-slab order, plan_grid_runs); null: rows in order
+    int *rr_run = nullptr;
+    int rr_len = 0, rr_nruns = 0;
+    // step counters of the 64-column SpMM sweeps (zero between launches):
+    // kSpmmCtrTiles column tiles x 8 XCDs x kCtrSet, kCtrStride ints apart.
+    // The multi-column SpMMs of one matrix run one at a time (stream order).
+    int *spmm_ctr = nullptr;
+    hbs::RowRuns row_runs() const
+    {
+        return hbs::RowRuns{rr_run, rr_len, rr_nruns, spmm_ctr, spmm_ctr ? hbs::kSpmmCtrTiles : 0,
+                            hbs::g_spmm_counters};
+    }
     int stream_nt = 0;                // non-temporal col/val loads
     int nsys = 1;                     // independent diagonal blocks (systems)
     std::vector<int> sys_rows;        // [nsys + 1] row offsets of the systems

@@ -1,5 +1,8 @@
 // hbs_plan.cpp -- row-block, workgroup and vector-block planning.
 #include <algorithm>
+#include <cstdlib>
+#include <map>
+#include <vector>
 
 #include "hbs_internal.h"
 
@@ -301,7 +304,73 @@ int plan_vec_into(int e0, int n, int elems_per_block, int sys, VecBlock *out, in
     return k;
 }
 
+// Row runs of the multi-column SpMM on a 3-D structured grid (plan_grid_runs).
+// The rows of a 7-point operator on an nx x ny x nz grid (x fastest) gather
+// X rows r +- 1, r +- nx and r +- nx*ny.  In row order the +-z uses of an X
+// row lie nx*ny rows apart -- 8 MiB of 64 FP64 columns at 128^2 -- so each
+// one misses an XCD's 4 MiB L2 and X is fetched about once per use.  Swept
+// in y-slabs of H lines, plane after plane, the three planes that use an X
+// row are consecutive, and a slab's window of three planes of X stays in
+// the L2; the +-y neighbours of the slab's edge lines are the only re-reads
+// ((H + 2) / H of X).  The runs are the grid's x-lines (nx consecutive
+// rows); each XCD takes an equal contiguous share of the runs, whole slabs
+// when ny / 8 is a multiple of H.
+namespace {
+constexpr long long kL2PerXcd = 4LL << 20;
+constexpr int kGridRowBytes = 64 * 8;     // one X row of a 64-column FP64 block
+}  // namespace
+
+int plan_grid_runs(int m, const int *rowptr, const int *col, std::vector<int> &runs, int &len)
+{
+    runs.clear();
+    len = 0;
+    if (m < 8 * 1024 || !rowptr || !col) return 0;
+    // offsets |c - r| of a sample of rows
+    std::map<long long, long long> cnt;
+    long long sampled = 0, entries = 0;
+    const int stride = std::max(1, m / 20000);
+    for (int r = 0; r < m; r += stride) {
+        ++sampled;
+        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            ++cnt[std::llabs((long long)col[j] - r)];
+            ++entries;
+        }
+    }
+    // the distinct offsets other than 0 must be 1, nx and nx * ny, each in at
+    // least a fifth of the rows
+    std::vector<long long> offs;
+    long long covered = cnt.count(0) ? cnt[0] : 0;
+    for (const auto &kv : cnt)
+        if (kv.first > 0 && kv.second * 5 >= sampled) {
+            offs.push_back(kv.first);
+            covered += kv.second;
+        }
+    if (covered != entries || offs.size() != 3 || offs[0] != 1) return 0;
+    const long long nx = offs[1], nxy = offs[2];
+    if (nxy % nx != 0 || m % nxy != 0) return 0;
+    const long long ny = nxy / nx, nz = m / nxy;
+    // worth it only when a plane of X outgrows the L2
+    if (nxy * kGridRowBytes <= kL2PerXcd || ny < 8 || nz < 3) return 0;
+    if (nx % 2 != 0) return 0;   // runs hold whole steps of two rows
+    // slab height: three planes of the slab within half the L2, a divisor of
+    // each XCD's share of the y-lines when there is one
+    long long h = g_grid_slab > 0 ? g_grid_slab : std::max(1LL, (kL2PerXcd / 2) / (3 * nx * kGridRowBytes));
+    h = std::min(h, std::max(1LL, ny / 8));
+    while (h > 1 && (ny / 8) % h != 0) --h;
+    for (long long y0 = 0; y0 < ny; y0 += h) {
+        const long long y1 = std::min(ny, y0 + h);
+        for (long long z = 0; z < nz; ++z)
+            for (long long y = y0; y < y1; ++y) runs.push_back((int)(z * nxy + y * nx));
+    }
+    len = (int)nx;
+    return (int)runs.size();
+}
+
 int g_vec_target = 512;
+int g_grid_slab = 0;
+int g_spmm_groups = 0;
+int g_spmm_v16 = 1;
+int g_spmm_counters = kCtrPerXcd;
 
 // Elements per vector workgroup: 512 * u with u in {1, 2, 4, 8}, the
 // smallest that keeps the workgroups (and the partials a consumer sums) at

@@ -147,6 +147,10 @@ _PROTOS = {
     "hbs_csr_plan_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "hbs_set_spmv_xseg": (_i, [_i]),
     "hbs_set_xseg_split": (_i, [_i]),
+    "hbs_set_spmm_order": (_i, [_i]),
+    "hbs_set_spmm_tuning": (_i, [_i, _i, _i, _i]),
+    "hbs_plan_grid_runs": (_i, [_i, _ip, _ip, _ip, _i, _ip, _ip]),
+    "hbs_csr_spmm_runs": (_i, [_vp, _ip, _ip]),
     "hbs_csr_xseg": (_i, [_vp, _ip]),
     "hbs_shard_pack_host": (_i, [_vp, _vp, _dp]),
     "hbs_vector_part": (_i, [_i, _dp, _ip, _i, _i, _dp, _ip]),
@@ -614,6 +618,9 @@ class DeviceCSR:
         ns = C.c_int()
         check(lib().hbs_csr_xseg(self.h, C.byref(ns)))
         d["xseg"] = ns.value
+        nr, ln = C.c_int(), C.c_int()
+        check(lib().hbs_csr_spmm_runs(self.h, C.byref(nr), C.byref(ln)))
+        d["spmm_runs"], d["spmm_run_len"] = nr.value, ln.value
         return d
 
     def spmv(self, x, y, alpha=1.0, beta=0.0):
@@ -785,6 +792,32 @@ def set_spmv_plan(block_nnz=0, serial_max_avg=64, max_groups=0, stream_nt=0, npt
 
 def set_xseg_split(split=1):
     check(lib().hbs_set_xseg_split(split))
+
+
+def set_spmm_order(mode=1):
+    """Process-wide: multi-column SpMM traversal for DeviceCSR objects created
+    afterwards (1 y-slab runs on a 3-D grid operator, 0 rows in order)."""
+    check(lib().hbs_set_spmm_order(mode))
+
+
+def set_spmm_tuning(slab_lines=0, groups=0, dynamic=16, wide_loads=1):
+    """Process-wide multi-column SpMM tuning: slab height of the grid runs
+    and step dealing (counters per XCD, 0 for a fixed stride; DeviceCSR
+    objects created afterwards), workgroups per launch, 16-byte X gathers."""
+    check(lib().hbs_set_spmm_tuning(slab_lines, groups, dynamic, wide_loads))
+
+
+def plan_grid_runs(rowptr, col):
+    """(runs, len) the SpMM traversal plans for a host CSR; runs is empty
+    when the matrix is not a 3-D grid operator that needs it."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+    c = np.ascontiguousarray(col, dtype=np.int32)
+    m = rp.size - 1
+    out = np.empty(max(m, 1), dtype=np.int32)
+    nr, ln = C.c_int(), C.c_int()
+    check(lib().hbs_plan_grid_runs(m, rp.ctypes.data_as(_ip), c.ctypes.data_as(_ip), out.ctypes.data_as(_ip),
+                                   int(out.size), C.byref(nr), C.byref(ln)))
+    return out[:nr.value].copy(), ln.value
 
 
 def set_spmv_xseg(mode=2):

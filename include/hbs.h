@@ -405,6 +405,31 @@ int hbs_set_xseg_split(int split);
 /* Segments of an x-segmented device CSR (0: CSR-stream SpMV). */
 int hbs_csr_xseg(hbs_csr a, int *nseg);
 
+/* Row traversal of the multi-column SpMM (hbs_csr_spmm_*, the block solve)
+ * for device CSRs created afterwards: 1 (default) sweeps a square matrix
+ * that is a 7-point operator on a 3-D grid -- offsets 0, +-1, +-nx, +-nx*ny
+ * only, a plane of X larger than an XCD's L2 -- as runs of x-lines in
+ * y-slabs, plane after plane, so the X rows a slab gathers stay in L2; 0
+ * takes the rows in order.  Each row's sum is the same either way.  No
+ * counterpart in the reference (its solver has one right-hand side). */
+int hbs_set_spmm_order(int mode);
+/* The runs that mode 1 plans for a host CSR (m x m): *nruns run starts in
+ * runs[0 .. cap) (error if more), each a run of *len consecutive rows;
+ * *nruns == 0 when the matrix is not such a grid. */
+int hbs_plan_grid_runs(int m, const int *rowptr, const int *col, int *runs, int cap, int *nruns, int *len);
+/* Tuning of the multi-column SpMM: y-lines per slab of the grid runs of
+ * device CSRs created afterwards (0: from the L2 size), workgroups per SpMM
+ * launch (0: 2048; else a multiple of 8), how the 64-column sweeps deal
+ * their row steps for device CSRs created afterwards (> 0: each XCD's
+ * waves take the next step from one of `dynamic` counters (1..16, default
+ * 16), so they work on neighbouring rows; 0: a fixed stride per wave) and whether hbs_csr_spmm_* gathers X in
+ * 16-byte pieces where k * element size and X, Y are 16-byte aligned (1,
+ * default) or one element per lane (0).  The multi-column SpMMs of one
+ * matrix (hbs_csr_spmm_*, the block solve) must not overlap in time. */
+int hbs_set_spmm_tuning(int slab_lines, int groups, int dynamic, int wide_loads);
+/* Runs of a device CSR's SpMM traversal (*nruns == 0: rows in order). */
+int hbs_csr_spmm_runs(hbs_csr a, int *nruns, int *len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -173,18 +173,69 @@ def test_spmv_transpose(ctx):
     assert np.array_equal(dy.download(), ref)
 
 
-@pytest.mark.parametrize("k,tile", [(64, 0), (64, 64), (64, 16), (8, 0), (5, 4)])
+@pytest.mark.parametrize("k,tile", [(64, 0), (64, 64), (64, 16), (8, 0), (5, 4), (130, 64)])
 def test_spmm_columns_bit_exact(ctx, k, tile):
     rp, c, v, xt, b = convdiff64()
     n = rp.size - 1
     X = np.random.default_rng(k).uniform(-1, 1, (n, k))
     A = hbs.DeviceCSR(ctx, rp, c, v)
     dX, dY = ctx.to_device(X.ravel()), ctx.empty(n * k)
+    Ao = orc.CSR(rp, c, v)
+    # twice: the second launch starts from the step counters the first left
+    for rep in range(2):
+        dY.zero()
+        A.spmm(dX, dY, k, tile)
+        Y = dY.download().reshape(n, k)
+        for j in range(k):
+            assert np.array_equal(Y[:, j], orc.spmv(Ao, X[:, j])), f"launch {rep} column {j}"
+
+
+def test_spmm_tiny_matrix_few_waves(ctx):
+    """A 25-row matrix: the 64-column SpMM runs 8 workgroups, 4 waves per
+    XCD, fewer than an XCD's step counters; every row is still formed,
+    launch after launch."""
+    rp, c, v = hbs.gen_convdiff2d(5, 5, 0.5)
+    n, k = rp.size - 1, 64
+    X = np.random.default_rng(3).uniform(-1, 1, (n, k))
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    dX, dY = ctx.to_device(X.ravel()), ctx.empty(n * k)
+    Ao = orc.CSR(rp, c, v)
+    for rep in range(3):
+        dY.zero()
+        A.spmm(dX, dY, k, 64)
+        Y = dY.download().reshape(n, k)
+        for j in range(k):
+            assert np.array_equal(Y[:, j], orc.spmv(Ao, X[:, j])), f"launch {rep} column {j}"
+
+
+@pytest.mark.parametrize("k,tile", [(64, 64), (64, 16), (24, 8)])
+def test_spmm_grid_runs_bit_exact(ctx, k, tile):
+    """A 3-D grid operator whose plane outgrows an XCD's L2 is swept in
+    y-slab runs: every column still equals the oracle's serial SpMV, and the
+    FP32 SpMM equals the one over rows in order."""
+    rp, c, v = hbs.gen_convdiff3d(128, 128, 6, 0.5)
+    n = rp.size - 1
+    X = np.random.default_rng(k + tile).uniform(-1, 1, (n, k))
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    assert A.info()["spmm_runs"] == 128 * 6 and A.info()["spmm_run_len"] == 128
+    dX, dY = ctx.to_device(X.ravel()), ctx.empty(n * k)
     A.spmm(dX, dY, k, tile)
     Y = dY.download().reshape(n, k)
     Ao = orc.CSR(rp, c, v)
     for j in range(k):
-        assert np.array_equal(Y[:, j], orc.spmv(Ao, X[:, j]))
+        assert np.array_equal(Y[:, j], orc.spmv(Ao, X[:, j])), f"column {j}"
+    X32 = X.astype(np.float32)
+    dX32, dY32 = ctx.to_device(X32.ravel()), ctx.empty(n * k, np.float32)
+    A.spmm(dX32, dY32, k, tile)
+    hbs.set_spmm_order(0)
+    try:
+        B = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    finally:
+        hbs.set_spmm_order(1)
+    assert B.info()["spmm_runs"] == 0
+    dZ32 = ctx.empty(n * k, np.float32)
+    B.spmm(dX32, dZ32, k, tile)
+    assert np.array_equal(dY32.download(), dZ32.download())
 
 
 def test_spmm_f32(ctx):

@@ -254,3 +254,29 @@ def test_mm_read_large_file_parallel_parse_bit_exact(tmp_path):
     m2, n2, rp2, c2, v2 = orc.mm_read(str(path))
     assert (m1, n1) == (m2, n2)
     assert np.array_equal(rp1, rp2) and np.array_equal(c1, c2) and np.array_equal(v1, v2)
+
+
+def test_spmm_grid_runs_plan():
+    """The multi-column SpMM traversal of a 3-D grid operator: x-lines in
+    y-slabs, plane after plane; together the runs cover every row once."""
+    nx, ny, nz = 128, 128, 6
+    rp, c, _ = hbs.gen_convdiff3d(nx, ny, nz, 0.5)
+    m = rp.size - 1
+    runs, ln = hbs.plan_grid_runs(rp, c)
+    assert ln == nx and runs.size == ny * nz
+    rows = (runs[:, None] + np.arange(ln)[None, :]).ravel()
+    assert np.array_equal(np.sort(rows), np.arange(m))
+    assert np.all(runs % nx == 0)
+    z, y = runs // (nx * ny), (runs % (nx * ny)) // nx
+    # slabs of h lines: z runs through every plane before the next slab
+    h = int(np.argmax(z != z[0]))
+    assert h >= 1 and (ny // 8) % h == 0
+    for s in range(ny // h):
+        blk = slice(s * h * nz, (s + 1) * h * nz)
+        assert np.array_equal(y[blk].reshape(nz, h), np.tile(np.arange(s * h, (s + 1) * h), (nz, 1)))
+        assert np.array_equal(z[blk].reshape(nz, h), np.repeat(np.arange(nz), h).reshape(nz, h))
+    # planes that fit an L2, a 2-D grid and scattered columns: rows in order
+    for rp2, c2 in (hbs.gen_convdiff3d(32, 32, 16, 0.5)[:2], hbs.gen_convdiff2d(512, 256, 0.5)[:2],
+                    hbs.gen_markov(20000, 4, 8, 500, 1.05, 3)[:2]):
+        r2, _ = hbs.plan_grid_runs(rp2, c2)
+        assert r2.size == 0
