@@ -27,8 +27,9 @@ struct BlockWork {
     double *X = nullptr, *R = nullptr, *R0 = nullptr, *P = nullptr, *V = nullptr, *S = nullptr, *T = nullptr;
     SolverState *st = nullptr;    // device [kBlockCols]
     SolverState *hst = nullptr;   // pinned host [3][kBlockCols]
-    double *part = nullptr;       // [kMaxDots][kBlockCols][ngroups], consumed before the next producer
+    double *part = nullptr;       // [kMaxDots][kBlockCols][kMaxBlockGroups], consumed before the next producer
     int ngroups = 0;              // a multiple of 8: rows are dealt per XCD
+    int tile = 0, rb = 0;         // columns per group, SpMM rows per lane step (block_tune)
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;   // (first column, columns, chunk)
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
@@ -56,6 +57,20 @@ int pow2_ceil(int v)
     return p;
 }
 
+constexpr int kMaxBlockGroups = 2048;
+int g_block_tile = 0, g_block_rb = 0;   // forced block-solve tuning (0: tuned)
+
+// Workgroups of the streaming and SpMM kernels for groups of `tile`
+// columns: up to kMaxBlockGroups, a multiple of the 8 XCDs; a pass of 4
+// waves covers 4 * (64 / kp) rows.
+int block_groups(int n, int tile)
+{
+    const int kp = pow2_ceil(std::min(tile, kBlockCols));
+    const long long pass = (long long)kWaves * (64 / kp);
+    const long long want = std::min<long long>(kMaxBlockGroups, (n + pass - 1) / pass);
+    return (int)std::max<long long>(8, (want + 7) / 8 * 8);
+}
+
 int block_work(hbs_solver s, int k, BlockWork **out)
 {
     BlockWork *w = s->blk;
@@ -70,17 +85,12 @@ int block_work(hbs_solver s, int k, BlockWork **out)
     w = new BlockWork();
     w->n = s->n;
     w->k = k;
-    // up to 2048 workgroups, a multiple of the 8 XCDs; a pass of 4 waves
-    // covers 4 * (64 / kp) rows
-    const int kp = pow2_ceil(std::min(k, kBlockCols));
-    const long long pass = (long long)kWaves * (64 / kp);
-    const long long want = std::min<long long>(2048, (w->n + pass - 1) / pass);
-    w->ngroups = (int)std::max<long long>(8, (want + 7) / 8 * 8);
+    w->ngroups = block_groups(w->n, k);
     const size_t nk = (size_t)w->n * k;
     int rc = HBS_OK;
     for (double **p : {&w->X, &w->R, &w->R0, &w->P, &w->V, &w->S, &w->T})
         if (rc == HBS_OK) rc = dalloc(p, nk);
-    if (rc == HBS_OK) rc = dalloc(&w->part, (size_t)kMaxDots * kBlockCols * w->ngroups);
+    if (rc == HBS_OK) rc = dalloc(&w->part, (size_t)kMaxDots * kBlockCols * kMaxBlockGroups);
     if (rc == HBS_OK) rc = dalloc(&w->st, kBlockCols);
     if (rc == HBS_OK && hipHostMalloc((void **)&w->hst, 3 * sizeof(SolverState) * kBlockCols) != hipSuccess)
         rc = fail(HBS_ERR_ALLOC, "pinned host allocation failed");
@@ -117,7 +127,86 @@ BlockArgs group_args(hbs_solver s, BlockWork *w, int c0, int kc)
     a.skip_done = 1;
     a.part = w->part;
     a.rr = s->A->row_runs();
+    a.rb = w->rb > 0 ? w->rb : 2;
     return a;
+}
+
+// The block solve's autotuner: for k right-hand sides, the column-group
+// width (the SpMM's RHS tile: 64, 32, 16 ... columns, at most k rounded up
+// to a power of two) and the SpMM rows per lane step (1, 2, 4; with the
+// tile's 64 / tile rows per wave, the rows a wavefront works on at once).
+// Each candidate times the (r0, v) SpMM of one group, best of 3 after a
+// warm-up, on the workspace's own blocks; the score is that time times the
+// groups the k columns need.  Forced choices (hbs_set_block_tuning) skip the
+// timing.  Once per workspace (k).
+int block_tune(hbs_solver s, BlockWork *w, hipStream_t q)
+{
+    const int kmax = pow2_ceil(std::min(w->k, kBlockCols));
+    std::vector<int> tiles, rbs;
+    if (g_block_tile > 0) tiles.push_back(std::min(g_block_tile, kmax));
+    else
+        for (int t = kmax; t >= std::max(1, kmax / 4); t /= 2) tiles.push_back(t);
+    if (g_block_rb > 0) rbs.push_back(g_block_rb);
+    else rbs = {1, 2, 4};
+    // the grid's runs hold whole steps of the 64-column sweep
+    const RowRuns rr = s->A->row_runs();
+    if (tiles.size() == 1 && rbs.size() == 1) {
+        w->tile = tiles[0];
+        w->rb = rr.run && rr.len % rbs[0] != 0 ? 2 : rbs[0];
+        w->ngroups = block_groups(w->n, w->tile);
+        return HBS_OK;
+    }
+    const size_t nk = (size_t)w->n * w->k;
+    HIPCHK(launch_fill_zero(w->P, sizeof(double) * nk, q));
+    HIPCHK(launch_fill_zero(w->R0, sizeof(double) * nk, q));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(HBS_ERR_HIP, "event creation failed");
+    }
+    double best = -1.0;
+    int bt = tiles[0], brb = 2, rc = HBS_OK;
+    for (int t : tiles) {
+        for (int rb : rbs) {
+            if (rr.run && rr.len % rb != 0) continue;
+            BlockArgs a = group_args(s, w, 0, std::min(t, w->k));
+            a.ngroups = block_groups(w->n, t);
+            a.rb = rb;
+            a.skip_done = 0;
+            a.x = w->P;
+            a.y = w->V;
+            a.w = w->R0;
+            float ms = 1e30f;
+            hipError_t e = launch_block_spmm(a, BEPI_DOT1, q);
+            for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+                float m = 0.f;
+                if ((e = hipEventRecord(e0, q)) != hipSuccess || (e = launch_block_spmm(a, BEPI_DOT1, q)) != hipSuccess ||
+                    (e = hipEventRecord(e1, q)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+                    (e = hipEventElapsedTime(&m, e0, e1)) != hipSuccess)
+                    break;
+                ms = std::min(ms, m);
+            }
+            if (e != hipSuccess) {
+                rc = fail(HBS_ERR_HIP, "block SpMM tuning launch failed: %s", hipGetErrorString(e));
+                break;
+            }
+            const double score = (double)ms * ((w->k + t - 1) / t);
+            if (best < 0.0 || score < best) {
+                best = score;
+                bt = t;
+                brb = rb;
+            }
+        }
+        if (rc != HBS_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != HBS_OK) return rc;
+    w->tile = bt;
+    w->rb = brb;
+    w->ngroups = block_groups(w->n, bt);
+    return HBS_OK;
 }
 
 hipError_t block_iteration(const BlockArgs &g, BlockWork *w, hipStream_t q)
@@ -199,6 +288,27 @@ int block_iterate(const BlockArgs &g, BlockWork *w, int c0, int maxit, int check
 
 }  // namespace
 
+extern "C" int hbs_set_block_tuning(int tile, int rows_per_step)
+{
+    if (tile < 0 || tile > kBlockCols || (tile & (tile - 1)) != 0 ||
+        (rows_per_step != 0 && rows_per_step != 1 && rows_per_step != 2 && rows_per_step != 4))
+        return fail(HBS_ERR_ARG, "tile 0 or a power of two <= %d; rows_per_step 0, 1, 2 or 4", kBlockCols);
+    g_block_tile = tile;
+    g_block_rb = rows_per_step;
+    return HBS_OK;
+}
+
+extern "C" int hbs_block_tuning(hbs_solver s, int k, int *tile, int *rows_per_step, int *rows_per_wave)
+{
+    if (!s || !tile || !rows_per_step || !rows_per_wave) return fail(HBS_ERR_ARG, "null argument");
+    const BlockWork *w = s->blk;
+    if (!w || w->k != k || w->tile == 0) return fail(HBS_ERR_ARG, "no block solve with k = %d tuned yet", k);
+    *tile = w->tile;
+    *rows_per_step = w->rb;
+    *rows_per_wave = (64 / pow2_ceil(w->tile)) * w->rb;
+    return HBS_OK;
+}
+
 extern "C" int hbs_bicgstab_block_f64(hbs_solver s, int k, const double *B, double *X, const hbs_solve_opts *opts,
                                       hbs_solve_info *infos)
 {
@@ -215,12 +325,13 @@ extern "C" int hbs_bicgstab_block_f64(hbs_solver s, int k, const double *B, doub
     BlockWork *w = nullptr;
     CHK(block_work(s, k, &w));
     const size_t nk = (size_t)w->n * k;
+    if (w->tile == 0) CHK(block_tune(s, w, q));
     hipEvent_t e0 = s->e0, e1 = s->e1;
     HIPCHK(hipEventRecord(e0, q));
     HIPCHK(hipMemcpyAsync(w->X, X, sizeof(double) * nk, hipMemcpyDeviceToDevice, q));
     std::vector<SolverState> hs((size_t)k);
-    for (int c0 = 0; c0 < k; c0 += kBlockCols) {
-        const int kc = std::min(kBlockCols, k - c0);
+    for (int c0 = 0; c0 < k; c0 += w->tile) {
+        const int kc = std::min(w->tile, k - c0);
         const BlockArgs g = group_args(s, w, c0, kc);
         HIPCHK(launch_state_init(w->st, kc, o.tol, o.maxit, o.fixed_iters ? 1 : 0, q));
         // R = B - A X ; R0 = R ; P = V = 0 ; iteration 0's rho and threshold

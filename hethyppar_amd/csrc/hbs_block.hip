@@ -91,7 +91,7 @@ __device__ __forceinline__ bool column_active(const BlockArgs &a, const LaneMap 
 // WIDE (kp == 64): one row per wave step; otherwise 64 / kp rows per wave.
 // The CSR arrays come as their own restrict arguments, so the compiler may
 // read them through the scalar cache.
-template <int EPI, bool WIDE>
+template <int EPI, bool WIDE, int RB>
 __global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a, const int *__restrict__ rowptr,
                                                    const int *__restrict__ col, const double *__restrict__ val)
 {
@@ -124,7 +124,6 @@ __global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a, const int *__re
         // RB consecutive rows per wave step (wave_sweep64), one contiguous
         // 64-column gather per entry; every wave arrives at the XCD's step
         // counter once (sweep_done), also when no column is active
-        constexpr int RB = 2;
         const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, per = a.ngroups / kXcds;
         const int nw = per * kWaves;
         const int wv = __builtin_amdgcn_readfirstlane(wi * kWaves + (int)(threadIdx.x >> 6));
@@ -155,7 +154,6 @@ __global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a, const int *__re
             });
         sweep_done(sc, nw);
     } else if (any) {
-        constexpr int RB = 2;   // rows per lane in flight together
         const RowSweep sw = row_sweep(a.rr, a.n, a.ngroups, RB * m.rstep);
         const double *__restrict__ x = a.x + m.c;
         for (int t0 = sw.first; t0 < sw.end; t0 += sw.stride) {
@@ -250,28 +248,37 @@ __global__ __launch_bounds__(kThreads) void k_bfin(SolverState *st, const double
     }
 }
 
+template <int EPI, int RB>
+void bspmm_launch(const BlockArgs &a, hipStream_t s)
+{
+    if (a.kp == 64) k_bspmm<EPI, true, RB><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
+    else k_bspmm<EPI, false, RB><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
+}
+template <int EPI>
+hipError_t bspmm_rb(const BlockArgs &a, hipStream_t s)
+{
+    switch (a.rb) {
+    case 1: bspmm_launch<EPI, 1>(a, s); break;
+    case 2: bspmm_launch<EPI, 2>(a, s); break;
+    case 4: bspmm_launch<EPI, 4>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_block_spmm(const BlockArgs &a, int epi, hipStream_t s)
 {
     if (a.ngroups <= 0) return hipSuccess;
-    if (a.rr.run && ((long long)a.rr.nruns * a.rr.len != a.n || a.rr.len % 2 != 0)) return hipErrorInvalidValue;
+    if (a.rr.run && ((long long)a.rr.nruns * a.rr.len != a.n || a.rr.len % 2 != 0 || a.rr.len % a.rb != 0))
+        return hipErrorInvalidValue;
     switch (epi) {
-    case BEPI_DOT1:
-        if (a.kp == 64) k_bspmm<BEPI_DOT1, true><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
-        else k_bspmm<BEPI_DOT1, false><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
-        break;
-    case BEPI_DOT2:
-        if (a.kp == 64) k_bspmm<BEPI_DOT2, true><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
-        else k_bspmm<BEPI_DOT2, false><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
-        break;
-    case BEPI_RESID:
-        if (a.kp == 64) k_bspmm<BEPI_RESID, true><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
-        else k_bspmm<BEPI_RESID, false><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
-        break;
+    case BEPI_DOT1: return bspmm_rb<BEPI_DOT1>(a, s);
+    case BEPI_DOT2: return bspmm_rb<BEPI_DOT2>(a, s);
+    case BEPI_RESID: return bspmm_rb<BEPI_RESID>(a, s);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_block_vec(const BlockArgs &a, int kind, hipStream_t s)

@@ -243,6 +243,7 @@ struct BlockArgs {
     int skip_done;                // 1: columns whose state is done are skipped
     double *part;                 // [kMaxDots][kBlockCols][ngroups]
     RowRuns rr;                   // SpMM row traversal
+    int rb;                       // SpMM rows per lane step (1, 2, 4)
 };
 enum BlockEpi : int {
     BEPI_DOT1 = 0,    // y = A x ; d0 = (w, y)

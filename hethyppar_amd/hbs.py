@@ -149,6 +149,8 @@ _PROTOS = {
     "hbs_set_xseg_split": (_i, [_i]),
     "hbs_set_spmm_order": (_i, [_i]),
     "hbs_set_spmm_tuning": (_i, [_i, _i, _i, _i]),
+    "hbs_set_block_tuning": (_i, [_i, _i]),
+    "hbs_block_tuning": (_i, [_vp, _i, _ip, _ip, _ip]),
     "hbs_plan_grid_runs": (_i, [_i, _ip, _ip, _ip, _i, _ip, _ip]),
     "hbs_csr_spmm_runs": (_i, [_vp, _ip, _ip]),
     "hbs_csr_xseg": (_i, [_vp, _ip]),
@@ -711,6 +713,13 @@ class Solver:
             check(rc)
         return list(infos)
 
+    def block_tuning(self, k):
+        """(tile, rows per lane step, rows per wavefront) the block solve's
+        autotuner chose for k right-hand sides."""
+        t, r, w = C.c_int(), C.c_int(), C.c_int()
+        check(lib().hbs_block_tuning(self.h, k, C.byref(t), C.byref(r), C.byref(w)))
+        return t.value, r.value, w.value
+
     def adjoint(self, g, x, lam, grad_val, opts=None, raise_on_error=True, **kw):
         info = SolveInfo()
         o = opts or default_opts(**kw)
@@ -798,6 +807,12 @@ def set_spmm_order(mode=1):
     """Process-wide: multi-column SpMM traversal for DeviceCSR objects created
     afterwards (1 y-slab runs on a 3-D grid operator, 0 rows in order)."""
     check(lib().hbs_set_spmm_order(mode))
+
+
+def set_block_tuning(tile=0, rows_per_step=0):
+    """Process-wide: force the block solve's column-group width and SpMM rows
+    per lane step for workspaces created afterwards (0: tuned)."""
+    check(lib().hbs_set_block_tuning(tile, rows_per_step))
 
 
 def set_spmm_tuning(slab_lines=0, groups=0, dynamic=16, wide_loads=1):

@@ -179,12 +179,21 @@ int hbs_bicgstab_systems_f64(hbs_solver s, const double *b_dev, double *x_dev,
 /* k right-hand sides: B_dev and X_dev are row-major n x k blocks (ld = k),
  * X holds X0 on entry.  One BiCGStab per column, each with its own scalars,
  * iteration count and stop test; columns are processed in groups of up to
- * 64 whose SpMM passes read A once for the whole group.  Each column's
+ * 64 (the tuned tile) whose SpMM passes read A once for the whole group.  Each column's
  * products are summed in CSR order, as the single-vector solve does.
  * infos[k]; returns the worst status.  No reference counterpart (the
  * reference solves one right-hand side per run). */
 int hbs_bicgstab_block_f64(hbs_solver s, int k, const double *B_dev, double *X_dev,
                            const hbs_solve_opts *opts, hbs_solve_info *infos);
+/* The block solve's autotuner picks, once per workspace (solver and k), the
+ * column-group width -- the SpMM's RHS tile, a power of two up to 64 and
+ * k -- and the SpMM rows per lane step (1, 2, 4), timing the (r0, v) SpMM
+ * of each candidate on the workspace.  hbs_set_block_tuning forces either
+ * (0: tuned) for workspaces created afterwards; hbs_block_tuning reports the
+ * choice for k: tile, rows per lane step, and the rows a wavefront works on
+ * at once ((64 / tile) * rows per step).  No reference counterpart. */
+int hbs_set_block_tuning(int tile, int rows_per_step);
+int hbs_block_tuning(hbs_solver s, int k, int *tile, int *rows_per_step, int *rows_per_wave);
 /* Mixed precision: FP32 BiCGStab sweeps (FP32 values and vectors, int32
  * indices, compensated dots) inside FP64 iterative refinement; FP64 for
  * r = b - A x, the scalar reductions and the convergence norm.  Needs

@@ -58,6 +58,21 @@ def test_plan_knob_validation_without_device():
     assert L.hbs_set_spmv_xseg(2) == 0
 
 
+def test_spmm_and_block_knob_validation_without_device():
+    L = hbs.lib()
+    assert L.hbs_set_spmm_order(2) == -1 and b"mode" in L.hbs_last_error()
+    assert L.hbs_set_spmm_order(1) == 0
+    assert L.hbs_set_spmm_tuning(0, 12, 16, 1) == -1 and b"groups" in L.hbs_last_error()
+    assert L.hbs_set_spmm_tuning(0, 0, 17, 1) == -1
+    assert L.hbs_set_spmm_tuning(0, 0, 16, 2) == -1
+    assert L.hbs_set_spmm_tuning(0, 0, 16, 1) == 0
+    assert L.hbs_set_block_tuning(48, 0) == -1 and b"tile" in L.hbs_last_error()
+    assert L.hbs_set_block_tuning(64, 3) == -1
+    assert L.hbs_set_block_tuning(0, 0) == 0
+    nr, ln = ctypes.c_int(), ctypes.c_int()
+    assert L.hbs_block_tuning(None, 4, ctypes.byref(nr), ctypes.byref(ln), ctypes.byref(ln)) == -1
+
+
 def test_wait_limit_knob_validation_without_device():
     L = hbs.lib()
     assert L.hbs_set_wait_limit(ctypes.c_double(0.0)) == -1 and b"wait limit" in L.hbs_last_error()

@@ -139,6 +139,54 @@ def test_block_columns_match_oracle(ctx, k):
     assert len(loose) <= MAX_ORDER_SENSITIVE * k
 
 
+@pytest.mark.parametrize("tile,rb", [(64, 1), (64, 4), (32, 2), (16, 1), (8, 4)])
+def test_block_forced_tiles_match_oracle(ctx, tile, rb):
+    """Every column-group width and SpMM rows per lane step the autotuner
+    can choose gives each column the oracle's solve."""
+    rp, c, v = hbs.gen_convdiff2d(24, 20, 0.7)
+    n, k = rp.size - 1, 40
+    Xt = hbs.gen_uniform(n * k, -1.0, 1.0, 7).reshape(n, k)
+    B = np.stack([hbs.host_spmv(rp, c, v, np.ascontiguousarray(Xt[:, j])) for j in range(k)], axis=1)
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    hbs.set_block_tuning(tile, rb)
+    try:
+        S = hbs.Solver(A)
+        dB = ctx.to_device(np.ascontiguousarray(B).ravel())
+        dX = ctx.to_device(np.zeros(n * k))
+        infos = S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=1000)
+        assert S.block_tuning(k) == (min(tile, 64), rb, (64 // tile) * rb)
+    finally:
+        hbs.set_block_tuning(0, 0)
+    X = dX.download().reshape(n, k)
+    Ao = orc.CSR(rp, c, v)
+    loose = []
+    for j in range(k):
+        bj = np.ascontiguousarray(B[:, j])
+        xo, ito, _, _ = orc.bicgstab(Ao, bj, tol=1e-8, maxit=1000)
+        if _check(X[:, j], infos[j], xo, ito, f"tile {tile} rb {rb} column {j}", Ao, bj,
+                  np.ascontiguousarray(Xt[:, j])) != "plain":
+            loose.append(j)
+    assert len(loose) <= MAX_ORDER_SENSITIVE * k
+
+
+def test_block_autotuner_choice(ctx):
+    """The tuner's choice is one of its candidates and fixed per workspace."""
+    rp, c, v = hbs.gen_convdiff2d(64, 64, 0.5)
+    n, k = rp.size - 1, 24
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    S = hbs.Solver(A)
+    dB = ctx.to_device(hbs.gen_uniform(n * k, -1.0, 1.0, 3))
+    dX = ctx.to_device(np.zeros(n * k))
+    S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=1000)
+    tile, rb, rpw = S.block_tuning(k)
+    assert tile in (32, 16, 8) and rb in (1, 2, 4) and rpw == (64 // tile) * rb
+    dX.zero()
+    S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=1000)
+    assert S.block_tuning(k) == (tile, rb, rpw)
+    with pytest.raises(hbs.HBSError):
+        S.block_tuning(k + 1)
+
+
 def test_block_fixed_iterations_golden(ctx):
     rp, c, v = sample320()
     b = hbs.vector_read(golden("sample320", "B.txt"), 320)
