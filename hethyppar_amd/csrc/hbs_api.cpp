@@ -28,6 +28,7 @@ int g_xseg = 2;           // x-segmented SpMV: 0 never, 1 every one-system matri
 int g_xseg_split = 1;     // k_xseg workgroups per x segment
 int g_spmm_order = 1;     // multi-column SpMM on a 3-D grid: 1 y-slab runs (plan_grid_runs), 0 rows in order
 int g_spmm_dynamic = 1;   // 64-column SpMM sweeps: 1 steps from per-XCD counters, 0 fixed stride
+int g_block3 = 1;         // 3x3-block matrices: kind-3 row blocks (1) or plain CSR-stream blocks (0)
 }  // namespace
 
 #define fail ::hbs::set_error
@@ -331,6 +332,27 @@ extern "C" int hbs_set_spmv_pass(int entries_per_thread)
     return HBS_OK;
 }
 
+extern "C" int hbs_set_spmv_block3(int mode)
+{
+    if (mode != 0 && mode != 1) return fail(HBS_ERR_ARG, "mode must be 0 or 1");
+    g_block3 = mode;
+    return HBS_OK;
+}
+
+extern "C" int hbs_detect_block3(int m, int n, const int *rowptr, const int *col, int *yes)
+{
+    if (m < 0 || n < 0 || !rowptr || !col || !yes) return fail(HBS_ERR_ARG, "bad argument");
+    *yes = detect_b3(m, n, rowptr, col) ? 1 : 0;
+    return HBS_OK;
+}
+
+extern "C" int hbs_csr_block3(hbs_csr a, int *on)
+{
+    if (!a || !on) return fail(HBS_ERR_ARG, "bad argument");
+    *on = a->bval != nullptr;
+    return HBS_OK;
+}
+
 extern "C" int hbs_set_spmm_order(int mode)
 {
     if (mode != 0 && mode != 1) return fail(HBS_ERR_ARG, "mode must be 0 (rows in order) or 1 (grid runs)");
@@ -413,6 +435,10 @@ extern "C" int hbs_csr_destroy(hbs_csr a)
     dfree(a->xs_pos);
     dfree(a->rperm);
     dfree(a->rr_run);
+    dfree(a->bcol);
+    dfree(a->bval);
+    dfree(a->bval32);
+    dfree(a->bsrc);
     dfree(a->spmm_ctr);
     dfree(a->xs_delta);
     dfree(a->xs_val);
@@ -537,17 +563,24 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     // looping over its run of blocks with the next block's loads in flight;
     // 256-thread ones up to kMaxGroups
     int max_groups = g_max_groups;
+    // 3x3-block matrices: row blocks of whole block rows, one column index
+    // per block of 9 entries (the CSR-stream SpMV reads 12 bytes per entry
+    // as 8 + 4/9), with 512-thread workgroups
+    const bool b3 = g_block3 && !xseg && nsys == 1 && spmv_threads(block_nnz) == kWideThreads &&
+                    block_nnz >= 3 * kLongRow && detect_b3(m, n, rowptr, col);
+    if (b3) npt = 4;   // 8 entries per thread spill in the 3x3-block kernel
     if (max_groups == 0) {
         max_groups = kMaxGroups;
         if (spmv_threads(block_nnz) == kWideThreads) {
-            const int res = spmv_resident_groups(block_nnz, npt, ctx->device, xseg);
+            const int res = spmv_resident_groups(block_nnz, npt, ctx->device, xseg, b3);
             if (res > 0) max_groups = std::min(max_groups, res);
         }
     }
     for (int q = 0; q < nsys; ++q) {
         const int r0 = a->sys_rows[(size_t)q], r1 = a->sys_rows[(size_t)q + 1];
         const int rb0 = (int)hp.blocks.size(), g0 = (int)hp.groups.size();
-        plan_rows(r0, r1 - r0, rowptr, block_nnz, g_serial_max_avg, q, hp);
+        if (b3) plan_rows_b3(r0, r1 - r0, rowptr, block_nnz, q, hp);
+        else plan_rows(r0, r1 - r0, rowptr, block_nnz, g_serial_max_avg, q, hp);
         const long long nz = (long long)rowptr[r1] - rowptr[r0];
         const int want = nsys == 1 ? max_groups
                                    : (int)std::max(1LL, nnz > 0 ? (long long)max_groups * nz / nnz : 1LL);
@@ -619,6 +652,22 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
             if (hipMemset(a->xs_prod32 + nnz, 0, sizeof(float) * kRowPad) != hipSuccess)
                 return bail(fail(HBS_ERR_HIP, "x-segment buffers"));
             if (launch_to_f32(nnz, a->xs_val, a->xs_val32, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
+                return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
+        }
+    }
+    if (b3) {
+        std::vector<int> bsrc, bcol;
+        b3_layout(m, rowptr, col, bsrc, bcol);
+        std::vector<double> bv((size_t)nnz);
+        for (int j = 0; j < nnz; ++j) bv[(size_t)j] = val[bsrc[(size_t)j]];
+        if ((rc = dalloc(&a->bcol, bcol.size())) || (rc = dalloc(&a->bval, (size_t)nnz)) ||
+            (rc = dalloc(&a->bsrc, (size_t)nnz)) || (rc = upload(a->bcol, bcol)) || (rc = upload(a->bval, bv)) ||
+            (rc = upload(a->bsrc, bsrc)))
+            return bail(rc);
+        if (a->val32) {
+            if ((rc = dalloc(&a->bval32, (size_t)nnz))) return bail(rc);
+            if (launch_to_f32(nnz, a->bval, a->bval32, ctx->stream) != hipSuccess ||
                 hipStreamSynchronize(ctx->stream) != hipSuccess)
                 return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
         }
@@ -761,6 +810,8 @@ extern "C" int hbs_csr_set_values(hbs_csr a, const double *val_dev)
         if (t->sval32) HIPCHK(launch_to_f32(t->nnz, t->sval, t->sval32, q));
         if (t->xs_val) HIPCHK(launch_gather_f64(t->nnz, t->val, t->xs_src, t->xs_val, q));
         if (t->xs_val32) HIPCHK(launch_to_f32(t->nnz, t->xs_val, t->xs_val32, q));
+        if (t->bval) HIPCHK(launch_gather_f64(t->nnz, t->val, t->bsrc, t->bval, q));
+        if (t->bval32) HIPCHK(launch_to_f32(t->nnz, t->bval, t->bval32, q));
     }
     return HBS_OK;
 }

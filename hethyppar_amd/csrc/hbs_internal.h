@@ -146,6 +146,8 @@ struct SpmvArgs {
     const T *prod;              // x-segmented matrices: the products of k_xseg, row-block major
     const unsigned short *ppos; //   and each one's position in its row block (CSR order)
     const unsigned short *rperm;    // x-segmented matrices: serial row-sum order (plan_row_order)
+    const int *bcol;            // kind-3 (3x3-block) blocks: each block's first column
+    const T *bval;              //   and the values block-major (b3_layout)
 };
 
 // x-segmented SpMV, product pass (k_xseg): workgroup s stages the x segment
@@ -289,6 +291,17 @@ struct HostPlan {
 // order's.  Returns the number of such blocks; rperm covers every row
 // (identity elsewhere).
 int plan_row_order(const int *rowptr, HostPlan &plan, int m, std::vector<unsigned short> &rperm);
+// 3x3-block matrices: every block row (rows 3R..3R+2) has three rows of the
+// same entries' columns, in aligned triples 3J, 3J+1, 3J+2, at most kLongRow
+// per row.  Their row blocks (kind 3, BlockAux{3, 0, K, 1}: whole block rows
+// of K blocks each) read the values block-major -- block row, block, row in
+// the block, column -- with one column per block, keep the products in that
+// order in LDS, and sum each row over its triples in CSR order.  b3_layout gives
+// the CSR entry of each block-major value (bsrc) and each block's first
+// column (bcol, entry base / 9 + k of block row base).
+bool detect_b3(int m, int n, const int *rowptr, const int *col);
+void plan_rows_b3(int row0, int m, const int *rowptr, int block_nnz, int sys, HostPlan &plan);
+void b3_layout(int m, const int *rowptr, const int *col, std::vector<int> &bsrc, std::vector<int> &bcol);
 // Row blocks of rows [row0, row0 + m) of system sys, appended to the plan.
 void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max_avg, int sys,
                HostPlan &plan);
@@ -343,7 +356,7 @@ int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 hipError_t prepare_spmv(int block_nnz, int npt);
 // FP64 SpMV workgroups of row blocks of block_nnz entries (npt per thread per
 // pass) resident at once on the device (0 when the query fails).
-int spmv_resident_groups(int block_nnz, int npt, int device, bool pre = false);
+int spmv_resident_groups(int block_nnz, int npt, int device, bool pre = false, bool b3 = false);
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, int npt, hipStream_t s);
 hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, int npt, hipStream_t s);
 hipError_t launch_chunks(const ChunkArgs<double> &a, int block_nnz, hipStream_t s);

@@ -111,7 +111,7 @@ __device__ __forceinline__ void issue_rowptr(const SpmvArgs<T> &a, const RowBloc
 // for a block without entries) and are ignored later.  Loads that always
 // issue let the compiler wait for exactly the ones it needs; loads behind a
 // branch make it wait for every load in flight, the next pass's included.
-template <typename T, int NPT, int TH, bool PRE>
+template <typename T, int NPT, int TH, bool PRE, bool B3>
 __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, int base,
                                            PassLoad<T, NPT> &L)
 {
@@ -126,6 +126,20 @@ __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock 
             L.pos[i] = pp[k];
         }
         issue_rowptr<T, NPT, TH>(a, rb, base, L);
+        return;
+    }
+    if constexpr (B3) {
+        // 3x3 blocks (a B3 kernel sees no other kind): values block-major, one column per block of 9 (the
+        // block row starts at a multiple of 9 entries), entry e's column is
+        // its block's first column + e % 3
+        const T *__restrict__ bvp = a.bval + rb.j0 + base;
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const int k = min((int)threadIdx.x + i * TH, hi);
+            const int e = base + k;
+            L.c[i] = a.bcol[(rb.j0 + e) / 9] + e % 3;
+            L.v[i] = bvp[k];
+        }
         return;
     }
     const bool sorted = ax.kind == 2;
@@ -188,19 +202,45 @@ __device__ __forceinline__ T sum_lds(const T *lds, int j0, int e, int step)
     return acc;
 }
 
+// Row t of a kind-3 block of kb-block rows, its products block-major in
+// LDS: row a of block row rb3 = t / 3 holds the triples at rb3 * 9kb +
+// 9k + 3a, k = 0 .. kb - 1.  Added in CSR order (k, then the triple's
+// columns) from +0.0, so the sum equals the serial CSR sum; 4 triples are
+// loaded per round.
+template <typename T>
+__device__ __forceinline__ T sum_b3(const T *lds, int t, int kb)
+{
+    constexpr int U = 4;
+    const int rb3 = t / 3, ra = t - rb3 * 3;
+    const T *p = lds + rb3 * 9 * kb + 3 * ra;
+    T acc = T(0);
+    for (int k0 = 0; k0 < kb; k0 += U) {
+        T v[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[u][c] = k0 + u < kb ? p[9 * (k0 + u) + c] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc += v[u][c];
+    }
+    return acc;
+}
+
 // Sums the rows of a block whose products (lds) and row offsets (lrp) are in
 // LDS and feeds the epilogue.  Serial blocks add each row's products in CSR
 // order from 0 (the reference's order): thread t sums rows t, t + TH, ...,
 // with their epilogue operands loaded together (the first one, w0, was
 // prefetched).  Other blocks use `lanes` lanes per row and an xor butterfly.
-template <typename T, int EPI, int TH, bool PRE>
+template <typename T, int EPI, int TH, bool PRE, bool B3>
 __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, T w0,
                                          const T *lds, const unsigned short *lrp, const int (&rq)[4], Acc<T> &d0,
                                          Acc<T> &d1)
 {
     constexpr int B = PRE ? 8 : 2;
     const int nr = rb.r1 - rb.r0;
-    const int lanes = ax.lanes;
+    const int lanes = B3 ? 1 : ax.lanes;   // kind-3 blocks are serial
     if (PRE && lanes <= 1 && ax.first) {
         // rows in the block's rperm order (longest first): slot t + q * TH
         // sums row rq[q], in CSR order as ever; its epilogue operand and
@@ -232,10 +272,19 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
         // reads makes the next row wait for every memory operation in flight
         // (the next pass's loads among them)
         T acc[kRowsPerThread];
+        if constexpr (B3) {
+            // one row at a time: its triples' loads are the batch
+#pragma unroll 1
+            for (int q = 0; q < kRowsPerThread; ++q) {
+                const int t = (int)threadIdx.x + q * TH;
+                acc[q] = t < nr ? sum_b3<T>(lds, t, ax.nchunks) : T(0);
+            }
+        } else {
 #pragma unroll
-        for (int q = 0; q < kRowsPerThread; ++q) {
-            const int t = (int)threadIdx.x + q * TH;
-            acc[q] = t < nr ? sum_lds<T, B>(lds, lrp[t], lrp[t + 1], 1) : T(0);
+            for (int q = 0; q < kRowsPerThread; ++q) {
+                const int t = (int)threadIdx.x + q * TH;
+                acc[q] = t < nr ? sum_lds<T, B>(lds, lrp[t], lrp[t + 1], 1) : T(0);
+            }
         }
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
@@ -279,8 +328,8 @@ struct Cursor {
 // rest of the state prologue, so they overlap it.  PRE (x-segmented
 // matrices): the passes load the products k_xseg formed, with their
 // positions, instead of gathering x.
-template <typename T, int EPI, int NPT, int TH, bool PRE>
-__global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_spmv(SpmvArgs<T> a)
+template <typename T, int EPI, int NPT, int TH, bool PRE, bool B3 = false>
+__global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(B3 ? 6 : 4))) void k_spmv(SpmvArgs<T> a)
 {
     constexpr int NW = TH / 64;
     constexpr int PASS = NPT * TH;
@@ -299,7 +348,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     Cursor cur{g.rb0, 0, a.blocks[g.rb0], a.aux[g.rb0]};
     PassLoad<T, NPT> LA, LB;
-    if (cur.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, cur.rb, cur.ax, 0, LA);
+    if (cur.ax.kind != 1) issue_pass<T, NPT, TH, PRE, B3>(a, cur.rb, cur.ax, 0, LA);
     T w0 = issue_w0<T, EPI>(a, cur.rb, cur.ax);
 
     // ||s|| test: the system's first workgroup applies it to the state it
@@ -322,11 +371,11 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // pass's go to `out`.  Returns false after the workgroup's last block.
     auto step = [&](const PassLoad<T, NPT> &in, PassLoad<T, NPT> &out) -> bool {
         const int nz = cur.rb.j1 - cur.rb.j0;
-        const bool gather = cur.ax.kind != 1;
+        const bool gather = B3 || cur.ax.kind != 1;   // a B3 kernel sees no long rows
         const bool last = !gather || cur.base + PASS >= nz;
         T xv[NPT];
         // kind-2 blocks with packed positions: column in the low pos_shift bits
-        const bool packed = cur.ax.kind == 2 && a.pos_shift;
+        const bool packed = !B3 && cur.ax.kind == 2 && a.pos_shift;
         const unsigned cmask = packed ? (1u << a.pos_shift) - 1u : ~0u;
         if (gather && !PRE) {
             // every lane gathers (past the block's end, at a column of the
@@ -367,17 +416,19 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         const bool more = nxt.b < g.rb1;
         if constexpr (!PRE) {
-            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, nxt.rb, nxt.ax, nxt.base, out);
+            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE, B3>(a, nxt.rb, nxt.ax, nxt.base, out);
         }
         if (gather) {
             const int lim = nz - cur.base;
-            const bool sorted = cur.ax.kind == 2;
+            const bool sorted = !B3 && cur.ax.kind == 2;
 #pragma unroll
             for (int i = 0; i < NPT; ++i) {
                 const int k = (int)threadIdx.x + i * TH;
                 if constexpr (PRE) {
                     if (k < lim) lds[in.pos[i]] = in.v[i];
                 } else {
+                    // kind 3 keeps its block-major order in LDS: the row sums
+                    // read each row's run of triples (sum_b3)
                     const int at = !sorted ? cur.base + k
                                            : packed ? (int)((unsigned)in.c[i] >> a.pos_shift) : (int)in.pos[i];
                     if (k < lim) lds[at] = in.v[i] * xv[i];
@@ -395,7 +446,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // pass's products and row pointers are in LDS, still ahead of the
         // row sums
         if constexpr (PRE) {
-            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, nxt.rb, nxt.ax, nxt.base, out);
+            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE, B3>(a, nxt.rb, nxt.ax, nxt.base, out);
         }
         if (last) {
             if (!gather) {
@@ -416,7 +467,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
             } else {
                 __syncthreads();
-                sum_rows<T, EPI, TH, PRE>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
+                sum_rows<T, EPI, TH, PRE, B3>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
                 __syncthreads();   // LDS is reused by the next row block
             }
             if (more) w0 = issue_w0<T, EPI>(a, nxt.rb, nxt.ax);
@@ -897,16 +948,16 @@ __global__ void k_state_init(SolverState *st, int nstates, double tol, int maxit
     }
 }
 
-template <typename T, int NPT, int TH, bool PRE = false>
+template <typename T, int NPT, int TH, bool PRE = false, bool B3 = false>
 hipError_t spmv_epi(const SpmvArgs<T> &a, int epi, size_t shmem, hipStream_t s)
 {
     switch (epi) {
-    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESIDB: k_spmv<T, EPI_RESIDB, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESIDB: k_spmv<T, EPI_RESIDB, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -928,14 +979,16 @@ hipError_t raise_lds_limit(const void *f, size_t bytes)
 }
 
 // Allow the dynamic LDS of the wide blocks.
-template <typename T, int NPT, int TH, bool PRE = false>
+template <typename T, int NPT, int TH, bool PRE = false, bool B3 = false>
 hipError_t spmv_allow_lds(size_t shmem)
 {
     hipError_t e = hipSuccess;
-    const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_DOT1, NPT, TH, PRE>,
-                       (const void *)k_spmv<T, EPI_DOT2, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_RESID, NPT, TH, PRE>,
-                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH, PRE>,
-                       (const void *)k_spmv<T, EPI_RESIDB, NPT, TH, PRE>};
+    const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH, PRE, B3>,
+                       (const void *)k_spmv<T, EPI_DOT1, NPT, TH, PRE, B3>,
+                       (const void *)k_spmv<T, EPI_DOT2, NPT, TH, PRE, B3>,
+                       (const void *)k_spmv<T, EPI_RESID, NPT, TH, PRE, B3>,
+                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH, PRE, B3>,
+                       (const void *)k_spmv<T, EPI_RESIDB, NPT, TH, PRE, B3>};
     for (const void *f : k)
         if (e == hipSuccess) e = raise_lds_limit(f, shmem);
     return e;
@@ -955,9 +1008,16 @@ hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, int npt, 
         if (spmv_threads(block_nnz) != kWideThreads || npt != kPreNpt) return hipErrorInvalidValue;
         return spmv_epi<T, kPreNpt, kWideThreads, true>(a, epi, prod + rowptr_lds_bytes<kWideThreads>(), s);
     }
+    if (a.bval && spmv_threads(block_nnz) != kWideThreads) return hipErrorInvalidValue;
     if (block_nnz <= 4 * kThreads) return spmv_epi<T, 4, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
     if (block_nnz <= 8 * kThreads) return spmv_epi<T, 8, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
     const size_t shmem = prod + rowptr_lds_bytes<kWideThreads>();
+    if (a.bval) {
+        // 3x3-block matrices: their own instantiation, so the register
+        // cost of the triple sums stays out of the plain kernels
+        if (npt == 4) return spmv_epi<T, 4, kWideThreads, false, true>(a, epi, shmem, s);
+        return spmv_epi<T, 8, kWideThreads, false, true>(a, epi, shmem, s);
+    }
     if (npt == 4) return spmv_epi<T, 4, kWideThreads>(a, epi, shmem, s);
     return spmv_epi<T, 8, kWideThreads>(a, epi, shmem, s);
 }
@@ -1004,13 +1064,17 @@ hipError_t prepare_spmv(int block_nnz, int npt)
     } else if (npt == 4) {
         e = spmv_allow_lds<double, 4, kWideThreads>(d);
         if (e == hipSuccess) e = spmv_allow_lds<float, 4, kWideThreads>(f);
+        if (e == hipSuccess) e = spmv_allow_lds<double, 4, kWideThreads, false, true>(d);
+        if (e == hipSuccess) e = spmv_allow_lds<float, 4, kWideThreads, false, true>(f);
     } else {
         e = spmv_allow_lds<double, 8, kWideThreads>(d);
         if (e == hipSuccess) e = spmv_allow_lds<float, 8, kWideThreads>(f);
+        if (e == hipSuccess) e = spmv_allow_lds<double, 8, kWideThreads, false, true>(d);
+        if (e == hipSuccess) e = spmv_allow_lds<float, 8, kWideThreads, false, true>(f);
     }
     return e;
 }
-int spmv_resident_groups(int block_nnz, int npt, int device, bool pre)
+int spmv_resident_groups(int block_nnz, int npt, int device, bool pre, bool b3)
 {
     const int th = spmv_threads(block_nnz);
     const size_t shmem = (size_t)block_nnz * sizeof(double) +
@@ -1018,6 +1082,8 @@ int spmv_resident_groups(int block_nnz, int npt, int device, bool pre)
     const void *f = th == kThreads ? (block_nnz <= 4 * kThreads ? (const void *)k_spmv<double, EPI_DOT2, 4, kThreads, false>
                                                                  : (const void *)k_spmv<double, EPI_DOT2, 8, kThreads, false>)
                     : pre          ? (const void *)k_spmv<double, EPI_DOT2, kPreNpt, kWideThreads, true>
+                    : b3           ? (npt == 4 ? (const void *)k_spmv<double, EPI_DOT2, 4, kWideThreads, false, true>
+                                               : (const void *)k_spmv<double, EPI_DOT2, 8, kWideThreads, false, true>)
                                    : (npt == 4 ? (const void *)k_spmv<double, EPI_DOT2, 4, kWideThreads, false>
                                                : (const void *)k_spmv<double, EPI_DOT2, 8, kWideThreads, false>);
     int per_cu = 0, cus = 0;

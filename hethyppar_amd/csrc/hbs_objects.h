@@ -96,6 +96,12 @@ struct hbs_csr_s {
     unsigned short *sperm = nullptr;  // position in the block of each sorted entry (pos_shift 0)
     int pos_shift = 0;                // > 0: scol = col | position << pos_shift, no sperm
     int *ssrc = nullptr;              // CSR entry of each sorted position (refreshes sval)
+    // 3x3-block matrices (detect_b3): values block-major and each block's
+    // first column, read by the kind-3 row blocks instead of col / val
+    int *bcol = nullptr;
+    double *bval = nullptr;
+    float *bval32 = nullptr;
+    int *bsrc = nullptr;              // CSR entry of each block-major value (refreshes bval)
     // x-segmented SpMV (columns scattered over all of x): k_xseg forms the
     // products per x segment, k_spmv<PRE> sums them (layout in XsegArgs)
     int xs_nseg = 0;                  // 0: CSR-stream SpMV
@@ -212,6 +218,9 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
         s.ppos = a->xs_pos;
         s.rperm = a->rperm;
     }
+    s.bcol = a->bcol;
+    if constexpr (sizeof(T) == 8) s.bval = reinterpret_cast<const T *>(a->bval);
+    else s.bval = reinterpret_cast<const T *>(a->bval32);
     s.part_out = a->part;
     s.stream_nt = a->stream_nt;
     s.alpha = T(1);

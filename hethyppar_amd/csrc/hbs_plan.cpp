@@ -73,6 +73,59 @@ void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max
     }
 }
 
+bool detect_b3(int m, int n, const int *rowptr, const int *col)
+{
+    if (m <= 0 || m % 3 != 0 || n % 3 != 0) return false;
+    for (int r = 0; r < m; r += 3) {
+        const int j0 = rowptr[r], len = rowptr[r + 1] - j0;
+        if (len <= 0 || len % 3 != 0 || len > kLongRow) return false;
+        for (int a = 1; a < 3; ++a)
+            if (rowptr[r + a + 1] - rowptr[r + a] != len) return false;
+        for (int k = 0; k < len; k += 3) {
+            const int c0 = col[j0 + k];
+            if (c0 % 3 != 0 || col[j0 + k + 1] != c0 + 1 || col[j0 + k + 2] != c0 + 2) return false;
+        }
+        for (int a = 1; a < 3; ++a)
+            for (int t = 0; t < len; ++t)
+                if (col[rowptr[r + a] + t] != col[j0 + t]) return false;
+    }
+    return true;
+}
+
+void plan_rows_b3(int row0, int m, const int *rowptr, int block_nnz, int sys, HostPlan &plan)
+{
+    const int end = row0 + m;
+    const int max_rows = max_rows_per_block(spmv_threads(block_nnz));
+    int r = row0;
+    while (r < end) {
+        const int r0 = r, j0 = rowptr[r], len = rowptr[r + 1] - rowptr[r];
+        do {
+            r += 3;
+        } while (r < end && rowptr[r + 3] - j0 <= block_nnz && r + 3 - r0 <= max_rows &&
+                 rowptr[r + 1] - rowptr[r] == len);
+        plan.max_row = std::max(plan.max_row, len);
+        plan.serial_rows += r - r0;
+        plan.blocks.push_back(RowBlock{r0, r, j0, rowptr[r]});
+        plan.aux.push_back(BlockAux{3, 0, len / 3, 1});
+    }
+    (void)sys;
+}
+
+void b3_layout(int m, const int *rowptr, const int *col, std::vector<int> &bsrc, std::vector<int> &bcol)
+{
+    const int nnz = rowptr[m];
+    bsrc.resize((size_t)nnz);
+    bcol.resize((size_t)nnz / 9);
+    for (int r = 0; r < m; r += 3) {
+        const int base = rowptr[r], kb = (rowptr[r + 1] - base) / 3;
+        for (int k = 0; k < kb; ++k) {
+            bcol[(size_t)base / 9 + k] = col[base + 3 * k];
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) bsrc[(size_t)base + 9 * k + 3 * a + c] = rowptr[r + a] + 3 * k + c;
+        }
+    }
+}
+
 int plan_row_order(const int *rowptr, HostPlan &plan, int m, std::vector<unsigned short> &rperm)
 {
     rperm.assign((size_t)std::max(m, 1), 0);

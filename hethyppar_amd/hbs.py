@@ -148,6 +148,9 @@ _PROTOS = {
     "hbs_set_spmv_xseg": (_i, [_i]),
     "hbs_set_xseg_split": (_i, [_i]),
     "hbs_set_spmm_order": (_i, [_i]),
+    "hbs_set_spmv_block3": (_i, [_i]),
+    "hbs_csr_block3": (_i, [_vp, _ip]),
+    "hbs_detect_block3": (_i, [_i, _i, _ip, _ip, _ip]),
     "hbs_set_spmm_tuning": (_i, [_i, _i, _i, _i]),
     "hbs_set_block_tuning": (_i, [_i, _i]),
     "hbs_block_tuning": (_i, [_vp, _i, _ip, _ip, _ip]),
@@ -166,11 +169,21 @@ _lib = None
 
 
 def lib():
-    """Load libhbs.so once; raise if it has not been built."""
+    """Load libhbs.so once; raise if it has not been built.
+
+    torch, where installed, is loaded first: it brings its own copy of the
+    HIP runtime, and libhbs.so binds to that copy only if it is already in
+    the process (the loader matches it by soname).  Loaded the other way
+    round, the process holds two runtimes, and whichever opens the GPU
+    second finds no device."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m hethyppar_amd.build`")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _PROTOS.items():
             f = getattr(L, name)
@@ -620,6 +633,9 @@ class DeviceCSR:
         ns = C.c_int()
         check(lib().hbs_csr_xseg(self.h, C.byref(ns)))
         d["xseg"] = ns.value
+        b3 = C.c_int()
+        check(lib().hbs_csr_block3(self.h, C.byref(b3)))
+        d["block3"] = b3.value
         nr, ln = C.c_int(), C.c_int()
         check(lib().hbs_csr_spmm_runs(self.h, C.byref(nr), C.byref(ln)))
         d["spmm_runs"], d["spmm_run_len"] = nr.value, ln.value
@@ -801,6 +817,23 @@ def set_spmv_plan(block_nnz=0, serial_max_avg=64, max_groups=0, stream_nt=0, npt
 
 def set_xseg_split(split=1):
     check(lib().hbs_set_xseg_split(split))
+
+
+def set_spmv_block3(mode=1):
+    """Process-wide: 3x3-block streams for 3x3-block matrices created
+    afterwards (1) or the plain CSR streams (0)."""
+    check(lib().hbs_set_spmv_block3(mode))
+
+
+def detect_block3(rowptr, col, n=None):
+    """Whether a host CSR is a 3x3-block matrix (kind-3 streams)."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+    c = np.ascontiguousarray(col, dtype=np.int32)
+    m = rp.size - 1
+    y = C.c_int()
+    check(lib().hbs_detect_block3(m, m if n is None else n, rp.ctypes.data_as(_ip), c.ctypes.data_as(_ip),
+                                  C.byref(y)))
+    return bool(y.value)
 
 
 def set_spmm_order(mode=1):

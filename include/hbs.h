@@ -414,6 +414,17 @@ int hbs_set_xseg_split(int split);
 /* Segments of an x-segmented device CSR (0: CSR-stream SpMV). */
 int hbs_csr_xseg(hbs_csr a, int *nseg);
 
+/* 3x3-block matrices (every block row's three rows share one list of
+ * aligned column triples, at most 256 entries per row; one system, CSR-stream
+ * SpMV) of device CSRs created afterwards: 1 (default) reads the values
+ * block-major with one column index per 3x3 block, 0 the plain CSR
+ * streams.  Each row is summed in CSR order either way. */
+int hbs_set_spmv_block3(int mode);
+/* Whether a device CSR uses the 3x3-block streams; whether a host CSR
+ * (m x n) is such a 3x3-block matrix. */
+int hbs_csr_block3(hbs_csr a, int *on);
+int hbs_detect_block3(int m, int n, const int *rowptr, const int *col, int *yes);
+
 /* Row traversal of the multi-column SpMM (hbs_csr_spmm_*, the block solve)
  * for device CSRs created afterwards: 1 (default) sweeps a square matrix
  * that is a 7-point operator on a 3-D grid -- offsets 0, +-1, +-nx, +-nx*ny

@@ -279,3 +279,75 @@ def test_set_values_updates_every_copy(ctx):
     A.spmv(dx32, dy32)
     ref = orc.spmv(orc.CSR(rp, c, v2), x.astype(np.float32).astype(np.float64))
     assert np.allclose(dy32.download(), ref, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("block_nnz", [2304, 4608, 9216, 16384])
+def test_spmv_block3_bit_exact(ctx, block_nnz):
+    """3x3-block matrices (FEM-like: 12 blocks per block row, one far) read
+    their values block-major with one column per block (512-thread row
+    blocks, above 2048 entries): every row equals the oracle's serial CSR
+    sum, FP32 within rounding, for every row block size; the same matrix
+    with the plain streams gives the same y."""
+    rp, c, v = hbs.gen_fem(4001, 12, 1.1, 9)
+    n = rp.size - 1
+    x = np.random.default_rng(block_nnz).uniform(-1, 1, n)
+    ref = orc.spmv(orc.CSR(rp, c, v), x)
+    hbs.set_spmv_plan(block_nnz=block_nnz)
+    try:
+        A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32 | hbs.HBS_CSR_TRANSPOSE)
+        hbs.set_spmv_block3(0)
+        try:
+            B = hbs.DeviceCSR(ctx, rp, c, v)
+        finally:
+            hbs.set_spmv_block3(1)
+    finally:
+        hbs.set_spmv_plan()
+    assert A.info()["block3"] == 1 and B.info()["block3"] == 0
+    dx, dy, dz = ctx.to_device(x), ctx.empty(n), ctx.empty(n)
+    A.spmv(dx, dy)
+    B.spmv(dx, dz)
+    assert np.array_equal(dy.download(), ref)
+    assert np.array_equal(dz.download(), ref)
+    dy.upload(np.full(n, 0.5))
+    A.spmv(dx, dy, alpha=2.0, beta=-1.0)
+    assert np.array_equal(dy.download(), 2.0 * ref - 0.5)
+    A.spmv_t(dx, dy)
+    tp, tc, tv = orc.csr_to_csc(orc.CSR(rp, c, v))
+    assert np.array_equal(dy.download(), orc.spmv(orc.CSR(tp, tc, tv), x))
+    dx32, dy32 = ctx.to_device(x.astype(np.float32)), ctx.empty(n, np.float32)
+    A.spmv(dx32, dy32)
+    ref32 = orc.spmv(orc.CSR(rp, c, v), x.astype(np.float32).astype(np.float64))
+    assert np.allclose(dy32.download(), ref32, rtol=1e-5, atol=1e-5 * np.abs(ref32).max())
+    # new values reach the block-major copies (FP64, FP32, transpose)
+    v2 = v * 0.75 - 0.125
+    A.set_values(ctx.to_device(v2))
+    A.spmv(dx, dy)
+    assert np.array_equal(dy.download(), orc.spmv(orc.CSR(rp, c, v2), x))
+    A.spmv_t(dx, dy)
+    tp, tc, tv = orc.csr_to_csc(orc.CSR(rp, c, v2))
+    assert np.array_equal(dy.download(), orc.spmv(orc.CSR(tp, tc, tv), x))
+    A.spmv(dx32, dy32)
+    ref32 = orc.spmv(orc.CSR(rp, c, v2), x.astype(np.float32).astype(np.float64))
+    assert np.allclose(dy32.download(), ref32, rtol=1e-5, atol=1e-5 * np.abs(ref32).max())
+
+
+def test_block3_solve_matches_oracle(ctx):
+    """The FP64 solve on a 3x3-block matrix (kind-3 row blocks) meets the
+    single-system bar."""
+    from parity import check
+    rp, c, v = hbs.gen_fem(6000, 12, 1.1, 4)
+    n = rp.size - 1
+    xt = hbs.gen_uniform(n, -1.0, 1.0, 8)
+    b = orc.spmv(orc.CSR(rp, c, v), xt)
+    hbs.set_spmv_plan(block_nnz=4608)
+    try:
+        A = hbs.DeviceCSR(ctx, rp, c, v)
+    finally:
+        hbs.set_spmv_plan()
+    assert A.info()["block3"] == 1
+    S = hbs.Solver(A)
+    db, dx = ctx.to_device(b), ctx.empty(n)
+    dx.zero()
+    info = S.bicgstab(db, dx, tol=1e-8, maxit=2000)
+    xo, ito, _, _ = orc.bicgstab(orc.CSR(rp, c, v), b, tol=1e-8, maxit=2000)
+    check(dx.download(), info, xo, ito, "fem 18k block3", orc.CSR(rp, c, v), b, xt)

@@ -280,3 +280,24 @@ def test_spmm_grid_runs_plan():
                     hbs.gen_markov(20000, 4, 8, 500, 1.05, 3)[:2]):
         r2, _ = hbs.plan_grid_runs(rp2, c2)
         assert r2.size == 0
+
+
+def test_block3_detection():
+    """3x3-block matrices (the FEM generator's) are recognised; a matrix
+    that breaks the pattern anywhere -- a row of another length, a column
+    triple out of line, a row of the block row with other columns -- is
+    not."""
+    rp, c, v = hbs.gen_fem(40, 6, 1.1, 3)
+    assert hbs.detect_block3(rp, c)
+    c2 = c.copy()
+    c2[rp[10] + 4] += 1                      # a triple out of line
+    assert not hbs.detect_block3(rp, c2)
+    c3 = c.copy()
+    r = 3 * 7 + 1                            # second row of block row 7
+    c3[rp[r]:rp[r] + 3] = c3[rp[r] + 3:rp[r] + 6]
+    c3[rp[r] + 3:rp[r] + 6] = c[rp[r]:rp[r] + 3]
+    assert not hbs.detect_block3(rp, c3)
+    grp, gc, _ = hbs.gen_convdiff2d(9, 9, 0.5)
+    assert not hbs.detect_block3(grp, gc)
+    assert hbs.detect_block3(rp[:-3], c[:rp[-4]], n=rp.size - 1)       # whole block rows dropped: still 3x3
+    assert not hbs.detect_block3(rp[:-1], c[:rp[-2]], n=rp.size - 1)   # a row count not a multiple of 3
