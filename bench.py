@@ -288,7 +288,7 @@ def probe(hbs, which):
     s.zero()
     cms = C.c_double()
     hbs.check(hbs.lib().hbs_bench_copy(ctx.h, s.ptr, d.ptr, cn, 3, C.byref(cms)))
-    print(json.dumps({"probe": which, "n": n, "nnz": nnz, "xseg": A.info()["xseg"],
+    print(json.dumps({"probe": which, "n": n, "nnz": nnz, "xseg": A.info()["xseg"], "block3": A.info()["block3"],
                       "spmv_fp32_ms": ms32, "spmv_fp32_bytes": spmv_bytes(n, n, nnz, 4),
                       "spmv_fp64_ms": ms64, "spmv_fp64_bytes": spmv_bytes(n, n, nnz, 8),
                       "copy_doubles": cn, "copy_ms": cms.value}), flush=True)

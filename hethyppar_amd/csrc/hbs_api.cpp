@@ -545,7 +545,7 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     bool xseg = (nsys <= 1 || !sys_rows) && nnz > 0 &&
                 (g_xseg == 1 || (g_xseg == 2 && (long long)n * 8 >= (8LL << 20) &&
                                  column_scatter(col, nnz, kSegBlockNnz) > 0.5));
-    const int block_nnz =
+    int block_nnz =
         xseg ? (g_block_nnz > 2048 ? g_block_nnz : kSegBlockNnz)
              : g_block_nnz > 0 ? g_block_nnz : default_block_nnz(m, nnz, spmv_resident_groups(6144, 4, ctx->device));
     if (!sys_rows) {
@@ -555,31 +555,33 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
         a->sys_rows.assign(sys_rows, sys_rows + nsys + 1);
     }
     a->nsys = nsys;
-    int npt = spmv_threads(block_nnz) == kThreads ? (block_nnz <= 4 * kThreads ? 4 : 8)
-                                                  : (xseg ? kPreNpt : g_spmv_npt ? g_spmv_npt : 4);
+    // 3x3-block matrices (k_spmv3): row blocks of whole block rows, at most
+    // block_nnz entries and one block per thread, one column index per block
+    // of 9 entries (the stream reads 8 + 4/9 bytes per entry instead of 12)
+    const bool b3 = g_block3 && !xseg && nsys == 1 && spmv_threads(block_nnz) == kWideThreads &&
+                    block_nnz >= 3 * kLongRow && detect_b3(m, n, rowptr, col);
+    const int group_nnz = block_nnz;   // row blocks' entry limit
+    if (b3) block_nnz = kB3BlockNnz;   // the launches' LDS
+    int npt = b3 ? kB3Npt
+                 : spmv_threads(block_nnz) == kThreads ? (block_nnz <= 4 * kThreads ? 4 : 8)
+                                                       : (xseg ? kPreNpt : g_spmv_npt ? g_spmv_npt : 4);
     if (prepare_spmv(block_nnz, npt) != hipSuccess)
         return bail(fail(HBS_ERR_HIP, "SpMV LDS limit for %d-entry row blocks not granted", block_nnz));
     // SpMV workgroups: 512-thread ones as many as are resident at once, each
     // looping over its run of blocks with the next block's loads in flight;
     // 256-thread ones up to kMaxGroups
     int max_groups = g_max_groups;
-    // 3x3-block matrices: row blocks of whole block rows, one column index
-    // per block of 9 entries (the CSR-stream SpMV reads 12 bytes per entry
-    // as 8 + 4/9), with 512-thread workgroups
-    const bool b3 = g_block3 && !xseg && nsys == 1 && spmv_threads(block_nnz) == kWideThreads &&
-                    block_nnz >= 3 * kLongRow && detect_b3(m, n, rowptr, col);
-    if (b3) npt = 4;   // 8 entries per thread spill in the 3x3-block kernel
     if (max_groups == 0) {
         max_groups = kMaxGroups;
         if (spmv_threads(block_nnz) == kWideThreads) {
-            const int res = spmv_resident_groups(block_nnz, npt, ctx->device, xseg, b3);
+            const int res = spmv_resident_groups(block_nnz, npt, ctx->device, xseg);
             if (res > 0) max_groups = std::min(max_groups, res);
         }
     }
     for (int q = 0; q < nsys; ++q) {
         const int r0 = a->sys_rows[(size_t)q], r1 = a->sys_rows[(size_t)q + 1];
         const int rb0 = (int)hp.blocks.size(), g0 = (int)hp.groups.size();
-        if (b3) plan_rows_b3(r0, r1 - r0, rowptr, block_nnz, q, hp);
+        if (b3) plan_rows_b3(r0, r1 - r0, rowptr, group_nnz, q, hp);
         else plan_rows(r0, r1 - r0, rowptr, block_nnz, g_serial_max_avg, q, hp);
         const long long nz = (long long)rowptr[r1] - rowptr[r0];
         const int want = nsys == 1 ? max_groups
@@ -657,17 +659,20 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
         }
     }
     if (b3) {
+        // k_spmv3's threads past a row block's blocks read up to kWideThreads
+        // values and columns past it: padding of zero values and column 0
         std::vector<int> bsrc, bcol;
-        b3_layout(m, rowptr, col, bsrc, bcol);
-        std::vector<double> bv((size_t)nnz);
+        b3_layout(hp, rowptr, col, bsrc, bcol);
+        std::vector<double> bv((size_t)nnz + kWideThreads, 0.0);
         for (int j = 0; j < nnz; ++j) bv[(size_t)j] = val[bsrc[(size_t)j]];
-        if ((rc = dalloc(&a->bcol, bcol.size())) || (rc = dalloc(&a->bval, (size_t)nnz)) ||
+        bcol.resize(bcol.size() + kWideThreads, 0);
+        if ((rc = dalloc(&a->bcol, bcol.size())) || (rc = dalloc(&a->bval, bv.size())) ||
             (rc = dalloc(&a->bsrc, (size_t)nnz)) || (rc = upload(a->bcol, bcol)) || (rc = upload(a->bval, bv)) ||
             (rc = upload(a->bsrc, bsrc)))
             return bail(rc);
         if (a->val32) {
-            if ((rc = dalloc(&a->bval32, (size_t)nnz))) return bail(rc);
-            if (launch_to_f32(nnz, a->bval, a->bval32, ctx->stream) != hipSuccess ||
+            if ((rc = dalloc(&a->bval32, bv.size()))) return bail(rc);
+            if (launch_to_f32((int)bv.size(), a->bval, a->bval32, ctx->stream) != hipSuccess ||
                 hipStreamSynchronize(ctx->stream) != hipSuccess)
                 return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
         }

@@ -294,14 +294,18 @@ int plan_row_order(const int *rowptr, HostPlan &plan, int m, std::vector<unsigne
 // 3x3-block matrices: every block row (rows 3R..3R+2) has three rows of the
 // same entries' columns, in aligned triples 3J, 3J+1, 3J+2, at most kLongRow
 // per row.  Their row blocks (kind 3, BlockAux{3, 0, K, 1}: whole block rows
-// of K blocks each) read the values block-major -- block row, block, row in
-// the block, column -- with one column per block, keep the products in that
-// order in LDS, and sum each row over its triples in CSR order.  b3_layout gives
-// the CSR entry of each block-major value (bsrc) and each block's first
-// column (bcol, entry base / 9 + k of block row base).
+// of K blocks each, at most kWideThreads blocks and block_nnz entries) are
+// summed by k_spmv3, one block per thread.  A row block's nbk blocks are
+// numbered in CSR order (block row, then block); its values are stored in 9
+// planes of nbk, plane 3a + c holding entry (a, c) of every block, and each
+// block's first column once (bcol, at the row block's first entry / 9 + its
+// number).  b3_layout gives the CSR entry of each stored value (bsrc, nnz
+// entries) and bcol (nnz / 9 entries); the device copies carry
+// kWideThreads entries of padding (zero values, column 0) past the end.
 bool detect_b3(int m, int n, const int *rowptr, const int *col);
 void plan_rows_b3(int row0, int m, const int *rowptr, int block_nnz, int sys, HostPlan &plan);
-void b3_layout(int m, const int *rowptr, const int *col, std::vector<int> &bsrc, std::vector<int> &bcol);
+void b3_layout(const HostPlan &plan, const int *rowptr, const int *col, std::vector<int> &bsrc,
+               std::vector<int> &bcol);
 // Row blocks of rows [row0, row0 + m) of system sys, appended to the plan.
 void plan_rows(int row0, int m, const int *rowptr, int block_nnz, int serial_max_avg, int sys,
                HostPlan &plan);
@@ -356,7 +360,7 @@ int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 hipError_t prepare_spmv(int block_nnz, int npt);
 // FP64 SpMV workgroups of row blocks of block_nnz entries (npt per thread per
 // pass) resident at once on the device (0 when the query fails).
-int spmv_resident_groups(int block_nnz, int npt, int device, bool pre = false, bool b3 = false);
+int spmv_resident_groups(int block_nnz, int npt, int device, bool pre = false);
 hipError_t launch_spmv(const SpmvArgs<double> &a, int epi, int block_nnz, int npt, hipStream_t s);
 hipError_t launch_spmv(const SpmvArgs<float> &a, int epi, int block_nnz, int npt, hipStream_t s);
 hipError_t launch_chunks(const ChunkArgs<double> &a, int block_nnz, hipStream_t s);
@@ -370,6 +374,11 @@ hipError_t prepare_xseg(int max_chunks, int max_granules);
 // Entries per thread per pass of the x-segmented row sums (k_spmv<PRE>): one
 // pass covers a row block of kSegBlockNnz entries.
 constexpr int kPreNpt = kSegBlockNnz / kWideThreads;
+// 3x3-block SpMV (k_spmv3): one block of 9 entries per thread, one pass per
+// row block of at most kB3BlockNnz entries, its products in LDS.
+constexpr int kB3Npt = 9;
+constexpr int kB3BlockNnz = kB3Npt * kWideThreads;
+constexpr size_t spmv3_lds_bytes(size_t elem_bytes) { return (size_t)kB3BlockNnz * elem_bytes; }
 // k_xseg entries per thread per pass: as many as the registers allow with
 // two workgroups per CU, so each workgroup keeps more bytes in flight
 template <typename T>

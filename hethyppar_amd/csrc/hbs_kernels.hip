@@ -111,7 +111,7 @@ __device__ __forceinline__ void issue_rowptr(const SpmvArgs<T> &a, const RowBloc
 // for a block without entries) and are ignored later.  Loads that always
 // issue let the compiler wait for exactly the ones it needs; loads behind a
 // branch make it wait for every load in flight, the next pass's included.
-template <typename T, int NPT, int TH, bool PRE, bool B3>
+template <typename T, int NPT, int TH, bool PRE>
 __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, int base,
                                            PassLoad<T, NPT> &L)
 {
@@ -126,20 +126,6 @@ __device__ __forceinline__ void issue_pass(const SpmvArgs<T> &a, const RowBlock 
             L.pos[i] = pp[k];
         }
         issue_rowptr<T, NPT, TH>(a, rb, base, L);
-        return;
-    }
-    if constexpr (B3) {
-        // 3x3 blocks (a B3 kernel sees no other kind): values block-major, one column per block of 9 (the
-        // block row starts at a multiple of 9 entries), entry e's column is
-        // its block's first column + e % 3
-        const T *__restrict__ bvp = a.bval + rb.j0 + base;
-#pragma unroll
-        for (int i = 0; i < NPT; ++i) {
-            const int k = min((int)threadIdx.x + i * TH, hi);
-            const int e = base + k;
-            L.c[i] = a.bcol[(rb.j0 + e) / 9] + e % 3;
-            L.v[i] = bvp[k];
-        }
         return;
     }
     const bool sorted = ax.kind == 2;
@@ -216,10 +202,14 @@ __device__ __forceinline__ T sum_b3(const T *lds, int t, int kb)
     T acc = T(0);
     for (int k0 = 0; k0 < kb; k0 += U) {
         T v[U][3];
+        // past the row's last triple: read that triple again, add +0.0
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[u][c] = k0 + u < kb ? p[9 * (k0 + u) + c] : T(0);
+            for (int c = 0; c < 3; ++c) {
+                const T r = p[9 * min(k0 + u, kb - 1) + c];
+                v[u][c] = k0 + u < kb ? r : T(0);
+            }
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -261,6 +251,22 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
         }
         return;
     }
+    if constexpr (B3) {
+        // row t in straight-line code (no loop join ahead of its epilogue
+        // operand, prefetched as w0); rows past TH (block rows of fewer
+        // than 3 blocks) one at a time after it
+        const int t = (int)threadIdx.x;
+        const T acc = sum_b3<T>(lds, min(t, nr - 1), ax.nchunks);
+        if (t < nr) row_out<T, EPI>(a, rb.r0 + t, acc, w0, d0, d1);
+#pragma unroll 1
+        for (int q = 1; q < kRowsPerThread && q * TH < nr; ++q) {
+            const int tq = t + q * TH;
+            const T wr = reads_w<EPI>() ? a.w[rb.r0 + min(tq, nr - 1)] : T(0);
+            const T aq = sum_b3<T>(lds, min(tq, nr - 1), ax.nchunks);
+            if (tq < nr) row_out<T, EPI>(a, rb.r0 + tq, aq, wr, d0, d1);
+        }
+        return;
+    }
     if (lanes <= 1) {
         T wr[kRowsPerThread];
 #pragma unroll
@@ -272,19 +278,10 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
         // reads makes the next row wait for every memory operation in flight
         // (the next pass's loads among them)
         T acc[kRowsPerThread];
-        if constexpr (B3) {
-            // one row at a time: its triples' loads are the batch
-#pragma unroll 1
-            for (int q = 0; q < kRowsPerThread; ++q) {
-                const int t = (int)threadIdx.x + q * TH;
-                acc[q] = t < nr ? sum_b3<T>(lds, t, ax.nchunks) : T(0);
-            }
-        } else {
 #pragma unroll
-            for (int q = 0; q < kRowsPerThread; ++q) {
-                const int t = (int)threadIdx.x + q * TH;
-                acc[q] = t < nr ? sum_lds<T, B>(lds, lrp[t], lrp[t + 1], 1) : T(0);
-            }
+        for (int q = 0; q < kRowsPerThread; ++q) {
+            const int t = (int)threadIdx.x + q * TH;
+            acc[q] = t < nr ? sum_lds<T, B>(lds, lrp[t], lrp[t + 1], 1) : T(0);
         }
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
@@ -317,43 +314,25 @@ struct Cursor {
     BlockAux ax;
 };
 
-// TH threads, NPT entries per thread per pass: a row block of up to
-// block_nnz entries takes ceil(block_nnz / (NPT * TH)) passes, each storing
-// its products in LDS, before the row sums.  The passes of a workgroup's
-// blocks form one stream, software-pipelined over two register sets: the
-// stream loads of pass i + 1 are issued right after the x gathers of pass i,
-// so they are in flight while pass i's products are stored and, after a
-// block's last pass, while its rows are summed.  A finished system returns
-// before any load; otherwise the first pass's loads are issued before the
-// rest of the state prologue, so they overlap it.  PRE (x-segmented
-// matrices): the passes load the products k_xseg formed, with their
-// positions, instead of gathering x.
-template <typename T, int EPI, int NPT, int TH, bool PRE, bool B3 = false>
-__global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(B3 ? 6 : 4))) void k_spmv(SpmvArgs<T> a)
+// A finished system's workgroups return before any load; its first one
+// hands the state on.
+template <typename T>
+__device__ __forceinline__ bool spmv_skip(const SpmvArgs<T> &a, const Group &g, int p)
 {
-    constexpr int NW = TH / 64;
-    constexpr int PASS = NPT * TH;
-    extern __shared__ __align__(16) unsigned char smem[];
-    unsigned short *lrp = reinterpret_cast<unsigned short *>(smem);
-    T *lds = reinterpret_cast<T *>(smem + rowptr_lds_bytes<TH>());
-    __shared__ double red[kMaxDots * NW];
-    __shared__ SolverState sh;
-    constexpr int ND = EpiDots<EPI>::n;
-
-    const int p = xcd_remap(blockIdx.x, a.ngroups);
-    const Group g = a.groups[p];
     if (a.st_in && a.check_done && a.st_in[g.sys].done) {
         if (a.st_out && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_out[g.sys] = a.st_in[g.sys];
-        return;
+        return true;
     }
-    Cursor cur{g.rb0, 0, a.blocks[g.rb0], a.aux[g.rb0]};
-    PassLoad<T, NPT> LA, LB;
-    if (cur.ax.kind != 1) issue_pass<T, NPT, TH, PRE, B3>(a, cur.rb, cur.ax, 0, LA);
-    T w0 = issue_w0<T, EPI>(a, cur.rb, cur.ax);
+    return false;
+}
 
-    // ||s|| test: the system's first workgroup applies it to the state it
-    // hands on; the product is formed either way (when the test fires, the
-    // consumer ignores it), so no other workgroup waits on the reduction.
+// ||s|| test: the system's first workgroup applies it to the state it hands
+// on; the product is formed either way (when the test fires, the consumer
+// ignores it), so no other workgroup waits on the reduction.  Then the
+// pending mark.
+template <typename T, int NW>
+__device__ __forceinline__ void spmv_state(const SpmvArgs<T> &a, const Group &g, int p, double *red, SolverState &sh)
+{
     if (a.st_in && a.fin_in == FIN_S && p == a.sys[g.sys].g0) {
         const SysRange sr = a.sys[g.sys];
         if (threadIdx.x == 0) sh = a.st_in[g.sys];
@@ -365,17 +344,67 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(B3 ? 6 : 4))
         }
     }
     if (a.st_mark && p == a.sys[g.sys].g0 && threadIdx.x == 0) a.st_mark[g.sys].pending = 1;
+}
+
+// The workgroup's partials of the fused dots, one per dot.
+template <typename T, int EPI, int NW>
+__device__ __forceinline__ void spmv_parts(const SpmvArgs<T> &a, int p, const Acc<T> &d0, const Acc<T> &d1,
+                                           double *red)
+{
+    constexpr int ND = EpiDots<EPI>::n;
+    if constexpr (ND > 0) {
+        double v[ND];
+        v[0] = d0.get();
+        if constexpr (ND > 1) v[1] = d1.get();
+        block_sum<ND, NW>(v, red);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 0; q < ND; ++q) a.part_out[(size_t)q * a.ngroups + p] = v[q];
+        }
+    }
+}
+
+// TH threads, NPT entries per thread per pass: a row block of up to
+// block_nnz entries takes ceil(block_nnz / (NPT * TH)) passes, each storing
+// its products in LDS, before the row sums.  The passes of a workgroup's
+// blocks form one stream, software-pipelined over two register sets: the
+// stream loads of pass i + 1 are issued right after the x gathers of pass i,
+// so they are in flight while pass i's products are stored and, after a
+// block's last pass, while its rows are summed.  A finished system returns
+// before any load; otherwise the first pass's loads are issued before the
+// rest of the state prologue, so they overlap it.  PRE (x-segmented
+// matrices): the passes load the products k_xseg formed, with their
+// positions, instead of gathering x.
+template <typename T, int EPI, int NPT, int TH, bool PRE>
+__global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_spmv(SpmvArgs<T> a)
+{
+    constexpr int NW = TH / 64;
+    constexpr int PASS = NPT * TH;
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned short *lrp = reinterpret_cast<unsigned short *>(smem);
+    T *lds = reinterpret_cast<T *>(smem + rowptr_lds_bytes<TH>());
+    __shared__ double red[kMaxDots * NW];
+    __shared__ SolverState sh;
+
+    const int p = xcd_remap(blockIdx.x, a.ngroups);
+    const Group g = a.groups[p];
+    if (spmv_skip(a, g, p)) return;
+    Cursor cur{g.rb0, 0, a.blocks[g.rb0], a.aux[g.rb0]};
+    PassLoad<T, NPT> LA, LB;
+    if (cur.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, cur.rb, cur.ax, 0, LA);
+    T w0 = issue_w0<T, EPI>(a, cur.rb, cur.ax);
+    spmv_state<T, NW>(a, g, p, red, sh);
 
     Acc<T> d0, d1;
     // One pass of the stream: the current pass's loads are in `in`; the next
     // pass's go to `out`.  Returns false after the workgroup's last block.
     auto step = [&](const PassLoad<T, NPT> &in, PassLoad<T, NPT> &out) -> bool {
         const int nz = cur.rb.j1 - cur.rb.j0;
-        const bool gather = B3 || cur.ax.kind != 1;   // a B3 kernel sees no long rows
+        const bool gather = cur.ax.kind != 1;
         const bool last = !gather || cur.base + PASS >= nz;
         T xv[NPT];
         // kind-2 blocks with packed positions: column in the low pos_shift bits
-        const bool packed = !B3 && cur.ax.kind == 2 && a.pos_shift;
+        const bool packed = cur.ax.kind == 2 && a.pos_shift;
         const unsigned cmask = packed ? (1u << a.pos_shift) - 1u : ~0u;
         if (gather && !PRE) {
             // every lane gathers (past the block's end, at a column of the
@@ -416,19 +445,17 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(B3 ? 6 : 4))
         }
         const bool more = nxt.b < g.rb1;
         if constexpr (!PRE) {
-            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE, B3>(a, nxt.rb, nxt.ax, nxt.base, out);
+            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, nxt.rb, nxt.ax, nxt.base, out);
         }
         if (gather) {
             const int lim = nz - cur.base;
-            const bool sorted = !B3 && cur.ax.kind == 2;
+            const bool sorted = cur.ax.kind == 2;
 #pragma unroll
             for (int i = 0; i < NPT; ++i) {
                 const int k = (int)threadIdx.x + i * TH;
                 if constexpr (PRE) {
                     if (k < lim) lds[in.pos[i]] = in.v[i];
                 } else {
-                    // kind 3 keeps its block-major order in LDS: the row sums
-                    // read each row's run of triples (sum_b3)
                     const int at = !sorted ? cur.base + k
                                            : packed ? (int)((unsigned)in.c[i] >> a.pos_shift) : (int)in.pos[i];
                     if (k < lim) lds[at] = in.v[i] * xv[i];
@@ -446,7 +473,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(B3 ? 6 : 4))
         // pass's products and row pointers are in LDS, still ahead of the
         // row sums
         if constexpr (PRE) {
-            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE, B3>(a, nxt.rb, nxt.ax, nxt.base, out);
+            if (more && nxt.ax.kind != 1) issue_pass<T, NPT, TH, PRE>(a, nxt.rb, nxt.ax, nxt.base, out);
         }
         if (last) {
             if (!gather) {
@@ -467,7 +494,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(B3 ? 6 : 4))
                 }
             } else {
                 __syncthreads();
-                sum_rows<T, EPI, TH, PRE, B3>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
+                sum_rows<T, EPI, TH, PRE, false>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
                 __syncthreads();   // LDS is reused by the next row block
             }
             if (more) w0 = issue_w0<T, EPI>(a, nxt.rb, nxt.ax);
@@ -484,17 +511,135 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(B3 ? 6 : 4))
             if (!step(LB, LA)) break;
         }
     }
+    spmv_parts<T, EPI, NW>(a, p, d0, d1, red);
+}
 
-    if constexpr (ND > 0) {
-        double v[ND];
-        v[0] = d0.get();
-        if constexpr (ND > 1) v[1] = d1.get();
-        block_sum<ND, NW>(v, red);
-        if (threadIdx.x == 0) {
+// A row block's descriptors loaded by vector loads, made uniform for the
+// scalar address arithmetic that uses them.
+__device__ __forceinline__ RowBlock uniform_rb(RowBlock r)
+{
+    r.r0 = __builtin_amdgcn_readfirstlane(r.r0);
+    r.r1 = __builtin_amdgcn_readfirstlane(r.r1);
+    r.j0 = __builtin_amdgcn_readfirstlane(r.j0);
+    r.j1 = __builtin_amdgcn_readfirstlane(r.j1);
+    return r;
+}
+__device__ __forceinline__ BlockAux uniform_ax(BlockAux x)
+{
+    x.kind = __builtin_amdgcn_readfirstlane(x.kind);
+    x.lanes = __builtin_amdgcn_readfirstlane(x.lanes);
+    x.nchunks = __builtin_amdgcn_readfirstlane(x.nchunks);
+    x.first = __builtin_amdgcn_readfirstlane(x.first);
+    return x;
+}
+
+// Workgroup barrier ordering LDS accesses only.  __syncthreads' release
+// fence makes a wave wait for its global stores, and on CDNA stores and
+// loads share one counter, so it waits for every load in flight too.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One 3x3 block of a kind-3 row block per thread: its first column and its
+// 9 values, v[3a + c] = row a, column c of the block.
+template <typename T>
+struct B3Load {
+    int bc;
+    T v[9];
+};
+
+// The loads of row block rb's 3x3 blocks: thread t takes its block t, whose
+// values sit in the row block's 9 planes of nbk values (plane 3a + c holds
+// entry (a, c) of every block), so each load instruction reads 64
+// consecutive values.  Threads past the row block's nbk blocks read the
+// next ones (the arrays are padded by kWideThreads entries past the last
+// block) and are ignored later: no load is clamped or behind a branch.
+template <typename T>
+__device__ __forceinline__ void issue_b3(const SpmvArgs<T> &a, const RowBlock &rb, B3Load<T> &L)
+{
+    const int nbk = (rb.j1 - rb.j0) / 9;
+    const int *__restrict__ cb = a.bcol + rb.j0 / 9;
+    L.bc = cb[threadIdx.x];
 #pragma unroll
-            for (int q = 0; q < ND; ++q) a.part_out[(size_t)q * a.ngroups + p] = v[q];
-        }
+    for (int q = 0; q < 9; ++q) {
+        const T *__restrict__ plane = a.bval + rb.j0 + q * nbk;
+        L.v[q] = plane[threadIdx.x];
     }
+}
+
+// SpMV of a matrix of 3x3 blocks (kind-3 row blocks of at most TH blocks,
+// one pass each), bit-identical to k_spmv's serial rows: thread t forms the
+// 9 products of block t (its three x values gathered once) and stores them
+// block-major in LDS, where the rows are summed in CSR order (sum_b3).  The
+// stream reads 8 + 4/9 bytes per entry, and per 9 entries a thread issues
+// 13 loads (1 column, 9 values, 3 x values) from uniform bases.  The loads
+// of the workgroup's next row block are issued right after this one's x
+// gathers, unconditionally (the last row block loads itself again), so no
+// branch join makes the products wait for them: they stay in flight while
+// the products are stored and the rows summed.  The descriptors of the row
+// block after that are loaded then too, a step before their use.
+template <typename T, int EPI>
+__global__ __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_spmv3(SpmvArgs<T> a)
+{
+    constexpr int TH = kWideThreads, NW = TH / 64;
+    extern __shared__ __align__(16) unsigned char smem[];
+    // 9 * TH products: every thread stores its block's, past nbk too
+    T *lds = reinterpret_cast<T *>(smem);
+    __shared__ double red[kMaxDots * NW];
+    __shared__ SolverState sh;
+
+    const int p = xcd_remap(blockIdx.x, a.ngroups);
+    const Group g = a.groups[p];
+    if (spmv_skip(a, g, p)) return;
+    Cursor cur{g.rb0, 0, a.blocks[g.rb0], a.aux[g.rb0]};
+    B3Load<T> LA, LB;
+    issue_b3<T>(a, cur.rb, LA);
+    // descriptors of row block min(cur.b + 1, g.rb1 - 1)
+    RowBlock prb = a.blocks[min(g.rb0 + 1, g.rb1 - 1)];
+    BlockAux pax = a.aux[min(g.rb0 + 1, g.rb1 - 1)];
+    T w0 = issue_w0<T, EPI>(a, cur.rb, cur.ax);
+    spmv_state<T, NW>(a, g, p, red, sh);
+
+    Acc<T> d0, d1;
+    const int rq[kRowsPerThread] = {0, 0, 0, 0};
+    auto step = [&](const B3Load<T> &in, B3Load<T> &out) -> bool {
+        // the block's three x values: 32-bit byte offset from the uniform
+        // base (n * sizeof(T) < 2^32), the columns as immediate offsets
+        const T *__restrict__ xp =
+            reinterpret_cast<const T *>(reinterpret_cast<const char *>(a.x) + (unsigned)in.bc * (unsigned)sizeof(T));
+        const T x0 = xp[0], x1 = xp[1], x2 = xp[2];
+        const bool more = cur.b + 1 < g.rb1;
+        const int nb = more ? cur.b + 1 : cur.b;
+        const RowBlock nrb = uniform_rb(prb);
+        const BlockAux nax = uniform_ax(pax);
+        issue_b3<T>(a, nrb, out);
+        prb = a.blocks[min(nb + 1, g.rb1 - 1)];
+        pax = a.aux[min(nb + 1, g.rb1 - 1)];
+        T *q = lds + 9 * (int)threadIdx.x;
+        q[0] = in.v[0] * x0;
+        q[1] = in.v[1] * x1;
+        q[2] = in.v[2] * x2;
+        q[3] = in.v[3] * x0;
+        q[4] = in.v[4] * x1;
+        q[5] = in.v[5] * x2;
+        q[6] = in.v[6] * x0;
+        q[7] = in.v[7] * x1;
+        q[8] = in.v[8] * x2;
+        lds_barrier();
+        sum_rows<T, EPI, TH, false, true>(a, cur.rb, cur.ax, w0, lds, nullptr, rq, d0, d1);
+        lds_barrier();   // LDS is reused by the next row block
+        w0 = issue_w0<T, EPI>(a, nrb, nax);
+        cur = Cursor{nb, 0, nrb, nax};
+        return more;
+    };
+    for (;;) {
+        if (!step(LA, LB)) break;
+        if (!step(LB, LA)) break;
+    }
+    spmv_parts<T, EPI, NW>(a, p, d0, d1, red);
 }
 
 // One workgroup per chunk of a long row: ordered tree sum of its products
@@ -948,16 +1093,32 @@ __global__ void k_state_init(SolverState *st, int nstates, double tol, int maxit
     }
 }
 
-template <typename T, int NPT, int TH, bool PRE = false, bool B3 = false>
+template <typename T, int NPT, int TH, bool PRE = false>
 hipError_t spmv_epi(const SpmvArgs<T> &a, int epi, size_t shmem, hipStream_t s)
 {
     switch (epi) {
-    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESIDB: k_spmv<T, EPI_RESIDB, NPT, TH, PRE, B3><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESIDB: k_spmv<T, EPI_RESIDB, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t spmv3_epi(const SpmvArgs<T> &a, int epi, hipStream_t s)
+{
+    const size_t shmem = spmv3_lds_bytes(sizeof(T));
+    switch (epi) {
+    case EPI_AXPBY: k_spmv3<T, EPI_AXPBY><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_DOT1: k_spmv3<T, EPI_DOT1><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_DOT2: k_spmv3<T, EPI_DOT2><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_RESID: k_spmv3<T, EPI_RESID><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_RESID2: k_spmv3<T, EPI_RESID2><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_RESIDB: k_spmv3<T, EPI_RESIDB><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -979,18 +1140,28 @@ hipError_t raise_lds_limit(const void *f, size_t bytes)
 }
 
 // Allow the dynamic LDS of the wide blocks.
-template <typename T, int NPT, int TH, bool PRE = false, bool B3 = false>
+template <typename T, int NPT, int TH, bool PRE = false>
 hipError_t spmv_allow_lds(size_t shmem)
 {
     hipError_t e = hipSuccess;
-    const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH, PRE, B3>,
-                       (const void *)k_spmv<T, EPI_DOT1, NPT, TH, PRE, B3>,
-                       (const void *)k_spmv<T, EPI_DOT2, NPT, TH, PRE, B3>,
-                       (const void *)k_spmv<T, EPI_RESID, NPT, TH, PRE, B3>,
-                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH, PRE, B3>,
-                       (const void *)k_spmv<T, EPI_RESIDB, NPT, TH, PRE, B3>};
+    const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_DOT1, NPT, TH, PRE>,
+                       (const void *)k_spmv<T, EPI_DOT2, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_RESID, NPT, TH, PRE>,
+                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH, PRE>,
+                       (const void *)k_spmv<T, EPI_RESIDB, NPT, TH, PRE>};
     for (const void *f : k)
         if (e == hipSuccess) e = raise_lds_limit(f, shmem);
+    return e;
+}
+
+template <typename T>
+hipError_t spmv3_allow_lds()
+{
+    hipError_t e = hipSuccess;
+    const void *k[] = {(const void *)k_spmv3<T, EPI_AXPBY>, (const void *)k_spmv3<T, EPI_DOT1>,
+                       (const void *)k_spmv3<T, EPI_DOT2>,  (const void *)k_spmv3<T, EPI_RESID>,
+                       (const void *)k_spmv3<T, EPI_RESID2>, (const void *)k_spmv3<T, EPI_RESIDB>};
+    for (const void *f : k)
+        if (e == hipSuccess) e = raise_lds_limit(f, spmv3_lds_bytes(sizeof(T)));
     return e;
 }
 
@@ -1008,16 +1179,14 @@ hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, int npt, 
         if (spmv_threads(block_nnz) != kWideThreads || npt != kPreNpt) return hipErrorInvalidValue;
         return spmv_epi<T, kPreNpt, kWideThreads, true>(a, epi, prod + rowptr_lds_bytes<kWideThreads>(), s);
     }
-    if (a.bval && spmv_threads(block_nnz) != kWideThreads) return hipErrorInvalidValue;
+    if (a.bval) {
+        // 3x3-block matrices: row blocks of at most kWideThreads blocks
+        if (block_nnz != kB3BlockNnz || npt != kB3Npt) return hipErrorInvalidValue;
+        return spmv3_epi<T>(a, epi, s);
+    }
     if (block_nnz <= 4 * kThreads) return spmv_epi<T, 4, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
     if (block_nnz <= 8 * kThreads) return spmv_epi<T, 8, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
     const size_t shmem = prod + rowptr_lds_bytes<kWideThreads>();
-    if (a.bval) {
-        // 3x3-block matrices: their own instantiation, so the register
-        // cost of the triple sums stays out of the plain kernels
-        if (npt == 4) return spmv_epi<T, 4, kWideThreads, false, true>(a, epi, shmem, s);
-        return spmv_epi<T, 8, kWideThreads, false, true>(a, epi, shmem, s);
-    }
     if (npt == 4) return spmv_epi<T, 4, kWideThreads>(a, epi, shmem, s);
     return spmv_epi<T, 8, kWideThreads>(a, epi, shmem, s);
 }
@@ -1061,29 +1230,29 @@ hipError_t prepare_spmv(int block_nnz, int npt)
     if (npt == kPreNpt) {
         e = spmv_allow_lds<double, kPreNpt, kWideThreads, true>(d);
         if (e == hipSuccess) e = spmv_allow_lds<float, kPreNpt, kWideThreads, true>(f);
+    } else if (npt == kB3Npt) {
+        e = spmv3_allow_lds<double>();
+        if (e == hipSuccess) e = spmv3_allow_lds<float>();
     } else if (npt == 4) {
         e = spmv_allow_lds<double, 4, kWideThreads>(d);
         if (e == hipSuccess) e = spmv_allow_lds<float, 4, kWideThreads>(f);
-        if (e == hipSuccess) e = spmv_allow_lds<double, 4, kWideThreads, false, true>(d);
-        if (e == hipSuccess) e = spmv_allow_lds<float, 4, kWideThreads, false, true>(f);
     } else {
         e = spmv_allow_lds<double, 8, kWideThreads>(d);
         if (e == hipSuccess) e = spmv_allow_lds<float, 8, kWideThreads>(f);
-        if (e == hipSuccess) e = spmv_allow_lds<double, 8, kWideThreads, false, true>(d);
-        if (e == hipSuccess) e = spmv_allow_lds<float, 8, kWideThreads, false, true>(f);
     }
     return e;
 }
-int spmv_resident_groups(int block_nnz, int npt, int device, bool pre, bool b3)
+int spmv_resident_groups(int block_nnz, int npt, int device, bool pre)
 {
     const int th = spmv_threads(block_nnz);
-    const size_t shmem = (size_t)block_nnz * sizeof(double) +
-                         (th == kWideThreads ? rowptr_lds_bytes<kWideThreads>() : rowptr_lds_bytes<kThreads>());
+    const bool b3 = npt == kB3Npt;
+    const size_t shmem = b3 ? spmv3_lds_bytes(sizeof(double))
+                            : (size_t)block_nnz * sizeof(double) +
+                                  (th == kWideThreads ? rowptr_lds_bytes<kWideThreads>() : rowptr_lds_bytes<kThreads>());
     const void *f = th == kThreads ? (block_nnz <= 4 * kThreads ? (const void *)k_spmv<double, EPI_DOT2, 4, kThreads, false>
                                                                  : (const void *)k_spmv<double, EPI_DOT2, 8, kThreads, false>)
                     : pre          ? (const void *)k_spmv<double, EPI_DOT2, kPreNpt, kWideThreads, true>
-                    : b3           ? (npt == 4 ? (const void *)k_spmv<double, EPI_DOT2, 4, kWideThreads, false, true>
-                                               : (const void *)k_spmv<double, EPI_DOT2, 8, kWideThreads, false, true>)
+                    : b3           ? (const void *)k_spmv3<double, EPI_DOT2>
                                    : (npt == 4 ? (const void *)k_spmv<double, EPI_DOT2, 4, kWideThreads, false>
                                                : (const void *)k_spmv<double, EPI_DOT2, 8, kWideThreads, false>);
     int per_cu = 0, cus = 0;

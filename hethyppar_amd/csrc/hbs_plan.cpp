@@ -95,7 +95,8 @@ bool detect_b3(int m, int n, const int *rowptr, const int *col)
 void plan_rows_b3(int row0, int m, const int *rowptr, int block_nnz, int sys, HostPlan &plan)
 {
     const int end = row0 + m;
-    const int max_rows = max_rows_per_block(spmv_threads(block_nnz));
+    const int max_rows = max_rows_per_block(kWideThreads);
+    block_nnz = std::min(block_nnz, kB3BlockNnz);   // at most one block per thread
     int r = row0;
     while (r < end) {
         const int r0 = r, j0 = rowptr[r], len = rowptr[r + 1] - rowptr[r];
@@ -111,17 +112,24 @@ void plan_rows_b3(int row0, int m, const int *rowptr, int block_nnz, int sys, Ho
     (void)sys;
 }
 
-void b3_layout(int m, const int *rowptr, const int *col, std::vector<int> &bsrc, std::vector<int> &bcol)
+void b3_layout(const HostPlan &plan, const int *rowptr, const int *col, std::vector<int> &bsrc,
+               std::vector<int> &bcol)
 {
-    const int nnz = rowptr[m];
-    bsrc.resize((size_t)nnz);
-    bcol.resize((size_t)nnz / 9);
-    for (int r = 0; r < m; r += 3) {
-        const int base = rowptr[r], kb = (rowptr[r + 1] - base) / 3;
-        for (int k = 0; k < kb; ++k) {
-            bcol[(size_t)base / 9 + k] = col[base + 3 * k];
-            for (int a = 0; a < 3; ++a)
-                for (int c = 0; c < 3; ++c) bsrc[(size_t)base + 9 * k + 3 * a + c] = rowptr[r + a] + 3 * k + c;
+    size_t nnz = 0;
+    for (const RowBlock &rb : plan.blocks) nnz = std::max(nnz, (size_t)rb.j1);
+    bsrc.resize(nnz);
+    bcol.resize(nnz / 9);
+    for (const RowBlock &rb : plan.blocks) {
+        const int nbk = (rb.j1 - rb.j0) / 9;
+        for (int r = rb.r0; r < rb.r1; r += 3) {
+            const int kb = (rowptr[r + 1] - rowptr[r]) / 3;
+            const int b0 = (rowptr[r] - rb.j0) / 9;   // the block row's first block in the row block
+            for (int k = 0; k < kb; ++k) {
+                bcol[(size_t)rb.j0 / 9 + b0 + k] = col[rowptr[r] + 3 * k];
+                for (int a = 0; a < 3; ++a)
+                    for (int c = 0; c < 3; ++c)
+                        bsrc[(size_t)rb.j0 + (size_t)(3 * a + c) * nbk + b0 + k] = rowptr[r + a] + 3 * k + c;
+            }
         }
     }
 }
