@@ -349,7 +349,7 @@ extern "C" int hbs_detect_block3(int m, int n, const int *rowptr, const int *col
 extern "C" int hbs_csr_block3(hbs_csr a, int *on)
 {
     if (!a || !on) return fail(HBS_ERR_ARG, "bad argument");
-    *on = a->bval != nullptr;
+    *on = a->bval == nullptr ? 0 : a->b3var ? 2 : 1;
     return HBS_OK;
 }
 
@@ -581,7 +581,7 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     for (int q = 0; q < nsys; ++q) {
         const int r0 = a->sys_rows[(size_t)q], r1 = a->sys_rows[(size_t)q + 1];
         const int rb0 = (int)hp.blocks.size(), g0 = (int)hp.groups.size();
-        if (b3) plan_rows_b3(r0, r1 - r0, rowptr, group_nnz, q, hp);
+        if (b3) a->b3var += plan_rows_b3(r0, r1 - r0, rowptr, group_nnz, q, hp);
         else plan_rows(r0, r1 - r0, rowptr, block_nnz, g_serial_max_avg, q, hp);
         const long long nz = (long long)rowptr[r1] - rowptr[r0];
         const int want = nsys == 1 ? max_groups

@@ -148,6 +148,7 @@ struct SpmvArgs {
     const unsigned short *rperm;    // x-segmented matrices: serial row-sum order (plan_row_order)
     const int *bcol;            // kind-3 (3x3-block) blocks: each block's first column
     const T *bval;              //   and the values block-major (b3_layout)
+    int b3var;                  //   some kind-3 blocks hold block rows of different lengths
 };
 
 // x-segmented SpMV, product pass (k_xseg): workgroup s stages the x segment
@@ -294,8 +295,10 @@ int plan_row_order(const int *rowptr, HostPlan &plan, int m, std::vector<unsigne
 // 3x3-block matrices: every block row (rows 3R..3R+2) has three rows of the
 // same entries' columns, in aligned triples 3J, 3J+1, 3J+2, at most kLongRow
 // per row.  Their row blocks (kind 3, BlockAux{3, 0, K, 1}: whole block rows
-// of K blocks each, at most kWideThreads blocks and block_nnz entries) are
-// summed by k_spmv3, one block per thread.  A row block's nbk blocks are
+// of K blocks each, at most kWideThreads blocks and block_nnz entries; K = 0
+// when the block rows' lengths differ, which the kernel then reads from the
+// row pointers) are summed by k_spmv3, one block per thread.  plan_rows_b3
+// returns the number of row blocks with K = 0.  A row block's nbk blocks are
 // numbered in CSR order (block row, then block); its values are stored in 9
 // planes of nbk, plane 3a + c holding entry (a, c) of every block, and each
 // block's first column once (bcol, at the row block's first entry / 9 + its
@@ -303,7 +306,7 @@ int plan_row_order(const int *rowptr, HostPlan &plan, int m, std::vector<unsigne
 // entries) and bcol (nnz / 9 entries); the device copies carry
 // kWideThreads entries of padding (zero values, column 0) past the end.
 bool detect_b3(int m, int n, const int *rowptr, const int *col);
-void plan_rows_b3(int row0, int m, const int *rowptr, int block_nnz, int sys, HostPlan &plan);
+int plan_rows_b3(int row0, int m, const int *rowptr, int block_nnz, int sys, HostPlan &plan);
 void b3_layout(const HostPlan &plan, const int *rowptr, const int *col, std::vector<int> &bsrc,
                std::vector<int> &bcol);
 // Row blocks of rows [row0, row0 + m) of system sys, appended to the plan.

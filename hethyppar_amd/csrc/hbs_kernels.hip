@@ -188,17 +188,14 @@ __device__ __forceinline__ T sum_lds(const T *lds, int j0, int e, int step)
     return acc;
 }
 
-// Row t of a kind-3 block of kb-block rows, its products block-major in
-// LDS: row a of block row rb3 = t / 3 holds the triples at rb3 * 9kb +
-// 9k + 3a, k = 0 .. kb - 1.  Added in CSR order (k, then the triple's
-// columns) from +0.0, so the sum equals the serial CSR sum; 4 triples are
-// loaded per round.
+// A row of a kind-3 block, its products block-major in LDS: row a of a block
+// row of kb blocks reads the triples at off + 9k, k = 0 .. kb - 1, added in
+// CSR order (k, then the triple's columns) from +0.0, so the sum equals the
+// serial CSR sum; 4 triples are loaded per round.
 template <typename T>
-__device__ __forceinline__ T sum_b3(const T *lds, int t, int kb)
+__device__ __forceinline__ T sum_b3(const T *p, int kb)
 {
     constexpr int U = 4;
-    const int rb3 = t / 3, ra = t - rb3 * 3;
-    const T *p = lds + rb3 * 9 * kb + 3 * ra;
     T acc = T(0);
     for (int k0 = 0; k0 < kb; k0 += U) {
         T v[U][3];
@@ -218,19 +215,48 @@ __device__ __forceinline__ T sum_b3(const T *lds, int t, int kb)
     return acc;
 }
 
+// Row t of kind-3 row block rb (rb.r0 is a multiple of 3, so row t is row
+// a = t % 3 of its block row): its blocks per row kb and the LDS offset of
+// its first triple.  Block rows of one length K (ax.nchunks): block row t / 3
+// starts at 9K(t / 3).  Otherwise (VAR) from the row's own pointers rp =
+// (rowptr[r], rowptr[r + 1]): kb = (rp.y - rp.x) / 3, and the block row's
+// products start where its entries do, at rp.x - 3a kb - rb.j0.
+template <bool VAR>
+__device__ __forceinline__ void b3_row(const RowBlock &rb, const BlockAux &ax, int t, int2 rp, int &kb, int &off)
+{
+    const int ra = t % 3;
+    if constexpr (VAR) {
+        kb = (rp.y - rp.x) / 3;
+        off = rp.x - rb.j0 + 3 * ra * (1 - kb);
+    } else {
+        kb = ax.nchunks;
+        off = (t / 3) * 9 * kb + 3 * ra;
+    }
+}
+
+// Row pointers of this thread's first row of a kind-3 block (VAR kernels),
+// loaded a row block ahead like the epilogue operand.
+template <bool VAR>
+__device__ __forceinline__ int2 issue_rp(const int *rowptr, const RowBlock &rb)
+{
+    if constexpr (!VAR) return make_int2(0, 0);
+    const int r = rb.r0 + min((int)threadIdx.x, rb.r1 - rb.r0 - 1);
+    return make_int2(rowptr[r], rowptr[r + 1]);
+}
+
 // Sums the rows of a block whose products (lds) and row offsets (lrp) are in
 // LDS and feeds the epilogue.  Serial blocks add each row's products in CSR
 // order from 0 (the reference's order): thread t sums rows t, t + TH, ...,
 // with their epilogue operands loaded together (the first one, w0, was
 // prefetched).  Other blocks use `lanes` lanes per row and an xor butterfly.
-template <typename T, int EPI, int TH, bool PRE, bool B3>
+template <typename T, int EPI, int TH, bool PRE>
 __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, T w0,
                                          const T *lds, const unsigned short *lrp, const int (&rq)[4], Acc<T> &d0,
                                          Acc<T> &d1)
 {
     constexpr int B = PRE ? 8 : 2;
     const int nr = rb.r1 - rb.r0;
-    const int lanes = B3 ? 1 : ax.lanes;   // kind-3 blocks are serial
+    const int lanes = ax.lanes;
     if (PRE && lanes <= 1 && ax.first) {
         // rows in the block's rperm order (longest first): slot t + q * TH
         // sums row rq[q], in CSR order as ever; its epilogue operand and
@@ -248,22 +274,6 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
             if (t < nr) row_out<T, EPI>(a, rb.r0 + rq[q], acc[q], wr[q], d0, d1);
-        }
-        return;
-    }
-    if constexpr (B3) {
-        // row t in straight-line code (no loop join ahead of its epilogue
-        // operand, prefetched as w0); rows past TH (block rows of fewer
-        // than 3 blocks) one at a time after it
-        const int t = (int)threadIdx.x;
-        const T acc = sum_b3<T>(lds, min(t, nr - 1), ax.nchunks);
-        if (t < nr) row_out<T, EPI>(a, rb.r0 + t, acc, w0, d0, d1);
-#pragma unroll 1
-        for (int q = 1; q < kRowsPerThread && q * TH < nr; ++q) {
-            const int tq = t + q * TH;
-            const T wr = reads_w<EPI>() ? a.w[rb.r0 + min(tq, nr - 1)] : T(0);
-            const T aq = sum_b3<T>(lds, min(tq, nr - 1), ax.nchunks);
-            if (tq < nr) row_out<T, EPI>(a, rb.r0 + tq, aq, wr, d0, d1);
         }
         return;
     }
@@ -494,7 +504,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
             } else {
                 __syncthreads();
-                sum_rows<T, EPI, TH, PRE, false>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
+                sum_rows<T, EPI, TH, PRE>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
                 __syncthreads();   // LDS is reused by the next row block
             }
             if (more) w0 = issue_w0<T, EPI>(a, nxt.rb, nxt.ax);
@@ -570,6 +580,31 @@ __device__ __forceinline__ void issue_b3(const SpmvArgs<T> &a, const RowBlock &r
     }
 }
 
+// Sums the rows of a kind-3 block (serial, CSR order) and feeds the
+// epilogue: row t in straight-line code (no loop join ahead of its epilogue
+// operand, prefetched as w0, and its row pointers, rp); rows past TH (block
+// rows of fewer than 3 blocks) one at a time after it.
+template <typename T, int EPI, int TH, bool VAR>
+__device__ __forceinline__ void sum_rows3(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, T w0, int2 rp,
+                                          const T *lds, Acc<T> &d0, Acc<T> &d1)
+{
+    const int nr = rb.r1 - rb.r0;
+    const int t = (int)threadIdx.x;
+    int kb, off;
+    b3_row<VAR>(rb, ax, min(t, nr - 1), rp, kb, off);
+    const T acc = sum_b3<T>(lds + off, kb);
+    if (t < nr) row_out<T, EPI>(a, rb.r0 + t, acc, w0, d0, d1);
+#pragma unroll 1
+    for (int q = 1; q < kRowsPerThread && q * TH < nr; ++q) {
+        const int tq = t + q * TH, tc = min(tq, nr - 1);
+        const T wr = reads_w<EPI>() ? a.w[rb.r0 + tc] : T(0);
+        const int2 rq = VAR ? make_int2(a.rowptr[rb.r0 + tc], a.rowptr[rb.r0 + tc + 1]) : make_int2(0, 0);
+        b3_row<VAR>(rb, ax, tc, rq, kb, off);
+        const T aq = sum_b3<T>(lds + off, kb);
+        if (tq < nr) row_out<T, EPI>(a, rb.r0 + tq, aq, wr, d0, d1);
+    }
+}
+
 // SpMV of a matrix of 3x3 blocks (kind-3 row blocks of at most TH blocks,
 // one pass each), bit-identical to k_spmv's serial rows: thread t forms the
 // 9 products of block t (its three x values gathered once) and stores them
@@ -581,7 +616,7 @@ __device__ __forceinline__ void issue_b3(const SpmvArgs<T> &a, const RowBlock &r
 // branch join makes the products wait for them: they stay in flight while
 // the products are stored and the rows summed.  The descriptors of the row
 // block after that are loaded then too, a step before their use.
-template <typename T, int EPI>
+template <typename T, int EPI, bool VAR>
 __global__ __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_spmv3(SpmvArgs<T> a)
 {
     constexpr int TH = kWideThreads, NW = TH / 64;
@@ -601,10 +636,10 @@ __global__ __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_eu(6)
     RowBlock prb = a.blocks[min(g.rb0 + 1, g.rb1 - 1)];
     BlockAux pax = a.aux[min(g.rb0 + 1, g.rb1 - 1)];
     T w0 = issue_w0<T, EPI>(a, cur.rb, cur.ax);
+    int2 rp = issue_rp<VAR>(a.rowptr, cur.rb);
     spmv_state<T, NW>(a, g, p, red, sh);
 
     Acc<T> d0, d1;
-    const int rq[kRowsPerThread] = {0, 0, 0, 0};
     auto step = [&](const B3Load<T> &in, B3Load<T> &out) -> bool {
         // the block's three x values: 32-bit byte offset from the uniform
         // base (n * sizeof(T) < 2^32), the columns as immediate offsets
@@ -629,9 +664,10 @@ __global__ __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_eu(6)
         q[7] = in.v[7] * x1;
         q[8] = in.v[8] * x2;
         lds_barrier();
-        sum_rows<T, EPI, TH, false, true>(a, cur.rb, cur.ax, w0, lds, nullptr, rq, d0, d1);
+        sum_rows3<T, EPI, TH, VAR>(a, cur.rb, cur.ax, w0, rp, lds, d0, d1);
         lds_barrier();   // LDS is reused by the next row block
         w0 = issue_w0<T, EPI>(a, nrb, nax);
+        rp = issue_rp<VAR>(a.rowptr, nrb);
         cur = Cursor{nb, 0, nrb, nax};
         return more;
     };
@@ -1108,17 +1144,17 @@ hipError_t spmv_epi(const SpmvArgs<T> &a, int epi, size_t shmem, hipStream_t s)
     return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, bool VAR>
 hipError_t spmv3_epi(const SpmvArgs<T> &a, int epi, hipStream_t s)
 {
     const size_t shmem = spmv3_lds_bytes(sizeof(T));
     switch (epi) {
-    case EPI_AXPBY: k_spmv3<T, EPI_AXPBY><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
-    case EPI_DOT1: k_spmv3<T, EPI_DOT1><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
-    case EPI_DOT2: k_spmv3<T, EPI_DOT2><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
-    case EPI_RESID: k_spmv3<T, EPI_RESID><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
-    case EPI_RESID2: k_spmv3<T, EPI_RESID2><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
-    case EPI_RESIDB: k_spmv3<T, EPI_RESIDB><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_AXPBY: k_spmv3<T, EPI_AXPBY, VAR><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_DOT1: k_spmv3<T, EPI_DOT1, VAR><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_DOT2: k_spmv3<T, EPI_DOT2, VAR><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_RESID: k_spmv3<T, EPI_RESID, VAR><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_RESID2: k_spmv3<T, EPI_RESID2, VAR><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
+    case EPI_RESIDB: k_spmv3<T, EPI_RESIDB, VAR><<<a.ngroups, kWideThreads, shmem, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1153,13 +1189,13 @@ hipError_t spmv_allow_lds(size_t shmem)
     return e;
 }
 
-template <typename T>
+template <typename T, bool VAR>
 hipError_t spmv3_allow_lds()
 {
     hipError_t e = hipSuccess;
-    const void *k[] = {(const void *)k_spmv3<T, EPI_AXPBY>, (const void *)k_spmv3<T, EPI_DOT1>,
-                       (const void *)k_spmv3<T, EPI_DOT2>,  (const void *)k_spmv3<T, EPI_RESID>,
-                       (const void *)k_spmv3<T, EPI_RESID2>, (const void *)k_spmv3<T, EPI_RESIDB>};
+    const void *k[] = {(const void *)k_spmv3<T, EPI_AXPBY, VAR>, (const void *)k_spmv3<T, EPI_DOT1, VAR>,
+                       (const void *)k_spmv3<T, EPI_DOT2, VAR>,  (const void *)k_spmv3<T, EPI_RESID, VAR>,
+                       (const void *)k_spmv3<T, EPI_RESID2, VAR>, (const void *)k_spmv3<T, EPI_RESIDB, VAR>};
     for (const void *f : k)
         if (e == hipSuccess) e = raise_lds_limit(f, spmv3_lds_bytes(sizeof(T)));
     return e;
@@ -1182,7 +1218,7 @@ hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, int npt, 
     if (a.bval) {
         // 3x3-block matrices: row blocks of at most kWideThreads blocks
         if (block_nnz != kB3BlockNnz || npt != kB3Npt) return hipErrorInvalidValue;
-        return spmv3_epi<T>(a, epi, s);
+        return a.b3var ? spmv3_epi<T, true>(a, epi, s) : spmv3_epi<T, false>(a, epi, s);
     }
     if (block_nnz <= 4 * kThreads) return spmv_epi<T, 4, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
     if (block_nnz <= 8 * kThreads) return spmv_epi<T, 8, kThreads>(a, epi, prod + rowptr_lds_bytes<kThreads>(), s);
@@ -1231,8 +1267,10 @@ hipError_t prepare_spmv(int block_nnz, int npt)
         e = spmv_allow_lds<double, kPreNpt, kWideThreads, true>(d);
         if (e == hipSuccess) e = spmv_allow_lds<float, kPreNpt, kWideThreads, true>(f);
     } else if (npt == kB3Npt) {
-        e = spmv3_allow_lds<double>();
-        if (e == hipSuccess) e = spmv3_allow_lds<float>();
+        e = spmv3_allow_lds<double, false>();
+        if (e == hipSuccess) e = spmv3_allow_lds<float, false>();
+        if (e == hipSuccess) e = spmv3_allow_lds<double, true>();
+        if (e == hipSuccess) e = spmv3_allow_lds<float, true>();
     } else if (npt == 4) {
         e = spmv_allow_lds<double, 4, kWideThreads>(d);
         if (e == hipSuccess) e = spmv_allow_lds<float, 4, kWideThreads>(f);
@@ -1252,7 +1290,7 @@ int spmv_resident_groups(int block_nnz, int npt, int device, bool pre)
     const void *f = th == kThreads ? (block_nnz <= 4 * kThreads ? (const void *)k_spmv<double, EPI_DOT2, 4, kThreads, false>
                                                                  : (const void *)k_spmv<double, EPI_DOT2, 8, kThreads, false>)
                     : pre          ? (const void *)k_spmv<double, EPI_DOT2, kPreNpt, kWideThreads, true>
-                    : b3           ? (const void *)k_spmv3<double, EPI_DOT2>
+                    : b3           ? (const void *)k_spmv3<double, EPI_DOT2, false>
                                    : (npt == 4 ? (const void *)k_spmv<double, EPI_DOT2, 4, kWideThreads, false>
                                                : (const void *)k_spmv<double, EPI_DOT2, 8, kWideThreads, false>);
     int per_cu = 0, cus = 0;

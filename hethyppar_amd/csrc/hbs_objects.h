@@ -102,6 +102,7 @@ struct hbs_csr_s {
     double *bval = nullptr;
     float *bval32 = nullptr;
     int *bsrc = nullptr;              // CSR entry of each block-major value (refreshes bval)
+    int b3var = 0;                    // some row blocks hold block rows of different lengths
     // x-segmented SpMV (columns scattered over all of x): k_xseg forms the
     // products per x segment, k_spmv<PRE> sums them (layout in XsegArgs)
     int xs_nseg = 0;                  // 0: CSR-stream SpMV
@@ -221,6 +222,7 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
     s.bcol = a->bcol;
     if constexpr (sizeof(T) == 8) s.bval = reinterpret_cast<const T *>(a->bval);
     else s.bval = reinterpret_cast<const T *>(a->bval32);
+    s.b3var = a->b3var;
     s.part_out = a->part;
     s.stream_nt = a->stream_nt;
     s.alpha = T(1);

@@ -92,24 +92,27 @@ bool detect_b3(int m, int n, const int *rowptr, const int *col)
     return true;
 }
 
-void plan_rows_b3(int row0, int m, const int *rowptr, int block_nnz, int sys, HostPlan &plan)
+int plan_rows_b3(int row0, int m, const int *rowptr, int block_nnz, int sys, HostPlan &plan)
 {
     const int end = row0 + m;
     const int max_rows = max_rows_per_block(kWideThreads);
     block_nnz = std::min(block_nnz, kB3BlockNnz);   // at most one block per thread
-    int r = row0;
+    int r = row0, nvar = 0;
     while (r < end) {
         const int r0 = r, j0 = rowptr[r], len = rowptr[r + 1] - rowptr[r];
+        bool same = true;
         do {
+            same = same && rowptr[r + 1] - rowptr[r] == len;
+            plan.max_row = std::max(plan.max_row, rowptr[r + 1] - rowptr[r]);
             r += 3;
-        } while (r < end && rowptr[r + 3] - j0 <= block_nnz && r + 3 - r0 <= max_rows &&
-                 rowptr[r + 1] - rowptr[r] == len);
-        plan.max_row = std::max(plan.max_row, len);
+        } while (r < end && rowptr[r + 3] - j0 <= block_nnz && r + 3 - r0 <= max_rows);
         plan.serial_rows += r - r0;
         plan.blocks.push_back(RowBlock{r0, r, j0, rowptr[r]});
-        plan.aux.push_back(BlockAux{3, 0, len / 3, 1});
+        plan.aux.push_back(BlockAux{3, 0, same ? len / 3 : 0, 1});
+        nvar += !same;
     }
     (void)sys;
+    return nvar;
 }
 
 void b3_layout(const HostPlan &plan, const int *rowptr, const int *col, std::vector<int> &bsrc,

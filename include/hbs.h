@@ -420,8 +420,10 @@ int hbs_csr_xseg(hbs_csr a, int *nseg);
  * block-major with one column index per 3x3 block, 0 the plain CSR
  * streams.  Each row is summed in CSR order either way. */
 int hbs_set_spmv_block3(int mode);
-/* Whether a device CSR uses the 3x3-block streams; whether a host CSR
- * (m x n) is such a 3x3-block matrix. */
+/* Whether a device CSR uses the 3x3-block streams (0 no; 1 yes, every row
+ * block's block rows of one length; 2 yes, some row blocks mix block rows
+ * of different lengths); whether a host CSR (m x n) is such a 3x3-block
+ * matrix. */
 int hbs_csr_block3(hbs_csr a, int *on);
 int hbs_detect_block3(int m, int n, const int *rowptr, const int *col, int *yes);
 

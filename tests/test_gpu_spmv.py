@@ -331,6 +331,40 @@ def test_spmv_block3_bit_exact(ctx, block_nnz):
     assert np.allclose(dy32.download(), ref32, rtol=1e-5, atol=1e-5 * np.abs(ref32).max())
 
 
+@pytest.mark.parametrize("block_nnz", [2304, 4608])
+def test_spmv_block3_mixed_lengths_bit_exact(ctx, block_nnz):
+    """A 3x3-block matrix whose block rows differ in length (the transpose
+    of the FEM-like matrix: its far blocks land on random block rows) keeps
+    whole row blocks of mixed lengths, each row's blocks located through its
+    row pointers: every row equals the oracle's serial CSR sum, with every
+    epilogue's alpha/beta form, and FP32 within rounding."""
+    rp0, c0, v0 = hbs.gen_fem(4001, 12, 1.1, 11)
+    rp, c, v = orc.csr_to_csc(orc.CSR(rp0, c0, v0))
+    rp, c, v = np.asarray(rp, np.int32), np.asarray(c, np.int32), np.asarray(v)
+    n = rp.size - 1
+    lens = np.diff(rp)
+    assert lens.min() < lens.max()                 # block rows of several lengths
+    assert hbs.detect_block3(rp, c)
+    x = np.random.default_rng(block_nnz + 1).uniform(-1, 1, n)
+    ref = orc.spmv(orc.CSR(rp, c, v), x)
+    hbs.set_spmv_plan(block_nnz=block_nnz)
+    try:
+        A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    finally:
+        hbs.set_spmv_plan()
+    assert A.info()["block3"] == 2
+    dx, dy = ctx.to_device(x), ctx.empty(n)
+    A.spmv(dx, dy)
+    assert np.array_equal(dy.download(), ref)
+    dy.upload(np.full(n, 0.5))
+    A.spmv(dx, dy, alpha=2.0, beta=-1.0)
+    assert np.array_equal(dy.download(), 2.0 * ref - 0.5)
+    dx32, dy32 = ctx.to_device(x.astype(np.float32)), ctx.empty(n, np.float32)
+    A.spmv(dx32, dy32)
+    ref32 = orc.spmv(orc.CSR(rp, c, v), x.astype(np.float32).astype(np.float64))
+    assert np.allclose(dy32.download(), ref32, rtol=1e-5, atol=1e-5 * np.abs(ref32).max())
+
+
 def test_block3_solve_matches_oracle(ctx):
     """The FP64 solve on a 3x3-block matrix (kind-3 row blocks) meets the
     single-system bar."""
