@@ -29,6 +29,10 @@ int g_xseg_split = 1;     // k_xseg workgroups per x segment
 int g_spmm_order = 1;     // multi-column SpMM on a 3-D grid: 1 y-slab runs (plan_grid_runs), 0 rows in order
 int g_spmm_dynamic = 1;   // 64-column SpMM sweeps: 1 steps from per-XCD counters, 0 fixed stride
 int g_block3 = 1;         // 3x3-block matrices: kind-3 row blocks (1) or plain CSR-stream blocks (0)
+int g_diag_split = 1;     // FP32 SpMVs of square x-segmented matrices: diagonal apart (1) or in the entries (0)
+// csr_upload options
+constexpr int kUpForceXseg = 1;   // x-segmented whatever the column scatter
+constexpr int kUpNoSplit = 2;     // no FP32 off-diagonal part (the part itself)
 }  // namespace
 
 #define fail ::hbs::set_error
@@ -411,6 +415,20 @@ extern "C" int hbs_set_spmv_xseg(int mode)
     return HBS_OK;
 }
 
+extern "C" int hbs_set_spmv_diag_split(int mode)
+{
+    if (mode < 0 || mode > 1) return fail(HBS_ERR_ARG, "mode must be 0 or 1");
+    g_diag_split = mode;
+    return HBS_OK;
+}
+
+extern "C" int hbs_csr_diag_split(hbs_csr a, int *mode)
+{
+    if (!a || !mode) return fail(HBS_ERR_ARG, "null argument");
+    *mode = !a->od ? 0 : a->diag32 ? 2 : 1;
+    return HBS_OK;
+}
+
 extern "C" int hbs_csr_destroy(hbs_csr a)
 {
     if (!a) return HBS_OK;
@@ -458,6 +476,10 @@ extern "C" int hbs_csr_destroy(hbs_csr a)
         delete[] a->evs;
     }
     if (a->tr) hbs_csr_destroy(a->tr);
+    if (a->od) hbs_csr_destroy(a->od);
+    dfree(a->diag32);
+    dfree(a->od_src);
+    dfree(a->diag_src);
     delete a;
     return HBS_OK;
 }
@@ -502,7 +524,61 @@ int default_block_nnz(int m, int nnz, int resident)
 }
 
 int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int *col, const double *val, int flags,
-               int nsys, const int *sys_rows, hbs_csr *out)
+               int nsys, const int *sys_rows, hbs_csr *out, int up = 0);
+
+// FP32 SpMVs of a square x-segmented matrix: the diagonal apart and the
+// off-diagonal entries (CSR order) as an x-segmented matrix of their own, so
+// the product pass and the row sums move no diagonal entry; each row adds
+// d_i x_i to its off-diagonal sum (x_i alone when every diagonal entry is
+// 1, as after a Jacobi scaling).  A row's first entry in column i is its
+// diagonal; further ones stay off-diagonal entries, and a row without one
+// has d_i = 0.  A no-op when the off-diagonal part would not be x-segmented.
+int split_diagonal(hbs_csr a, const int *rowptr, const int *col, const double *val)
+{
+    const int m = a->m, nnz = a->nnz;
+    std::vector<int> orp((size_t)m + 1, 0), ocol, osrc, dsrc((size_t)m, nnz);
+    ocol.reserve((size_t)nnz);
+    osrc.reserve((size_t)nnz);
+    bool unit = true;
+    for (int r = 0; r < m; ++r) {
+        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            if (col[j] == r && dsrc[(size_t)r] == nnz) {
+                dsrc[(size_t)r] = j;
+            } else {
+                ocol.push_back(col[j]);
+                osrc.push_back(j);
+            }
+        }
+        orp[(size_t)r + 1] = (int)ocol.size();
+        unit = unit && dsrc[(size_t)r] < nnz && (float)val[dsrc[(size_t)r]] == 1.0f;
+    }
+    const int onz = (int)ocol.size();
+    if (onz == 0) return HBS_OK;
+    std::vector<double> oval((size_t)onz);
+    for (int k = 0; k < onz; ++k) oval[(size_t)k] = val[osrc[(size_t)k]];
+    hbs_csr od = nullptr;
+    CHK(csr_upload(a->ctx, m, m, onz, orp.data(), ocol.data(), oval.data(), HBS_CSR_F32, 1, nullptr, &od,
+                   kUpForceXseg | kUpNoSplit));
+    if (od->xs_nseg == 0) {
+        hbs_csr_destroy(od);
+        return HBS_OK;
+    }
+    a->od = od;
+    int rc;
+    if ((rc = dalloc(&a->od_src, (size_t)onz)) || (rc = upload(a->od_src, osrc)) ||
+        (rc = dalloc(&a->diag_src, (size_t)m)) || (rc = upload(a->diag_src, dsrc)))
+        return rc;
+    if (!unit) {
+        if ((rc = dalloc(&a->diag32, (size_t)m))) return rc;
+        if (launch_gather_to_f32(m, a->val, a->diag_src, a->diag32, a->ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(a->ctx->stream) != hipSuccess)
+            return fail(HBS_ERR_HIP, "FP32 diagonal copy failed");
+    }
+    return HBS_OK;
+}
+
+int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int *col, const double *val, int flags,
+               int nsys, const int *sys_rows, hbs_csr *out, int up)
 {
     hbs_csr a = new hbs_csr_s();
     a->ctx = ctx;
@@ -543,8 +619,8 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
     // windows of kSegBlockNnz entries touch a new line for more than half
     // the entries
     bool xseg = (nsys <= 1 || !sys_rows) && nnz > 0 &&
-                (g_xseg == 1 || (g_xseg == 2 && (long long)n * 8 >= (8LL << 20) &&
-                                 column_scatter(col, nnz, kSegBlockNnz) > 0.5));
+                (g_xseg == 1 || (up & kUpForceXseg) ||
+                 (g_xseg == 2 && (long long)n * 8 >= (8LL << 20) && column_scatter(col, nnz, kSegBlockNnz) > 0.5));
     int block_nnz =
         xseg ? (g_block_nnz > 2048 ? g_block_nnz : kSegBlockNnz)
              : g_block_nnz > 0 ? g_block_nnz : default_block_nnz(m, nnz, spmv_resident_groups(6144, 4, ctx->device));
@@ -657,6 +733,8 @@ int csr_upload(hbs_ctx ctx, int m, int n, int nnz, const int *rowptr, const int 
                 hipStreamSynchronize(ctx->stream) != hipSuccess)
                 return bail(fail(HBS_ERR_HIP, "FP32 value copy failed"));
         }
+        if (a->val32 && m == n && g_diag_split && !(up & kUpNoSplit) && (rc = split_diagonal(a, rowptr, col, val)))
+            return bail(rc);
     }
     if (b3) {
         // k_spmv3's threads past a row block's blocks read up to kWideThreads
@@ -817,6 +895,19 @@ extern "C" int hbs_csr_set_values(hbs_csr a, const double *val_dev)
         if (t->xs_val32) HIPCHK(launch_to_f32(t->nnz, t->xs_val, t->xs_val32, q));
         if (t->bval) HIPCHK(launch_gather_f64(t->nnz, t->val, t->bsrc, t->bval, q));
         if (t->bval32) HIPCHK(launch_to_f32(t->nnz, t->bval, t->bval32, q));
+    }
+    if (hbs_csr od = a->od) {
+        // the FP32 off-diagonal part and the diagonal (whose new values need
+        // not be 1: from here on the row sums read them)
+        HIPCHK(launch_gather_f64(od->nnz, a->val, a->od_src, od->val, q));
+        HIPCHK(launch_to_f32(od->nnz, od->val, od->val32, q));
+        HIPCHK(launch_gather_f64(od->nnz, od->val, od->xs_src, od->xs_val, q));
+        HIPCHK(launch_to_f32(od->nnz, od->xs_val, od->xs_val32, q));
+        if (!a->diag32) {
+            CHK(dalloc(&a->diag32, (size_t)a->m));
+            ++a->diag_gen;   // graphs captured with the unit-diagonal row sums are stale
+        }
+        HIPCHK(launch_gather_to_f32(a->m, a->val, a->diag_src, a->diag32, q));
     }
     return HBS_OK;
 }

@@ -41,6 +41,11 @@ __global__ void k_to_f32(int n, const double *src, float *dst)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         dst[i] = (float)src[i];
 }
+__global__ void k_gather_f32(int n, const double *in, const int *idx, float *out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = (float)in[idx[i]];
+}
 __global__ void k_gather(int n, const double *in, const int *idx, double *out)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -226,6 +231,12 @@ hipError_t launch_to_f32(int n, const double *src, float *dst, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     k_to_f32<<<grid_for(n), kThreads, 0, s>>>(n, src, dst);
+    return hipGetLastError();
+}
+hipError_t launch_gather_to_f32(int n, const double *in, const int *idx, float *out, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_gather_f32<<<grid_for(n), kThreads, 0, s>>>(n, in, idx, out);
     return hipGetLastError();
 }
 hipError_t launch_gather_f64(int n, const double *in, const int *idx, double *out, hipStream_t s)

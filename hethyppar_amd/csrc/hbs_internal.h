@@ -149,6 +149,8 @@ struct SpmvArgs {
     const int *bcol;            // kind-3 (3x3-block) blocks: each block's first column
     const T *bval;              //   and the values block-major (b3_layout)
     int b3var;                  //   some kind-3 blocks hold block rows of different lengths
+    int dmode;                  // x-segmented row sums add a diagonal term: 0 none, 1 x[r], 2 diag[r] * x[r]
+    const T *diag;
 };
 
 // x-segmented SpMV, product pass (k_xseg): workgroup s stages the x segment
@@ -432,6 +434,8 @@ hipError_t launch_to_f32(int n, const double *src, float *dst, hipStream_t s);
 
 // Halo kernels: out[k] = in[idx[k]] and out[idx[k]] = in[k].
 hipError_t launch_gather_f64(int n, const double *in, const int *idx, double *out, hipStream_t s);
+// out[i] = (float)in[idx[i]]
+hipError_t launch_gather_to_f32(int n, const double *in, const int *idx, float *out, hipStream_t s);
 hipError_t launch_scatter_f64(int n, const double *in, const int *idx, double *out, hipStream_t s);
 
 // Adjoint gradient on the pattern: g[j] = -lambda[row(j)] * x[col[j]].

@@ -76,6 +76,30 @@ __device__ __forceinline__ constexpr bool reads_w()
     return EPI != EPI_AXPBY;
 }
 
+// Diagonal operands of row r of an x-segmented matrix whose diagonal is
+// kept apart (DG 1: unit diagonal, x[r]; DG 2: diag[r] and x[r]), loaded
+// with the row's epilogue operand; diag_add adds the row's diagonal term to
+// the sum of its off-diagonal products.
+template <typename T, int DG>
+struct DiagOp {
+    T x, d;
+};
+template <typename T, int DG>
+__device__ __forceinline__ DiagOp<T, DG> diag_load(const SpmvArgs<T> &a, int r)
+{
+    DiagOp<T, DG> o{T(0), T(0)};
+    if constexpr (DG >= 1) o.x = a.x[r];
+    if constexpr (DG == 2) o.d = a.diag[r];
+    return o;
+}
+template <typename T, int DG>
+__device__ __forceinline__ T diag_add(T acc, const DiagOp<T, DG> &o)
+{
+    if constexpr (DG == 1) return acc + o.x;
+    else if constexpr (DG == 2) return acc + o.d * o.x;
+    else return acc;
+}
+
 // One pass over a row block: the entries base + t + TH * i (i < NPT) of the
 // block, as loaded column indices, values and (kind 2) CSR positions -- or,
 // for x-segmented matrices (PRE), the products and their CSR positions.  A
@@ -249,7 +273,7 @@ __device__ __forceinline__ int2 issue_rp(const int *rowptr, const RowBlock &rb)
 // order from 0 (the reference's order): thread t sums rows t, t + TH, ...,
 // with their epilogue operands loaded together (the first one, w0, was
 // prefetched).  Other blocks use `lanes` lanes per row and an xor butterfly.
-template <typename T, int EPI, int TH, bool PRE>
+template <typename T, int EPI, int TH, bool PRE, int DG>
 __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, T w0,
                                          const T *lds, const unsigned short *lrp, const int (&rq)[4], Acc<T> &d0,
                                          Acc<T> &d1)
@@ -262,9 +286,12 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
         // sums row rq[q], in CSR order as ever; its epilogue operand and
         // output sit at that row
         T wr[kRowsPerThread], acc[kRowsPerThread];
+        DiagOp<T, DG> dg[kRowsPerThread];
 #pragma unroll
-        for (int q = 0; q < kRowsPerThread; ++q)
+        for (int q = 0; q < kRowsPerThread; ++q) {
             wr[q] = reads_w<EPI>() ? a.w[rb.r0 + min(rq[q], nr - 1)] : T(0);
+            dg[q] = diag_load<T, DG>(a, rb.r0 + min(rq[q], nr - 1));
+        }
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
@@ -273,16 +300,18 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
-            if (t < nr) row_out<T, EPI>(a, rb.r0 + rq[q], acc[q], wr[q], d0, d1);
+            if (t < nr) row_out<T, EPI>(a, rb.r0 + rq[q], diag_add<T, DG>(acc[q], dg[q]), wr[q], d0, d1);
         }
         return;
     }
     if (lanes <= 1) {
         T wr[kRowsPerThread];
+        DiagOp<T, DG> dg[kRowsPerThread];
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
             wr[q] = q == 0 ? w0 : (reads_w<EPI>() ? a.w[rb.r0 + min(t, nr - 1)] : T(0));
+            dg[q] = diag_load<T, DG>(a, rb.r0 + min(t, nr - 1));
         }
         // all sums first, then the stores: a store between two rows' LDS
         // reads makes the next row wait for every memory operation in flight
@@ -296,7 +325,7 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
-            if (t < nr) row_out<T, EPI>(a, rb.r0 + t, acc[q], wr[q], d0, d1);
+            if (t < nr) row_out<T, EPI>(a, rb.r0 + t, diag_add<T, DG>(acc[q], dg[q]), wr[q], d0, d1);
         }
     } else {
         const int gi = threadIdx.x / lanes, l = threadIdx.x & (lanes - 1);
@@ -305,13 +334,15 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
             const int t = t0 + gi;
             T acc = T(0);
             T wr = w0;
+            DiagOp<T, DG> dg{T(0), T(0)};
             if (t < nr) {
                 if (t0 > 0 && reads_w<EPI>() && l == 0) wr = a.w[rb.r0 + t];
+                if (l == 0) dg = diag_load<T, DG>(a, rb.r0 + t);
                 const int e = lrp[t + 1];
                 acc = sum_lds<T, B>(lds, lrp[t] + l, e, lanes);
             }
             for (int o = lanes >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-            if (t < nr && l == 0) row_out<T, EPI>(a, rb.r0 + t, acc, wr, d0, d1);
+            if (t < nr && l == 0) row_out<T, EPI>(a, rb.r0 + t, diag_add<T, DG>(acc, dg), wr, d0, d1);
         }
     }
 }
@@ -385,7 +416,7 @@ __device__ __forceinline__ void spmv_parts(const SpmvArgs<T> &a, int p, const Ac
 // rest of the state prologue, so they overlap it.  PRE (x-segmented
 // matrices): the passes load the products k_xseg formed, with their
 // positions, instead of gathering x.
-template <typename T, int EPI, int NPT, int TH, bool PRE>
+template <typename T, int EPI, int NPT, int TH, bool PRE, int DG = 0>
 __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_spmv(SpmvArgs<T> a)
 {
     constexpr int NW = TH / 64;
@@ -493,6 +524,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 // loads are one round trip per 64 chunks, not one per chunk.
                 if (threadIdx.x < 64) {
                     const T wr = reads_w<EPI>() && threadIdx.x == 0 ? a.w[cur.rb.r0] : T(0);
+                    const DiagOp<T, DG> dg = diag_load<T, DG>(a, cur.rb.r0);
                     T total = T(0);
                     for (int c0 = 0; c0 < cur.ax.nchunks; c0 += 64) {
                         const int c = c0 + (int)threadIdx.x;
@@ -500,11 +532,11 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         const int lim = min(64, cur.ax.nchunks - c0);
                         for (int l = 0; l < lim; ++l) total += __shfl(v, l, 64);
                     }
-                    if (threadIdx.x == 0) row_out<T, EPI>(a, cur.rb.r0, total, wr, d0, d1);
+                    if (threadIdx.x == 0) row_out<T, EPI>(a, cur.rb.r0, diag_add<T, DG>(total, dg), wr, d0, d1);
                 }
             } else {
                 __syncthreads();
-                sum_rows<T, EPI, TH, PRE>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
+                sum_rows<T, EPI, TH, PRE, DG>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
                 __syncthreads();   // LDS is reused by the next row block
             }
             if (more) w0 = issue_w0<T, EPI>(a, nxt.rb, nxt.ax);
@@ -1129,16 +1161,16 @@ __global__ void k_state_init(SolverState *st, int nstates, double tol, int maxit
     }
 }
 
-template <typename T, int NPT, int TH, bool PRE = false>
+template <typename T, int NPT, int TH, bool PRE = false, int DG = 0>
 hipError_t spmv_epi(const SpmvArgs<T> &a, int epi, size_t shmem, hipStream_t s)
 {
     switch (epi) {
-    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
-    case EPI_RESIDB: k_spmv<T, EPI_RESIDB, NPT, TH, PRE><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_AXPBY: k_spmv<T, EPI_AXPBY, NPT, TH, PRE, DG><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT1: k_spmv<T, EPI_DOT1, NPT, TH, PRE, DG><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_DOT2: k_spmv<T, EPI_DOT2, NPT, TH, PRE, DG><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID: k_spmv<T, EPI_RESID, NPT, TH, PRE, DG><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESID2: k_spmv<T, EPI_RESID2, NPT, TH, PRE, DG><<<a.ngroups, TH, shmem, s>>>(a); break;
+    case EPI_RESIDB: k_spmv<T, EPI_RESIDB, NPT, TH, PRE, DG><<<a.ngroups, TH, shmem, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1176,14 +1208,16 @@ hipError_t raise_lds_limit(const void *f, size_t bytes)
 }
 
 // Allow the dynamic LDS of the wide blocks.
-template <typename T, int NPT, int TH, bool PRE = false>
+template <typename T, int NPT, int TH, bool PRE = false, int DG = 0>
 hipError_t spmv_allow_lds(size_t shmem)
 {
     hipError_t e = hipSuccess;
-    const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_DOT1, NPT, TH, PRE>,
-                       (const void *)k_spmv<T, EPI_DOT2, NPT, TH, PRE>, (const void *)k_spmv<T, EPI_RESID, NPT, TH, PRE>,
-                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH, PRE>,
-                       (const void *)k_spmv<T, EPI_RESIDB, NPT, TH, PRE>};
+    const void *k[] = {(const void *)k_spmv<T, EPI_AXPBY, NPT, TH, PRE, DG>,
+                       (const void *)k_spmv<T, EPI_DOT1, NPT, TH, PRE, DG>,
+                       (const void *)k_spmv<T, EPI_DOT2, NPT, TH, PRE, DG>,
+                       (const void *)k_spmv<T, EPI_RESID, NPT, TH, PRE, DG>,
+                       (const void *)k_spmv<T, EPI_RESID2, NPT, TH, PRE, DG>,
+                       (const void *)k_spmv<T, EPI_RESIDB, NPT, TH, PRE, DG>};
     for (const void *f : k)
         if (e == hipSuccess) e = raise_lds_limit(f, shmem);
     return e;
@@ -1210,10 +1244,18 @@ hipError_t spmv_dispatch(const SpmvArgs<T> &a, int epi, int block_nnz, int npt, 
     if (a.ngroups <= 0) return hipSuccess;
     if (block_nnz <= 0 || block_nnz > kMaxBlockNnz) return hipErrorInvalidValue;
     const size_t prod = (size_t)block_nnz * sizeof(T);
+    if (a.dmode != 0 && !a.prod) return hipErrorInvalidValue;   // a diagonal apart only with x segments
     if (a.prod) {
         // x-segmented: 512-thread workgroups, one pass per row block
         if (spmv_threads(block_nnz) != kWideThreads || npt != kPreNpt) return hipErrorInvalidValue;
-        return spmv_epi<T, kPreNpt, kWideThreads, true>(a, epi, prod + rowptr_lds_bytes<kWideThreads>(), s);
+        const size_t shmem = prod + rowptr_lds_bytes<kWideThreads>();
+        if constexpr (sizeof(T) == 4) {
+            // FP32 copies of matrices with their diagonal apart
+            if (a.dmode == 1) return spmv_epi<T, kPreNpt, kWideThreads, true, 1>(a, epi, shmem, s);
+            if (a.dmode == 2) return spmv_epi<T, kPreNpt, kWideThreads, true, 2>(a, epi, shmem, s);
+        }
+        if (a.dmode != 0) return hipErrorInvalidValue;
+        return spmv_epi<T, kPreNpt, kWideThreads, true>(a, epi, shmem, s);
     }
     if (a.bval) {
         // 3x3-block matrices: row blocks of at most kWideThreads blocks
@@ -1266,6 +1308,8 @@ hipError_t prepare_spmv(int block_nnz, int npt)
     if (npt == kPreNpt) {
         e = spmv_allow_lds<double, kPreNpt, kWideThreads, true>(d);
         if (e == hipSuccess) e = spmv_allow_lds<float, kPreNpt, kWideThreads, true>(f);
+        if (e == hipSuccess) e = spmv_allow_lds<float, kPreNpt, kWideThreads, true, 1>(f);
+        if (e == hipSuccess) e = spmv_allow_lds<float, kPreNpt, kWideThreads, true, 2>(f);
     } else if (npt == kB3Npt) {
         e = spmv3_allow_lds<double, false>();
         if (e == hipSuccess) e = spmv3_allow_lds<float, false>();

@@ -115,6 +115,14 @@ struct hbs_csr_s {
     unsigned char *xs_delta = nullptr;
     double *xs_val = nullptr, *xs_prod = nullptr;
     float *xs_val32 = nullptr, *xs_prod32 = nullptr;
+    // FP32 SpMVs of a square x-segmented matrix: the off-diagonal entries as
+    // an x-segmented matrix of their own (od), the diagonal added per row
+    // (d_i x_i; x_i alone while every diagonal entry is 1, diag32 null)
+    hbs_csr_s *od = nullptr;
+    float *diag32 = nullptr;
+    int *od_src = nullptr;            // [od->nnz] CSR entry of each off-diagonal entry (refreshes od->val)
+    int *diag_src = nullptr;          // [m] CSR entry of each row's diagonal, nnz (a zero) for none
+    int diag_gen = 0;                 // counts changes of the diagonal's form (unit -> values)
     hbs::SpmvPlan plan;
     hbs::SysRange *sys = nullptr;     // [nsys] {g0, g1, 0, 0}
     double *chunk_sums = nullptr;     // [nchunks]
@@ -180,6 +188,7 @@ struct hbs_solver_s {
     double *chunk_sums = nullptr; // long-row chunk sums
     static constexpr int kGraphLens = 33;             // chunks of 1..32 iterations
     hipGraphExec_t graph[2][kGraphLens][2] = {};       // [FP64, FP32][chunk length][poll slot]
+    int graph_diag_gen = 0;                            // A->diag_gen the FP32 graphs were captured at
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int seq = 0;                  // chunks launched without graphs: the mirror's completion stamp
@@ -195,9 +204,28 @@ namespace hbs {
 int wait_event_limited(hipEvent_t ev);
 
 // SpMV launch arguments of matrix a with values val (FP64 or FP32 copy).
+// The matrix whose streams T's SpMVs read: FP32 SpMVs of a matrix with
+// its diagonal apart read the off-diagonal part.
+template <typename T>
+hbs_csr spmv_mat(hbs_csr a)
+{
+    return sizeof(T) == 4 && a->od ? a->od : a;
+}
+
+// SpMV launch arguments of matrix a with values val (FP64 or FP32 copy; an
+// FP32 SpMV of a matrix with its diagonal apart reads a->od's FP32 values
+// and the diagonal).
 template <typename T>
 SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
 {
+    if constexpr (sizeof(T) == 4) {
+        if (a->od) {
+            SpmvArgs<T> s = spmv_args<T>(a->od, a->od->val32);
+            s.dmode = a->diag32 ? 2 : 1;
+            s.diag = a->diag32;
+            return s;
+        }
+    }
     SpmvArgs<T> s{};
     s.blocks = a->plan.blocks;
     s.aux = a->plan.aux;
@@ -234,8 +262,9 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
 // Queue one SpMV: for x-segmented matrices the product pass, then the
 // long-row chunk sums (if any), then the row blocks.
 template <typename T>
-hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a, int epi, const SolverState *chunk_st, hipStream_t q)
+hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a_in, int epi, const SolverState *chunk_st, hipStream_t q)
 {
+    const hbs_csr a = spmv_mat<T>(a_in);
     if (a->xs_nseg > 0) {
         XsegArgs<T> x{};
         x.nseg = a->xs_nseg;

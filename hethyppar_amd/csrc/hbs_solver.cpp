@@ -103,13 +103,16 @@ extern "C" int hbs_solver_create(hbs_csr a, hbs_solver *out)
         ranges[(size_t)q].v1 = (int)vb.size();
     }
     s->nvb = (int)vb.size();
-    const int ng = a->plan.ngroups;
+    // SpMV partials and chunk sums of A's plan or, for FP32 SpMVs of a
+    // matrix with its diagonal apart, of its off-diagonal part's
+    const int ng = std::max(a->plan.ngroups, a->od ? a->od->plan.ngroups : 0);
+    const int nch = std::max(a->plan.nchunks, a->od ? a->od->plan.nchunks : 0);
     if ((rc = dalloc(&s->vblocks, vb.size())) || (rc = dalloc(&s->sys, (size_t)s->nsys))) return bail(rc);
     if ((rc = upload(s->vblocks, vb)) || (rc = upload(s->sys, ranges))) return bail(rc);
     const size_t ps = (size_t)std::max(ng, 1) * kMaxDots, pv = (size_t)std::max(s->nvb, 1) * kMaxDots;
     if ((rc = dalloc(&s->partA, ps)) || (rc = dalloc(&s->partB, pv)) || (rc = dalloc(&s->partC, ps)) ||
         (rc = dalloc(&s->partD, pv)) || (rc = dalloc(&s->partE, ps)) || (rc = dalloc(&s->partF, pv)) ||
-        (rc = dalloc(&s->chunk_sums, a->plan.nchunks)))
+        (rc = dalloc(&s->chunk_sums, nch)))
         return bail(rc);
     for (hipEvent_t *e : {&s->ev[0], &s->ev[1], &s->e0, &s->e1})
         if (hipEventCreate(e) != hipSuccess) return bail(fail(HBS_ERR_HIP, "event creation failed"));
@@ -172,7 +175,7 @@ hipError_t launch_iteration(hbs_solver s, const Work<T> &w)
 {
     hipStream_t q = s->A->ctx->stream;
     SolverState *A = s->slot(kSlotA), *B = s->slot(kSlotB);
-    const int ng = s->A->plan.ngroups;
+    const int ng = spmv_mat<T>(s->A)->plan.ngroups;
     hipError_t e;
     VecArgs<T> va = vec_args<T>(s, w);
     // P = ((P - V w) b) + R, after FIN_XR of the previous iteration
@@ -251,6 +254,18 @@ int run_chunk(hbs_solver s, const Work<T> &w, int k, int slot, bool use_graph, i
         *stamp = s->seq;
         HIPCHK(launch_chunk<T>(s, w, k, slot, s->seq));
         return HBS_OK;
+    }
+    // FP32 graphs captured before the matrix's diagonal changed form run
+    // the old row sums: drop them (no graph of this solver is in flight
+    // between solves)
+    if (gi == 1 && s->graph_diag_gen != s->A->diag_gen) {
+        for (auto &bylen : s->graph[1])
+            for (hipGraphExec_t &g : bylen)
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
+        s->graph_diag_gen = s->A->diag_gen;
     }
     // one graph per chunk length and poll slot, captured once and kept
     hipGraphExec_t &ge = s->graph[gi][k][slot];
@@ -670,7 +685,7 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
         std::vector<VecBlock> vb;
         const int epb = std::min(2 * vec_elems_per_block(n), 8192);
         plan_vec(0, n, epb, 0, vb);
-        std::vector<SysRange> r = A->sys_host;
+        std::vector<SysRange> r = spmv_mat<float>(A)->sys_host;   // the FP32 SpMVs' workgroups
         r[0].v0 = 0;
         r[0].v1 = (int)vb.size();
         if ((int)vb.size() > s->nvb) return fail(HBS_ERR_ARG, "FP32 vector plan larger than the FP64 one");

@@ -157,6 +157,8 @@ _PROTOS = {
     "hbs_plan_grid_runs": (_i, [_i, _ip, _ip, _ip, _i, _ip, _ip]),
     "hbs_csr_spmm_runs": (_i, [_vp, _ip, _ip]),
     "hbs_csr_xseg": (_i, [_vp, _ip]),
+    "hbs_set_spmv_diag_split": (_i, [_i]),
+    "hbs_csr_diag_split": (_i, [_vp, _ip]),
     "hbs_shard_pack_host": (_i, [_vp, _vp, _dp]),
     "hbs_vector_part": (_i, [_i, _dp, _ip, _i, _i, _dp, _ip]),
     "hbs_vector_read_parted": (_i, [C.c_char_p, _i, _ip, _i, _i, _dpp, _ip]),
@@ -636,6 +638,9 @@ class DeviceCSR:
         b3 = C.c_int()
         check(lib().hbs_csr_block3(self.h, C.byref(b3)))
         d["block3"] = b3.value
+        ds = C.c_int()
+        check(lib().hbs_csr_diag_split(self.h, C.byref(ds)))
+        d["diag_split"] = ds.value
         nr, ln = C.c_int(), C.c_int()
         check(lib().hbs_csr_spmm_runs(self.h, C.byref(nr), C.byref(ln)))
         d["spmm_runs"], d["spmm_run_len"] = nr.value, ln.value
@@ -866,6 +871,12 @@ def plan_grid_runs(rowptr, col):
     check(lib().hbs_plan_grid_runs(m, rp.ctypes.data_as(_ip), c.ctypes.data_as(_ip), out.ctypes.data_as(_ip),
                                    int(out.size), C.byref(nr), C.byref(ln)))
     return out[:nr.value].copy(), ln.value
+
+
+def set_spmv_diag_split(mode=1):
+    """Process-wide: FP32 SpMVs of square x-segmented DeviceCSR objects
+    created afterwards keep the diagonal apart (1) or in the entries (0)."""
+    check(lib().hbs_set_spmv_diag_split(mode))
 
 
 def set_spmv_xseg(mode=2):

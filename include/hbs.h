@@ -414,6 +414,18 @@ int hbs_set_xseg_split(int split);
 /* Segments of an x-segmented device CSR (0: CSR-stream SpMV). */
 int hbs_csr_xseg(hbs_csr a, int *nseg);
 
+/* FP32 SpMVs (hbs_csr_spmv_f32 and the FP32 sweeps of the mixed solve) of
+ * square x-segmented device CSRs created afterwards with HBS_CSR_F32: 1
+ * (default) keeps the diagonal apart -- the off-diagonal entries form an
+ * x-segmented matrix of their own and each row adds d_i x_i (x_i alone while
+ * every diagonal entry is 1) to its off-diagonal sum, so the product pass
+ * and the row sums move no diagonal entry; 0 keeps it among the entries.
+ * FP64 SpMVs always sum every row in CSR order.  hbs_csr_diag_split: 0 not
+ * split, 1 split with a unit diagonal, 2 split with diagonal values (also
+ * after hbs_csr_set_values on a matrix split with a unit diagonal). */
+int hbs_set_spmv_diag_split(int mode);
+int hbs_csr_diag_split(hbs_csr a, int *mode);
+
 /* 3x3-block matrices (every block row's three rows share one list of
  * aligned column triples, at most 256 entries per row; one system, CSR-stream
  * SpMV) of device CSRs created afterwards: 1 (default) reads the values

@@ -148,6 +148,102 @@ def test_xseg_set_values_and_transpose(ctx):
     assert np.allclose(dy32.download(), ref, rtol=1e-5, atol=1e-5)
 
 
+def _diag_cases():
+    rp, c, v = hbs.gen_circuit(60000, nheavy=12, heavy_lo=300, heavy_hi=30000, seed=7)
+    out = [("unit", rp, c, v, 1)]
+    rp, c, v = hbs.gen_circuit(60000, nheavy=12, heavy_lo=300, heavy_hi=30000, seed=7, equilibrate=False)
+    out.append(("values", rp, c, v, 2))
+    # rows without a diagonal entry, and rows with two (the first is the
+    # diagonal, the second stays an off-diagonal entry)
+    rp2, c2, v2 = [0], [], []
+    rng = np.random.default_rng(12)
+    for r in range(rp.size - 1):
+        cols, vals = list(c[rp[r]:rp[r + 1]]), list(v[rp[r]:rp[r + 1]])
+        if r % 7 == 3:
+            keep = [k for k in range(len(cols)) if cols[k] != r]
+            cols, vals = [cols[k] for k in keep], [vals[k] for k in keep]
+        elif r % 11 == 5:
+            cols.append(r)
+            vals.append(rng.standard_normal())
+        c2 += cols
+        v2 += vals
+        rp2.append(len(c2))
+    out.append(("missing_and_twice", np.array(rp2, np.int32), np.array(c2, np.int32), np.array(v2), 2))
+    return out
+
+
+@pytest.mark.parametrize("case", _diag_cases(), ids=lambda c: c[0])
+def test_xseg_fp32_diagonal_apart(ctx, case):
+    """FP32 SpMVs of a square x-segmented matrix keep the diagonal apart:
+    the off-diagonal sum plus d_i x_i (x_i for a unit diagonal) lies within
+    the FP32 summation bound of the oracle's product of the FP32 values; the
+    FP64 SpMV of the same matrix stays bit-exact; new values (a diagonal no
+    longer 1) reach both parts."""
+    name, rp, c, v, mode = case
+    n = rp.size - 1
+    x = np.random.default_rng(31).uniform(-1, 1, n)
+    A = _forced(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    info = A.info()
+    print(name, info, flush=True)
+    assert info["xseg"] > 0 and info["diag_split"] == mode
+    dx, dy = ctx.to_device(x), ctx.empty(n)
+    A.spmv(dx, dy)
+    _check_f64(rp, c, v, x, dy.download())
+    u = np.finfo(np.float32).eps / 2
+    k = np.diff(rp)
+
+    def check32(vals):
+        x32 = x.astype(np.float32)
+        dx32, dy32 = ctx.to_device(x32), ctx.empty(n, np.float32)
+        A.spmv(dx32, dy32)
+        y32 = dy32.download().astype(np.float64)
+        ref = orc.spmv(orc.CSR(rp, c, vals.astype(np.float32).astype(np.float64)), x32.astype(np.float64))
+        bound = 2 * (2 * k * u / (1 - k * u)) * orc.spmv(orc.CSR(rp, c, np.abs(vals)), np.abs(x))
+        assert np.all(np.abs(y32 - ref) <= bound * 1.01 + 1e-30)
+
+    check32(v)
+    v2 = v * 0.5 - 0.125
+    A.set_values(ctx.to_device(v2))
+    assert A.info()["diag_split"] == 2
+    check32(v2)
+    A.spmv(dx, dy)
+    assert np.array_equal(dy.download()[k <= 256], orc.spmv(orc.CSR(rp, c, v2), x)[k <= 256])
+
+
+def test_xseg_fp32_diagonal_apart_off_matches_on(ctx):
+    """The same mixed solve with the diagonal apart and among the entries:
+    both meet the bar (relres <= 1e-8, forward error no worse than twice the
+    oracle's); the split takes no more FP32 iterations than two over the
+    other's."""
+    rp, c, v = hbs.gen_circuit(150000, nheavy=10, heavy_lo=1000, heavy_hi=20000, seed=3)
+    n = rp.size - 1
+    xt = hbs.gen_uniform(n, -1.0, 1.0, 13)
+    b = hbs.host_spmv(rp, c, v, xt)
+    xo, ito, _, _ = orc.bicgstab(orc.CSR(rp, c, v), b, tol=1e-8, maxit=2000)
+    e_o = np.linalg.norm(xo - xt) / np.linalg.norm(xt)
+    res = {}
+    for split in (1, 0):
+        hbs.set_spmv_diag_split(split)
+        try:
+            A = _forced(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+        finally:
+            hbs.set_spmv_diag_split()
+        assert A.info()["diag_split"] == split
+        S = hbs.Solver(A)
+        db, dx = ctx.to_device(b), ctx.empty(n)
+        dx.zero()
+        im = S.bicgstab_mixed(db, dx, tol=1e-8, maxit=4000)
+        e_m = np.linalg.norm(dx.download() - xt) / np.linalg.norm(xt)
+        print(f"split {split}: iters {im.iters} sweeps {im.sweeps} relres {im.relres:.3e} err {e_m:.2e} "
+              f"(oracle {ito}, {e_o:.2e})", flush=True)
+        assert im.status == 0 and im.relres <= 1.01e-8
+        assert e_m <= 2 * max(e_o, 1e-7)
+        res[split] = im.iters
+        S.close()
+        A.close()
+    assert res[1] <= res[0] + 2
+
+
 @pytest.mark.parametrize("graph", [0, 1])
 def test_xseg_bicgstab_fp64_and_mixed(ctx, graph):
     rp, c, v = hbs.gen_circuit(150000, nheavy=10, heavy_lo=1000, heavy_hi=20000, seed=3)
