@@ -183,9 +183,12 @@ def main():
     ms_fp64, i64 = out[False]
 
     # dominant kernel: the x-segmented FP32 SpMV (k_xseg + k_chunks + row
-    # sums), one HIP event pair on the library's stream around back-to-back
-    # launches
+    # sums, the diagonal apart), one HIP event pair on the library's stream
+    # around back-to-back launches
     log("bench: SpMV roofline")
+    diag_split = A.info()["diag_split"]
+    rows = np.repeat(np.arange(N3, dtype=c.dtype), np.diff(rp))
+    nnz_diag = int(np.unique(rows[c == rows]).size)   # rows with a diagonal entry: one each goes apart
     x32, y32 = ctx.to_device(xt.astype(np.float32)), ctx.empty(N3, np.float32)
     spmv32_ms = A.bench_spmv(x32, y32, args.spmv_reps)
     x64, y64 = ctx.to_device(xt), ctx.empty(N3)
@@ -230,10 +233,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profile_value("spmv_circuit3p5M_pmc.txt", "traffic_bytes_per_spmv_fp32"),
-                     "kernel": "x-segmented FP32 SpMV (k_xseg<float,16> + k_chunks<float> + "
-                               "k_spmv<float,EPI_AXPBY,16,512,true>)",
+                     "kernel": "x-segmented FP32 SpMV, diagonal apart (k_xseg<float,16> + k_chunks<float> + "
+                               "k_spmv<float,EPI_AXPBY,16,512,true,DG>)",
                      "launch_ms": round(spmv32_ms, 5), "bytes_per_launch": B32,
-                     "bytes_formula": "nnz*(4+4) + 4*(m+1) + 4*m + 4*n"},
+                     "bytes_formula": "nnz*(4+4) + 4*(m+1) + 4*m + 4*n (CSR, every entry)",
+                     "diag_split": diag_split,
+                     "format_bytes": B32 - 8 * nnz_diag + (4 * N3 if diag_split == 2 else 0),
+                     "format_formula": "(nnz - diagonal entries)*(4+4) + 4*(m+1) + 4*m + 4*n "
+                                       "(+ 4*m diagonal values unless unit)"},
         "spmv_fp64": {"ms": round(spmv64_ms, 5), "bytes": B64,
                       "gbs": round(B64 / (spmv64_ms * 1e-3) / 1e9, 1),
                       "frac": round(B64 / (spmv64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
@@ -269,28 +276,40 @@ def probe(hbs, which):
         probe_block(hbs, ctx)
         ctx.close()
         return
+    flags = hbs.HBS_CSR_F32
     if which == "circuit":
         rp, c, v, xt, _ = circuit(hbs)
     elif which == "fem":
         rp, c, v = hbs.gen_fem(483334, 12, 1.1, 6)
         xt = hbs.gen_uniform(rp.size - 1, -1.0, 1.0, 62)
+        flags |= hbs.HBS_CSR_TRANSPOSE
     else:
         rp, c, v = hbs.gen_markov(445000, 12, 22, 8900, 1.05, 2)
         xt = hbs.gen_uniform(445000, -1.0, 1.0, 12)
     n, nnz = rp.size - 1, int(c.size)
-    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=flags)
     x32, y32 = ctx.to_device(xt.astype(np.float32)), ctx.empty(n, np.float32)
     ms32 = A.bench_spmv(x32, y32, 10)
     x64, y64 = ctx.to_device(xt), ctx.empty(n)
     ms64 = A.bench_spmv(x64, y64, 10)
+    extra = {}
+    if flags & hbs.HBS_CSR_TRANSPOSE:
+        # the adjoint's SpMV: A^T x, FP64, 1 + 10 launches
+        A.spmv_t(x64, y64)
+        ctx.timer_start()
+        for _ in range(10):
+            A.spmv_t(x64, y64)
+        extra["spmv_t_fp64_ms"] = ctx.timer_stop() / 10
     cn = 1 << 25
     s, d = ctx.empty(cn), ctx.empty(cn)
     s.zero()
     cms = C.c_double()
     hbs.check(hbs.lib().hbs_bench_copy(ctx.h, s.ptr, d.ptr, cn, 3, C.byref(cms)))
-    print(json.dumps({"probe": which, "n": n, "nnz": nnz, "xseg": A.info()["xseg"], "block3": A.info()["block3"],
+    info = A.info()
+    print(json.dumps({"probe": which, "n": n, "nnz": nnz, "xseg": info["xseg"], "block3": info["block3"],
+                      "diag_split": info["diag_split"],
                       "spmv_fp32_ms": ms32, "spmv_fp32_bytes": spmv_bytes(n, n, nnz, 4),
-                      "spmv_fp64_ms": ms64, "spmv_fp64_bytes": spmv_bytes(n, n, nnz, 8),
+                      "spmv_fp64_ms": ms64, "spmv_fp64_bytes": spmv_bytes(n, n, nnz, 8), **extra,
                       "copy_doubles": cn, "copy_ms": cms.value}), flush=True)
     ctx.close()
 

@@ -268,6 +268,33 @@ __device__ __forceinline__ int2 issue_rp(const int *rowptr, const RowBlock &rb)
     return make_int2(rowptr[r], rowptr[r + 1]);
 }
 
+// Epilogue operands (w[r], diagonal operands) of this thread's row slots
+// of a serial block, rows in rperm order where the block takes it (PRE).
+// The x-segmented row sums load them before the next pass's streams: the
+// memory counter retires loads in order, so operands loaded after those
+// streams would make the row sums wait for the whole next pass.
+template <typename T, int DG>
+struct EpiOps {
+    T wr[kRowsPerThread];
+    DiagOp<T, DG> dg[kRowsPerThread];
+};
+template <typename T, int EPI, int TH, int DG>
+__device__ __forceinline__ void epi_load(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax,
+                                         const int (&rq)[4], EpiOps<T, DG> &o)
+{
+    // rperm order only in kind-0 blocks (a kind-1 block's `first` is its
+    // first chunk); every row index is clamped into the block
+    const int hi = max(rb.r1 - rb.r0 - 1, 0);
+    const bool perm = ax.kind == 0 && ax.first;
+#pragma unroll
+    for (int q = 0; q < kRowsPerThread; ++q) {
+        const int t = perm ? rq[q] : (int)threadIdx.x + q * TH;
+        const int r = rb.r0 + min(max(t, 0), hi);
+        o.wr[q] = reads_w<EPI>() ? a.w[r] : T(0);
+        o.dg[q] = diag_load<T, DG>(a, r);
+    }
+}
+
 // Sums the rows of a block whose products (lds) and row offsets (lrp) are in
 // LDS and feeds the epilogue.  Serial blocks add each row's products in CSR
 // order from 0 (the reference's order): thread t sums rows t, t + TH, ...,
@@ -275,8 +302,8 @@ __device__ __forceinline__ int2 issue_rp(const int *rowptr, const RowBlock &rb)
 // prefetched).  Other blocks use `lanes` lanes per row and an xor butterfly.
 template <typename T, int EPI, int TH, bool PRE, int DG>
 __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &rb, const BlockAux &ax, T w0,
-                                         const T *lds, const unsigned short *lrp, const int (&rq)[4], Acc<T> &d0,
-                                         Acc<T> &d1)
+                                         const T *lds, const unsigned short *lrp, const int (&rq)[4],
+                                         const EpiOps<T, DG> &eo, Acc<T> &d0, Acc<T> &d1)
 {
     constexpr int B = PRE ? 8 : 2;
     const int nr = rb.r1 - rb.r0;
@@ -285,13 +312,7 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
         // rows in the block's rperm order (longest first): slot t + q * TH
         // sums row rq[q], in CSR order as ever; its epilogue operand and
         // output sit at that row
-        T wr[kRowsPerThread], acc[kRowsPerThread];
-        DiagOp<T, DG> dg[kRowsPerThread];
-#pragma unroll
-        for (int q = 0; q < kRowsPerThread; ++q) {
-            wr[q] = reads_w<EPI>() ? a.w[rb.r0 + min(rq[q], nr - 1)] : T(0);
-            dg[q] = diag_load<T, DG>(a, rb.r0 + min(rq[q], nr - 1));
-        }
+        T acc[kRowsPerThread];
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
@@ -300,7 +321,7 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
-            if (t < nr) row_out<T, EPI>(a, rb.r0 + rq[q], diag_add<T, DG>(acc[q], dg[q]), wr[q], d0, d1);
+            if (t < nr) row_out<T, EPI>(a, rb.r0 + rq[q], diag_add<T, DG>(acc[q], eo.dg[q]), eo.wr[q], d0, d1);
         }
         return;
     }
@@ -310,8 +331,13 @@ __device__ __forceinline__ void sum_rows(const SpmvArgs<T> &a, const RowBlock &r
 #pragma unroll
         for (int q = 0; q < kRowsPerThread; ++q) {
             const int t = (int)threadIdx.x + q * TH;
-            wr[q] = q == 0 ? w0 : (reads_w<EPI>() ? a.w[rb.r0 + min(t, nr - 1)] : T(0));
-            dg[q] = diag_load<T, DG>(a, rb.r0 + min(t, nr - 1));
+            if constexpr (PRE) {
+                wr[q] = eo.wr[q];
+                dg[q] = eo.dg[q];
+            } else {
+                wr[q] = q == 0 ? w0 : (reads_w<EPI>() ? a.w[rb.r0 + min(t, nr - 1)] : T(0));
+                dg[q] = diag_load<T, DG>(a, rb.r0 + min(t, nr - 1));
+            }
         }
         // all sums first, then the stores: a store between two rows' LDS
         // reads makes the next row wait for every memory operation in flight
@@ -405,6 +431,16 @@ __device__ __forceinline__ void spmv_parts(const SpmvArgs<T> &a, int p, const Ac
     }
 }
 
+// Workgroup barrier ordering LDS accesses only.  __syncthreads' release
+// fence makes a wave wait for its global stores, and on CDNA stores and
+// loads share one counter, so it waits for every load in flight too.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // TH threads, NPT entries per thread per pass: a row block of up to
 // block_nnz entries takes ceil(block_nnz / (NPT * TH)) passes, each storing
 // its products in LDS, before the row sums.  The passes of a workgroup's
@@ -437,6 +473,10 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
     spmv_state<T, NW>(a, g, p, red, sh);
 
     Acc<T> d0, d1;
+    // PRE: the block's rperm rows (loaded with its first pass) and its
+    // epilogue operands
+    int rqv[kRowsPerThread] = {0, 0, 0, 0};
+    EpiOps<T, DG> eo;
     // One pass of the stream: the current pass's loads are in `in`; the next
     // pass's go to `out`.  Returns false after the workgroup's last block.
     auto step = [&](const PassLoad<T, NPT> &in, PassLoad<T, NPT> &out) -> bool {
@@ -459,19 +499,22 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // first pass of a block: its row pointers, staged in LDS below (PRE:
         // loaded with the pass, one step earlier)
         int rpv[kRowsPerThread];
-        int rqv[kRowsPerThread] = {0, 0, 0, 0};
         const bool stage = gather && cur.base == 0;
+        if constexpr (PRE) {
+            // every pass, before its scatter and the next pass's loads; the
+            // rperm rows come with a block's first pass (a long-row block
+            // loads no pass: its `in` is another block's, or unset)
+            if (a.rperm) {
+#pragma unroll
+                for (int q = 0; q < kRowsPerThread; ++q) rqv[q] = stage ? in.rq[q] : rqv[q];
+            }
+            epi_load<T, EPI, TH, DG>(a, cur.rb, cur.ax, rqv, eo);
+        }
         if (stage) {
 #pragma unroll
             for (int q = 0; q < kRowsPerThread; ++q)
                 rpv[q] = PRE ? in.rp[q]
                              : a.rowptr[cur.rb.r0 + min((int)threadIdx.x + q * TH, cur.rb.r1 - cur.rb.r0)];
-            if constexpr (PRE) {
-                if (a.rperm) {
-#pragma unroll
-                    for (int q = 0; q < kRowsPerThread; ++q) rqv[q] = in.rq[q];
-                }
-            }
         }
         Cursor nxt = cur;
         if (last) {
@@ -535,9 +578,11 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     if (threadIdx.x == 0) row_out<T, EPI>(a, cur.rb.r0, diag_add<T, DG>(total, dg), wr, d0, d1);
                 }
             } else {
-                __syncthreads();
-                sum_rows<T, EPI, TH, PRE, DG>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, d0, d1);
-                __syncthreads();   // LDS is reused by the next row block
+                // LDS-only barriers: the next pass's loads stay in flight
+                // through the row sums
+                lds_barrier();
+                sum_rows<T, EPI, TH, PRE, DG>(a, cur.rb, cur.ax, w0, lds, lrp, rqv, eo, d0, d1);
+                lds_barrier();   // LDS is reused by the next row block
             }
             if (more) w0 = issue_w0<T, EPI>(a, nxt.rb, nxt.ax);
         }
@@ -573,16 +618,6 @@ __device__ __forceinline__ BlockAux uniform_ax(BlockAux x)
     x.nchunks = __builtin_amdgcn_readfirstlane(x.nchunks);
     x.first = __builtin_amdgcn_readfirstlane(x.first);
     return x;
-}
-
-// Workgroup barrier ordering LDS accesses only.  __syncthreads' release
-// fence makes a wave wait for its global stores, and on CDNA stores and
-// loads share one counter, so it waits for every load in flight too.
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // One 3x3 block of a kind-3 row block per thread: its first column and its

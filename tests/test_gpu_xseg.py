@@ -98,6 +98,33 @@ def test_xseg_spmv_bit_exact(ctx, case, split):
     assert np.all(np.abs(y32 - ref32) <= bound * 1.01 + 1e-30)
 
 
+def test_xseg_two_pass_blocks_in_row_order(ctx):
+    """Row blocks of 16384 entries take two product passes each; the rows of
+    power-law blocks are summed in their rperm order, which the block's
+    first pass loads and its last pass uses: FP64 bit-exact on serial rows,
+    FP32 (diagonal apart) within its bound."""
+    rp, c, v = hbs.gen_circuit(60000, nheavy=12, heavy_lo=300, heavy_hi=30000, seed=11)
+    n = rp.size - 1
+    x = np.random.default_rng(23).uniform(-1, 1, n)
+    hbs.set_spmv_plan(block_nnz=16384)
+    try:
+        A = _forced(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    finally:
+        hbs.set_spmv_plan()
+    assert A.info()["xseg"] > 0
+    dx, dy = ctx.to_device(x), ctx.empty(n)
+    A.spmv(dx, dy)
+    _check_f64(rp, c, v, x, dy.download())
+    x32 = x.astype(np.float32)
+    dx32, dy32 = ctx.to_device(x32), ctx.empty(n, np.float32)
+    A.spmv(dx32, dy32)
+    ref32 = orc.spmv(orc.CSR(rp, c, v.astype(np.float32).astype(np.float64)), x32.astype(np.float64))
+    u = np.finfo(np.float32).eps / 2
+    k = np.diff(rp)
+    bound = 2 * (2 * k * u / (1 - k * u)) * orc.spmv(orc.CSR(rp, c, np.abs(v)), np.abs(x))
+    assert np.all(np.abs(dy32.download().astype(np.float64) - ref32) <= bound * 1.01 + 1e-30)
+
+
 def test_xseg_alpha_beta(ctx):
     rp, c, v = random_csr(5000, 5000, 0.003, 21)
     x = np.random.default_rng(4).standard_normal(5000)
