@@ -29,6 +29,7 @@ struct BlockWork {
     SolverState *hst = nullptr;   // pinned host [3][kBlockCols]
     double *part = nullptr;       // [kMaxDots][kBlockCols][kMaxBlockGroups], consumed before the next producer
     int ngroups = 0;              // a multiple of 8: rows are dealt per XCD
+    int sgroups = 0;              // the SpMMs' workgroups (a multiple of 8, at most ngroups; block_tune)
     int tile = 0, rb = 0;         // columns per group, SpMM rows per lane step (block_tune)
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;   // (first column, columns, chunk)
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -59,6 +60,7 @@ int pow2_ceil(int v)
 
 constexpr int kMaxBlockGroups = 2048;
 int g_block_tile = 0, g_block_rb = 0;   // forced block-solve tuning (0: tuned)
+int g_block_sgroups = 0;                 // forced SpMM workgroups of the block solve (0: tuned)
 
 // Workgroups of the streaming and SpMM kernels for groups of `tile`
 // columns: up to kMaxBlockGroups, a multiple of the 8 XCDs; a pass of 4
@@ -86,6 +88,7 @@ int block_work(hbs_solver s, int k, BlockWork **out)
     w->n = s->n;
     w->k = k;
     w->ngroups = block_groups(w->n, k);
+    w->sgroups = w->ngroups;
     const size_t nk = (size_t)w->n * k;
     int rc = HBS_OK;
     for (double **p : {&w->X, &w->R, &w->R0, &w->P, &w->V, &w->S, &w->T})
@@ -154,6 +157,7 @@ int block_tune(hbs_solver s, BlockWork *w, hipStream_t q)
         w->tile = tiles[0];
         w->rb = rr.run && rr.len % rbs[0] != 0 ? 2 : rbs[0];
         w->ngroups = block_groups(w->n, w->tile);
+        w->sgroups = g_block_sgroups > 0 ? std::min(g_block_sgroups, w->ngroups) : w->ngroups;
         return HBS_OK;
     }
     const size_t nk = (size_t)w->n * w->k;
@@ -166,12 +170,22 @@ int block_tune(hbs_solver s, BlockWork *w, hipStream_t q)
         return fail(HBS_ERR_HIP, "event creation failed");
     }
     double best = -1.0;
-    int bt = tiles[0], brb = 2, rc = HBS_OK;
+    int bt = tiles[0], brb = 2, bsg = block_groups(w->n, tiles[0]), rc = HBS_OK;
     for (int t : tiles) {
+        // SpMM workgroups: the streaming kernels' count, a half and a
+        // quarter of it (fewer rows in flight per XCD keep more of the X
+        // rows the sweep re-reads in its L2)
+        const int g0 = block_groups(w->n, t);
+        std::vector<int> sgs;
+        if (g_block_sgroups > 0) sgs.push_back(std::min(g_block_sgroups, g0));
+        else
+            for (int d = 1; d <= 4; d *= 2)
+                if (g0 / d >= 8 && (d == 1 || g0 / d >= 256) && (g0 / d) % 8 == 0) sgs.push_back(g0 / d);
+        for (int sg : sgs)
         for (int rb : rbs) {
             if (rr.run && rr.len % rb != 0) continue;
             BlockArgs a = group_args(s, w, 0, std::min(t, w->k));
-            a.ngroups = block_groups(w->n, t);
+            a.ngroups = sg;
             a.rb = rb;
             a.skip_done = 0;
             a.x = w->P;
@@ -196,6 +210,7 @@ int block_tune(hbs_solver s, BlockWork *w, hipStream_t q)
                 best = score;
                 bt = t;
                 brb = rb;
+                bsg = sg;
             }
         }
         if (rc != HBS_OK) break;
@@ -206,6 +221,7 @@ int block_tune(hbs_solver s, BlockWork *w, hipStream_t q)
     w->tile = bt;
     w->rb = brb;
     w->ngroups = block_groups(w->n, bt);
+    w->sgroups = bsg;
     return HBS_OK;
 }
 
@@ -213,19 +229,20 @@ hipError_t block_iteration(const BlockArgs &g, BlockWork *w, hipStream_t q)
 {
     hipError_t e;
     BlockArgs a = g;
-    if ((e = launch_block_vec(a, BVEC_P, q)) != hipSuccess) return e;
+    a.ngroups = w->sgroups;   // the SpMMs' workgroups, and the partials they write
+    if ((e = launch_block_vec(g, BVEC_P, q)) != hipSuccess) return e;
     a.x = g.P;
     a.y = g.V;
     a.w = g.R0;
     if ((e = launch_block_spmm(a, BEPI_DOT1, q)) != hipSuccess) return e;
-    if ((e = launch_block_fin(w->st, g.kc, w->part, w->ngroups, FIN_ALPHA, q)) != hipSuccess) return e;
+    if ((e = launch_block_fin(w->st, g.kc, w->part, w->sgroups, FIN_ALPHA, q)) != hipSuccess) return e;
     if ((e = launch_block_vec(g, BVEC_S, q)) != hipSuccess) return e;
     if ((e = launch_block_fin(w->st, g.kc, w->part, w->ngroups, FIN_S, q)) != hipSuccess) return e;
     a.x = g.S;
     a.y = g.T;
     a.w = g.S;
     if ((e = launch_block_spmm(a, BEPI_DOT2, q)) != hipSuccess) return e;
-    if ((e = launch_block_fin(w->st, g.kc, w->part, w->ngroups, FIN_OMEGA, q)) != hipSuccess) return e;
+    if ((e = launch_block_fin(w->st, g.kc, w->part, w->sgroups, FIN_OMEGA, q)) != hipSuccess) return e;
     if ((e = launch_block_vec(g, BVEC_XR, q)) != hipSuccess) return e;
     return launch_block_fin(w->st, g.kc, w->part, w->ngroups, FIN_XR, q);
 }
@@ -298,6 +315,22 @@ extern "C" int hbs_set_block_tuning(int tile, int rows_per_step)
     return HBS_OK;
 }
 
+extern "C" int hbs_set_block_spmm_groups(int groups)
+{
+    if (groups < 0 || groups % 8 != 0) return fail(HBS_ERR_ARG, "groups: 0 (tuned) or a positive multiple of 8");
+    g_block_sgroups = groups;
+    return HBS_OK;
+}
+
+extern "C" int hbs_block_spmm_groups(hbs_solver s, int k, int *groups)
+{
+    if (!s || !groups) return fail(HBS_ERR_ARG, "null argument");
+    const BlockWork *w = s->blk;
+    if (!w || w->k != k || w->tile == 0) return fail(HBS_ERR_ARG, "no block solve with k = %d tuned yet", k);
+    *groups = w->sgroups;
+    return HBS_OK;
+}
+
 extern "C" int hbs_block_tuning(hbs_solver s, int k, int *tile, int *rows_per_step, int *rows_per_wave)
 {
     if (!s || !tile || !rows_per_step || !rows_per_wave) return fail(HBS_ERR_ARG, "null argument");
@@ -337,19 +370,23 @@ extern "C" int hbs_bicgstab_block_f64(hbs_solver s, int k, const double *B, doub
         // R = B - A X ; R0 = R ; P = V = 0 ; iteration 0's rho and threshold
         BlockArgs a = g;
         a.skip_done = 0;
+        a.ngroups = w->sgroups;
         a.x = g.X;
         a.y = g.R;
         a.w = B + c0;
         HIPCHK(launch_block_spmm(a, BEPI_RESID, q));
+        a.ngroups = w->ngroups;
         HIPCHK(launch_block_vec(a, BVEC_INIT, q));
-        HIPCHK(launch_block_fin(w->st, kc, w->part, w->ngroups, FIN_XR, q));
+        HIPCHK(launch_block_fin(w->st, kc, w->part, w->sgroups, FIN_XR, q));   // the SpMM's partials
         CHK(block_iterate(g, w, c0, std::max(o.maxit, 1), o.check_every, o.use_graph != 0, q));
         // sqrt(||B - AX||^2 / ||B||^2) per column
         a.x = g.X;
         a.y = g.V;
         a.w = B + c0;
+        a.ngroups = w->sgroups;
         HIPCHK(launch_block_spmm(a, BEPI_RESID, q));
-        HIPCHK(launch_block_fin(w->st, kc, w->part, w->ngroups, FIN_RESID, q));
+        HIPCHK(launch_block_fin(w->st, kc, w->part, w->sgroups, FIN_RESID, q));
+        a.ngroups = w->ngroups;
         a.x = B + c0;
         HIPCHK(launch_block_vec(a, BVEC_DOT, q));
         HIPCHK(launch_block_fin(w->st, kc, w->part, w->ngroups, FIN_STORE, q));

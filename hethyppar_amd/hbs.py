@@ -154,6 +154,8 @@ _PROTOS = {
     "hbs_set_spmm_tuning": (_i, [_i, _i, _i, _i]),
     "hbs_set_block_tuning": (_i, [_i, _i]),
     "hbs_block_tuning": (_i, [_vp, _i, _ip, _ip, _ip]),
+    "hbs_block_spmm_groups": (_i, [_vp, _i, _ip]),
+    "hbs_set_block_spmm_groups": (_i, [_i]),
     "hbs_plan_grid_runs": (_i, [_i, _ip, _ip, _ip, _i, _ip, _ip]),
     "hbs_csr_spmm_runs": (_i, [_vp, _ip, _ip]),
     "hbs_csr_xseg": (_i, [_vp, _ip]),
@@ -741,6 +743,12 @@ class Solver:
         check(lib().hbs_block_tuning(self.h, k, C.byref(t), C.byref(r), C.byref(w)))
         return t.value, r.value, w.value
 
+    def block_spmm_groups(self, k):
+        """Workgroups of the block solve's SpMMs for k right-hand sides."""
+        g = C.c_int()
+        check(lib().hbs_block_spmm_groups(self.h, k, C.byref(g)))
+        return g.value
+
     def adjoint(self, g, x, lam, grad_val, opts=None, raise_on_error=True, **kw):
         info = SolveInfo()
         o = opts or default_opts(**kw)
@@ -851,6 +859,12 @@ def set_block_tuning(tile=0, rows_per_step=0):
     """Process-wide: force the block solve's column-group width and SpMM rows
     per lane step for workspaces created afterwards (0: tuned)."""
     check(lib().hbs_set_block_tuning(tile, rows_per_step))
+
+
+def set_block_spmm_groups(groups=0):
+    """Process-wide: workgroups of the block solve's SpMMs for workspaces
+    created afterwards (0: tuned; else a multiple of 8)."""
+    check(lib().hbs_set_block_spmm_groups(groups))
 
 
 def set_spmm_tuning(slab_lines=0, groups=0, dynamic=16, wide_loads=1):

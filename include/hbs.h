@@ -194,6 +194,15 @@ int hbs_bicgstab_block_f64(hbs_solver s, int k, const double *B_dev, double *X_d
  * at once ((64 / tile) * rows per step).  No reference counterpart. */
 int hbs_set_block_tuning(int tile, int rows_per_step);
 int hbs_block_tuning(hbs_solver s, int k, int *tile, int *rows_per_step, int *rows_per_wave);
+/* The tuner also times the SpMMs with the streaming kernels' workgroup
+ * count, a half and a quarter of it (fewer rows in flight per XCD keep
+ * more of the gathered X rows in its L2).  hbs_set_block_spmm_groups forces
+ * the count for workspaces created afterwards (0: tuned; else a multiple of
+ * 8, capped at the streaming kernels'); hbs_block_spmm_groups reports the
+ * workgroups the SpMMs of the block solve for k use.  No reference
+ * counterpart. */
+int hbs_set_block_spmm_groups(int groups);
+int hbs_block_spmm_groups(hbs_solver s, int k, int *groups);
 /* Mixed precision: FP32 BiCGStab sweeps (FP32 values and vectors, int32
  * indices, compensated dots) inside FP64 iterative refinement; FP64 for
  * r = b - A x, the scalar reductions and the convergence norm.  Needs
@@ -453,7 +462,8 @@ int hbs_set_spmm_order(int mode);
 int hbs_plan_grid_runs(int m, const int *rowptr, const int *col, int *runs, int cap, int *nruns, int *len);
 /* Tuning of the multi-column SpMM: y-lines per slab of the grid runs of
  * device CSRs created afterwards (0: from the L2 size), workgroups per SpMM
- * launch (0: 2048; else a multiple of 8), how the 64-column sweeps deal
+ * launch (0: 1024 for 64-column tiles gathered in 16-byte pieces, else 2048;
+ * else a multiple of 8), how the 64-column sweeps deal
  * their row steps for device CSRs created afterwards (> 0: each XCD's
  * waves take the next step from one of `dynamic` counters (1..16, default
  * 16), so they work on neighbouring rows; 0: a fixed stride per wave) and whether hbs_csr_spmm_* gathers X in

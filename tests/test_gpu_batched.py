@@ -169,6 +169,41 @@ def test_block_forced_tiles_match_oracle(ctx, tile, rb):
     assert len(loose) <= MAX_ORDER_SENSITIVE * k
 
 
+@pytest.mark.parametrize("sgroups", [512, 1024])
+def test_block_spmm_groups_forced_match_oracle(ctx, sgroups):
+    """The block solve's SpMMs on fewer workgroups than its streaming
+    kernels (their partials reduced over their own count) give each column
+    the oracle's solve."""
+    rp, c, v = hbs.gen_convdiff2d(128, 128, 0.6)
+    n, k = rp.size - 1, 64
+    Xt = hbs.gen_uniform(n * k, -1.0, 1.0, 9).reshape(n, k)
+    B = np.stack([hbs.host_spmv(rp, c, v, np.ascontiguousarray(Xt[:, j])) for j in range(k)], axis=1)
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    hbs.set_block_tuning(64, 4)
+    hbs.set_block_spmm_groups(sgroups)
+    try:
+        S = hbs.Solver(A)
+        dB = ctx.to_device(np.ascontiguousarray(B).ravel())
+        dX = ctx.to_device(np.zeros(n * k))
+        infos = S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=2000)
+        assert S.block_spmm_groups(k) == sgroups
+    finally:
+        hbs.set_block_tuning(0, 0)
+        hbs.set_block_spmm_groups(0)
+    X = dX.download().reshape(n, k)
+    Ao = orc.CSR(rp, c, v)
+    loose = []
+    for j in range(0, k, 4):
+        bj = np.ascontiguousarray(B[:, j])
+        xo, ito, _, _ = orc.bicgstab(Ao, bj, tol=1e-8, maxit=2000)
+        if _check(X[:, j], infos[j], xo, ito, f"spmm groups {sgroups} column {j}", Ao, bj,
+                  np.ascontiguousarray(Xt[:, j])) != "plain":
+            loose.append(j)
+    assert len(loose) <= MAX_ORDER_SENSITIVE * (k // 4)
+    for j in range(k):
+        assert infos[j].status == 0 and infos[j].relres <= 1.01e-8
+
+
 def test_block_autotuner_choice(ctx):
     """The tuner's choice is one of its candidates and fixed per workspace."""
     rp, c, v = hbs.gen_convdiff2d(64, 64, 0.5)
@@ -180,6 +215,8 @@ def test_block_autotuner_choice(ctx):
     S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=1000)
     tile, rb, rpw = S.block_tuning(k)
     assert tile in (32, 16, 8) and rb in (1, 2, 4) and rpw == (64 // tile) * rb
+    g = S.block_spmm_groups(k)
+    assert g > 0 and g % 8 == 0
     dX.zero()
     S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=1000)
     assert S.block_tuning(k) == (tile, rb, rpw)
