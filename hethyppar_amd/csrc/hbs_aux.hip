@@ -135,8 +135,16 @@ __global__ __launch_bounds__(kThreads) void k_spmm64(int m, const int *__restric
         const int cbase = (int)blockIdx.y * 64;
         wave_sweep64v<T>(rowptr, col, val, X + cbase, ld, min(64, k - cbase), rr, p0, x1, ps, sc.mine,
                          [&](int, int row, const Cols16<T> &acc, int cc) {
-                             if (cbase + cc < k)
-                                 *reinterpret_cast<Cols16<T> *>(Y + (size_t)row * ld + cbase + cc) = acc;
+                             // Y stored non-temporally, out of the L2 the
+                             // gathered X rows share (FP64 128^3 x 64: 0.81
+                             // -> 0.77 ms)
+                             if (cbase + cc < k) {
+                                 using V = T __attribute__((ext_vector_type(Cols16<T>::N)));
+                                 V vv;
+#pragma unroll
+                                 for (int e = 0; e < Cols16<T>::N; ++e) vv[e] = acc.v[e];
+                                 __builtin_nontemporal_store(vv, reinterpret_cast<V *>(Y + (size_t)row * ld + cbase + cc));
+                             }
                          });
     } else {
         sweep_span(sc, wv, nw, x0, RB, p0, ps);

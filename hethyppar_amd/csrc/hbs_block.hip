@@ -103,18 +103,21 @@ __global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a, const int *__re
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = 0.0;
     // epilogue of row rq of this lane's column (w = the row's operand)
+    // y is stored non-temporally: it is read back only by the next kernel,
+    // and kept out of the L2 it leaves room for the X rows the sweep
+    // gathers again
     auto emit = [&](int rq, double acc, double wr) {
         const size_t i = (size_t)rq * a.ld + m.c;
         if constexpr (EPI == BEPI_DOT1) {
-            a.y[i] = acc;
+            __builtin_nontemporal_store(acc, a.y + i);
             v[0] += wr * acc;
         } else if constexpr (EPI == BEPI_DOT2) {
-            a.y[i] = acc;
+            __builtin_nontemporal_store(acc, a.y + i);
             v[0] += acc * wr;
             v[1] += acc * acc;
         } else {
             const double o = wr - acc;
-            a.y[i] = o;
+            __builtin_nontemporal_store(o, a.y + i);
             v[0] += o * o;
             v[1] += o * o;
         }
