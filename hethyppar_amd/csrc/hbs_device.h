@@ -321,10 +321,12 @@ __device__ __forceinline__ void wave_sweep64(const int *__restrict__ rowptr, con
         T xv[RB][C], ev[RB][C];
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
+            // gathers past the row's entries (wave-uniform) are not issued
+            const int lq_len = q < nr ? rp[q + 1] - rp[q] : 0;
 #pragma unroll
             for (int i = 0; i < C; ++i) {
                 const int cj = __builtin_amdgcn_readlane(vc, q * C + i);
-                xv[q][i] = xl[(size_t)cj * ld];
+                xv[q][i] = i < lq_len ? xl[(size_t)cj * ld] : T(0);
                 if constexpr (sizeof(T) == 8) {
                     const long long bits = __builtin_bit_cast(long long, vv);
                     const int lo = __builtin_amdgcn_readlane((int)bits, q * C + i);
@@ -454,6 +456,15 @@ __device__ __forceinline__ void wave_sweep64v(const int *__restrict__ rowptr, co
             rpe = lq == q ? a1 : rpe;
         }
         const int len = lq < nr ? rpe - rpq : 0;
+        // the step's longest row (wave-uniform): gathers past it are not
+        // issued (a 7-point row leaves the 8th of a chunk to padding), since
+        // the vector-memory pipeline, not HBM, bounds this sweep
+        int lmax = 0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int lq_len = __builtin_amdgcn_readlane(vrp, q + 1) - __builtin_amdgcn_readlane(vrp, q);
+            lmax = q < nr ? max(lmax, lq_len) : lmax;
+        }
         int vc1;
         T vv1;
         load_cv(vrp1, vc1, vv1);
@@ -490,7 +501,10 @@ __device__ __forceinline__ void wave_sweep64v(const int *__restrict__ rowptr, co
                 cj = lq == q ? cq : cj;
                 e = lq == q ? eq : e;
             }
-            xv[i] = *reinterpret_cast<const Cols16<T> *>(xl + (size_t)cj * ld);
+            if (i < lmax) xv[i] = *reinterpret_cast<const Cols16<T> *>(xl + (size_t)cj * ld);
+            else
+#pragma unroll
+                for (int u = 0; u < N; ++u) xv[i].v[u] = T(0);
             ev[i] = e;
         }
         tk3 = tk4;

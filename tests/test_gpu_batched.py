@@ -204,6 +204,30 @@ def test_block_spmm_groups_forced_match_oracle(ctx, sgroups):
         assert infos[j].status == 0 and infos[j].relres <= 1.01e-8
 
 
+def test_block_fixed_dealing_reruns_bit_identical(ctx):
+    """With the 64-column sweep's steps at a fixed stride per wave (no step
+    counters), each workgroup sums the same rows every time, so the
+    per-column dots -- and the whole block solve -- repeat bit for bit; with
+    steps dealt from counters the rows a workgroup sums vary run to run."""
+    rp, c, v = hbs.gen_convdiff3d(128, 128, 4, 0.5)
+    n, k = rp.size - 1, 64
+    B = hbs.gen_uniform(n * k, -1.0, 1.0, 21)
+    hbs.set_spmm_tuning(0, 0, 0, 1)
+    try:
+        A = hbs.DeviceCSR(ctx, rp, c, v)
+    finally:
+        hbs.set_spmm_tuning()
+    assert A.info()["spmm_runs"] > 0
+    S = hbs.Solver(A)
+    out = []
+    for _ in range(2):
+        dB, dX = ctx.to_device(B), ctx.to_device(np.zeros(n * k))
+        infos = S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=2000)
+        assert all(i.status == 0 and i.relres <= 1.01e-8 for i in infos)
+        out.append((dX.download(), [i.iters for i in infos]))
+    assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
+
+
 def test_block_autotuner_choice(ctx):
     """The tuner's choice is one of its candidates and fixed per workspace."""
     rp, c, v = hbs.gen_convdiff2d(64, 64, 0.5)
