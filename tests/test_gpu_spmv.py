@@ -208,7 +208,7 @@ def test_spmm_tiny_matrix_few_waves(ctx):
             assert np.array_equal(Y[:, j], orc.spmv(Ao, X[:, j])), f"launch {rep} column {j}"
 
 
-@pytest.mark.parametrize("k,tile", [(64, 64), (64, 16), (24, 8)])
+@pytest.mark.parametrize("k,tile", [(64, 64), (40, 64), (64, 16), (24, 8)])
 def test_spmm_grid_runs_bit_exact(ctx, k, tile):
     """A 3-D grid operator whose plane outgrows an XCD's L2 is swept in
     y-slab runs: every column still equals the oracle's serial SpMV, and the
