@@ -415,6 +415,13 @@ extern "C" int hbs_set_spmv_xseg(int mode)
     return HBS_OK;
 }
 
+extern "C" int hbs_set_xseg_fuse(int mode)
+{
+    if (mode < 0 || mode > 1) return fail(HBS_ERR_ARG, "mode must be 0 or 1");
+    g_xseg_fuse = mode;
+    return HBS_OK;
+}
+
 extern "C" int hbs_set_spmv_diag_split(int mode)
 {
     if (mode < 0 || mode > 1) return fail(HBS_ERR_ARG, "mode must be 0 or 1");

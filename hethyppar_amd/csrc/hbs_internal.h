@@ -177,7 +177,23 @@ struct XsegArgs {
     const T *x;
     T *prod;
     const SolverState *st;       // null: always run; else skip when st->done
+    // Solver iterations: the vector the products read, updated by the pass
+    // itself (one system).  XFUSE_P: x = P = ((P - V omega) beta) + R after
+    // FIN_XR of the previous iteration's (r, r0), ||r||^2 partials;
+    // XFUSE_S: x = S = R - V alpha after FIN_ALPHA of the (r0, v) partials,
+    // with a ||S||^2 partial per workgroup.  Each segment's first share
+    // writes its elements of the vector (xv_out, may be x) and its partial
+    // (the others 0); workgroup 0 writes the state it derived to st_out.
+    int fuse;
+    T *xv_out;
+    const T *R, *V;
+    const SolverState *st_in;
+    SolverState *st_out;
+    const double *part_in;
+    int nparts_in, pin0, pin1;   // partials [pin0, pin1), nparts_in apart per dot
+    double *part_out;            // XFUSE_S: [nseg * split]
 };
+enum XsegFuse : int { XFUSE_NONE = 0, XFUSE_P = 1, XFUSE_S = 2 };
 
 template <typename T>
 struct ChunkArgs {
@@ -356,6 +372,7 @@ extern int g_grid_slab;       // y-lines per slab of plan_grid_runs (0: from the
 extern int g_spmm_groups;     // workgroups of the multi-column SpMM (0: 2048)
 extern int g_spmm_v16;        // 64-column SpMM: 16-byte gathers where aligned (1) or 8/4-byte (0)
 extern int g_spmm_counters;   // step counters per XCD of the dealt 64-column sweeps
+extern int g_xseg_fuse;       // solvers created afterwards: P and S updates in x-segmented product passes
 int vec_packs_per_thread(int elems_per_block, int elem_bytes);
 
 // ---- launchers (hbs_kernels.hip) ----

@@ -793,31 +793,139 @@ __device__ __forceinline__ void xseg_issue(const XsegArgs<T> &a, int g0, int lim
     }
 }
 
-template <typename T, int NPT>
+// Solver prologue of a fused product pass (XsegArgs::fuse): the state
+// after the pending reduction (FIN_XR for XFUSE_P, FIN_ALPHA for XFUSE_S),
+// derived by every workgroup from the same partials in the same order;
+// workgroup 0 hands it on.  Returns whether the system is done.
+template <typename T, int FUSE>
+__device__ __forceinline__ bool xseg_state(const XsegArgs<T> &a, SolverState &sh, double *red)
+{
+    constexpr int NW = kWideThreads / 64;
+    if (threadIdx.x == 0) sh = *a.st_in;
+    __syncthreads();
+    const bool act = FUSE == XFUSE_P ? (!sh.done && sh.pending) : !sh.done;
+    if (act) {
+        double v[2] = {0.0, 0.0};
+        if constexpr (FUSE == XFUSE_P) {
+            reduce_parts<2, NW>(a.part_in, a.nparts_in, a.pin0, a.pin1, v, red);
+        } else {
+            double v1[1];
+            reduce_parts<1, NW>(a.part_in, a.nparts_in, a.pin0, a.pin1, v1, red);
+            v[0] = v1[0];
+        }
+        if (threadIdx.x == 0) apply_fin(sh, FUSE == XFUSE_P ? FIN_XR : FIN_ALPHA, v[0], v[1]);
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.st_out = sh;
+    return sh.done != 0;
+}
+
+// Loads of one round of a fused segment update: U elements per thread,
+// i0 + u * kWideThreads (clamped into the segment) of R, V (and P).
+template <typename T>
+constexpr int stage_unroll()
+{
+    return 4;
+}
+template <typename T, int FUSE>
+struct StageLoad {
+    static constexpr int U = stage_unroll<T>();
+    T r[U], v[U], p[U];
+};
+template <typename T, int FUSE>
+__device__ __forceinline__ void stage_issue(const XsegArgs<T> &a, int c0, int nc, int i0, StageLoad<T, FUSE> &L)
+{
+#pragma unroll
+    for (int u = 0; u < StageLoad<T, FUSE>::U; ++u) {
+        const int idx = c0 + min(i0 + u * kWideThreads, nc - 1);
+        L.r[u] = a.R[idx];
+        L.v[u] = a.V[idx];
+        L.p[u] = FUSE == XFUSE_P ? a.x[idx] : T(0);
+    }
+}
+
+// The segment's x staged in LDS.  Fused: each element of the vector is
+// updated here (the same arithmetic and roundings as k_vec_P / k_vec_S),
+// written out by the segment's first share and, for XFUSE_S, its square
+// added to this workgroup's ||S||^2 partial (written by every workgroup, 0
+// by the other shares).  The first round's loads (L) were issued before the
+// state prologue; each later round loads after the previous one is stored.
+template <typename T, int FUSE>
+__device__ __forceinline__ void xseg_stage(const XsegArgs<T> &a, const SolverState &sh, T *xs, int c0, int nc,
+                                           bool first_share, int w, double *red, StageLoad<T, FUSE> &L)
+{
+    if constexpr (FUSE == XFUSE_NONE) {
+        for (int i = (int)threadIdx.x; i < nc; i += kWideThreads) xs[i] = a.x[c0 + i];
+    } else {
+        constexpr int NW = kWideThreads / 64, U = StageLoad<T, FUSE>::U;
+        const T om = (T)sh.omega, be = (T)sh.beta, al = (T)sh.alpha;
+        Acc<T> d;
+        for (int i0 = (int)threadIdx.x;;) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * kWideThreads;
+                if (i < nc) {
+                    T o;
+                    if constexpr (FUSE == XFUSE_P) o = ((L.p[u] - L.v[u] * om) * be) + L.r[u];
+                    else o = L.r[u] - L.v[u] * al;
+                    xs[i] = o;
+                    if (first_share) {
+                        a.xv_out[c0 + i] = o;
+                        if constexpr (FUSE == XFUSE_S) d.add(o, o);
+                    }
+                }
+            }
+            i0 += U * kWideThreads;
+            if (i0 >= nc) break;
+            stage_issue<T, FUSE>(a, c0, nc, i0, L);
+        }
+        if constexpr (FUSE == XFUSE_S) {
+            double v1[1] = {d.get()};
+            block_sum<1, NW>(v1, red);
+            if (threadIdx.x == 0) a.part_out[w] = v1[0];
+        }
+    }
+}
+
+template <typename T, int NPT, int FUSE>
 __global__ __launch_bounds__(kWideThreads) void k_xseg(XsegArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *xs = reinterpret_cast<T *>(smem);
     int *offl = reinterpret_cast<int *>(smem + (size_t)a.seg_cols * sizeof(T));
     int *gchl = offl + a.max_chunks;
+    __shared__ double red[kMaxDots * (kWideThreads / 64)];
+    __shared__ SolverState sh;
     // workgroup w takes share w % split of segment w / split (the shares of
     // one segment and neighbouring segments on one XCD); entries f in
     // [f0, f1) of the segment, numbered from its first entry
     const int w = xcd_remap(blockIdx.x, a.nseg * a.split);
     const int s = w / a.split, part = w - s * a.split;
-    if (a.st && a.st->done) return;
+    if constexpr (FUSE == XFUSE_NONE) {
+        if (a.st && a.st->done) return;
+    }
     const int g0 = a.sbase[s], cnt = a.sbase[s + 1] - g0;
     const int f0 = (int)((long long)cnt * part / a.split), f1 = (int)((long long)cnt * (part + 1) / a.split);
-    if (f1 <= f0) return;
+    // fused: every share updates the segment (its first writes it) and
+    // writes its partial, with or without entries of its own
+    if (FUSE == XFUSE_NONE && f1 <= f0) return;
     XLoad<T, NPT> LA, LB;
-    xseg_issue<T, NPT>(a, g0, f1, f0, LA);
     const int c0 = s * a.seg_cols, nc = min(a.seg_cols, a.n - c0);
-    for (int i = (int)threadIdx.x; i < nc; i += kWideThreads) xs[i] = a.x[c0 + i];
+    StageLoad<T, FUSE> L;
+    if constexpr (FUSE != XFUSE_NONE) {
+        // the update's first loads overlap the state prologue; the entries'
+        // first pass loads behind it (fewer registers live across it)
+        stage_issue<T, FUSE>(a, c0, nc, (int)threadIdx.x, L);
+        if (xseg_state<T, FUSE>(a, sh, red)) return;
+    }
+    xseg_issue<T, NPT>(a, g0, f1, f0, LA);
+    xseg_stage<T, FUSE>(a, sh, xs, c0, nc, part == 0, w, red, L);
     const int cb = a.cbase[s], ncs = a.cbase[s + 1] - cb;
     for (int i = (int)threadIdx.x; i < ncs; i += kWideThreads) offl[i] = a.off[cb + i];
     const int gb = a.gbase[s], ngs = a.gbase[s + 1] - gb;
     for (int i = (int)threadIdx.x; i < ngs; i += kWideThreads) gchl[i] = a.gch[gb + i];
     __syncthreads();
+    if (FUSE != XFUSE_NONE && f1 <= f0) return;
     T *__restrict__ prod = a.prod + g0;
     constexpr int PASS = NPT * kWideThreads;
     auto step = [&](const XLoad<T, NPT> &in, XLoad<T, NPT> &out, int base) -> bool {
@@ -1393,9 +1501,15 @@ size_t xseg_lds_bytes(int elem_bytes, int max_chunks, int max_granules)
 }
 hipError_t prepare_xseg(int max_chunks, int max_granules)
 {
-    hipError_t e = raise_lds_limit((const void *)k_xseg<double, xseg_npt<double>()>, xseg_lds_bytes(8, max_chunks, max_granules));
-    if (e == hipSuccess)
-        e = raise_lds_limit((const void *)k_xseg<float, xseg_npt<float>()>, xseg_lds_bytes(4, max_chunks, max_granules));
+    hipError_t e = hipSuccess;
+    const void *k[] = {(const void *)k_xseg<double, xseg_npt<double>(), XFUSE_NONE>,
+                       (const void *)k_xseg<double, xseg_npt<double>(), XFUSE_P>,
+                       (const void *)k_xseg<double, xseg_npt<double>(), XFUSE_S>,
+                       (const void *)k_xseg<float, xseg_npt<float>(), XFUSE_NONE>,
+                       (const void *)k_xseg<float, xseg_npt<float>(), XFUSE_P>,
+                       (const void *)k_xseg<float, xseg_npt<float>(), XFUSE_S>};
+    for (int i = 0; i < 6 && e == hipSuccess; ++i)
+        e = raise_lds_limit(k[i], xseg_lds_bytes(i < 3 ? 8 : 4, max_chunks, max_granules));
     return e;
 }
 template <typename T>
@@ -1404,7 +1518,19 @@ hipError_t xseg_dispatch(const XsegArgs<T> &a, hipStream_t s)
     if (a.nseg <= 0) return hipSuccess;
     if (a.seg_cols != kSegCols || a.split < 1 || (long long)a.nseg * a.split > INT_MAX) return hipErrorInvalidValue;
     const size_t lds = xseg_lds_bytes((int)sizeof(T), a.max_chunks, a.max_granules);
-    k_xseg<T, xseg_npt<T>()><<<a.nseg * a.split, kWideThreads, lds, s>>>(a);
+    const int grid = a.nseg * a.split;
+    switch (a.fuse) {
+    case XFUSE_NONE: k_xseg<T, xseg_npt<T>(), XFUSE_NONE><<<grid, kWideThreads, lds, s>>>(a); break;
+    case XFUSE_P:
+        if (!a.st_in || !a.st_out || !a.R || !a.V || !a.xv_out) return hipErrorInvalidValue;
+        k_xseg<T, xseg_npt<T>(), XFUSE_P><<<grid, kWideThreads, lds, s>>>(a);
+        break;
+    case XFUSE_S:
+        if (!a.st_in || !a.st_out || !a.R || !a.V || !a.xv_out || !a.part_out) return hipErrorInvalidValue;
+        k_xseg<T, xseg_npt<T>(), XFUSE_S><<<grid, kWideThreads, lds, s>>>(a);
+        break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 hipError_t launch_xseg(const XsegArgs<double> &a, hipStream_t s) { return xseg_dispatch<double>(a, s); }

@@ -188,6 +188,11 @@ struct hbs_solver_s {
     double *chunk_sums = nullptr; // long-row chunk sums
     static constexpr int kGraphLens = 33;             // chunks of 1..32 iterations
     hipGraphExec_t graph[2][kGraphLens][2] = {};       // [FP64, FP32][chunk length][poll slot]
+    // x-segmented iterations with the P and S updates in the product passes:
+    // per precision, the SpMV's workgroups and the product pass's partials
+    // ({g0, g1, 0, nseg * split}) for the ||s|| test; the partials buffer
+    hbs::SysRange *sysx[2] = {nullptr, nullptr};
+    double *partX = nullptr;
     int graph_diag_gen = 0;                            // A->diag_gen the FP32 graphs were captured at
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -262,7 +267,8 @@ SpmvArgs<T> spmv_args(hbs_csr a, const T *val)
 // Queue one SpMV: for x-segmented matrices the product pass, then the
 // long-row chunk sums (if any), then the row blocks.
 template <typename T>
-hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a_in, int epi, const SolverState *chunk_st, hipStream_t q)
+hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a_in, int epi, const SolverState *chunk_st, hipStream_t q,
+                    const XsegArgs<T> *fuse = nullptr)
 {
     const hbs_csr a = spmv_mat<T>(a_in);
     if (a->xs_nseg > 0) {
@@ -285,6 +291,20 @@ hipError_t run_spmv(const SpmvArgs<T> &s, const hbs_csr a_in, int epi, const Sol
         x.x = s.x;
         x.prod = const_cast<T *>(s.prod);
         x.st = chunk_st;
+        if (fuse) {
+            // the product pass updates its x (a solver iteration's P or S)
+            x.fuse = fuse->fuse;
+            x.xv_out = fuse->xv_out;
+            x.R = fuse->R;
+            x.V = fuse->V;
+            x.st_in = fuse->st_in;
+            x.st_out = fuse->st_out;
+            x.part_in = fuse->part_in;
+            x.nparts_in = fuse->nparts_in;
+            x.pin0 = fuse->pin0;
+            x.pin1 = fuse->pin1;
+            x.part_out = fuse->part_out;
+        }
         if (!x.xval || !x.prod) return hipErrorInvalidValue;
         hipError_t e = launch_xseg(x, q);
         if (e != hipSuccess) return e;

@@ -435,6 +435,7 @@ int g_grid_slab = 0;
 int g_spmm_groups = 0;
 int g_spmm_v16 = 1;
 int g_spmm_counters = kCtrPerXcd;
+int g_xseg_fuse = 1;
 
 // Elements per vector workgroup: 512 * u with u in {1, 2, 4, 8}, the
 // smallest that keeps the workgroups (and the partials a consumer sums) at

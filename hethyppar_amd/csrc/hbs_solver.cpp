@@ -46,8 +46,9 @@ extern "C" int hbs_solver_destroy(hbs_solver s)
     (void)hipStreamSynchronize(s->A->ctx->stream);
     for (double **p : {&s->X, &s->R, &s->R0, &s->P, &s->V, &s->S, &s->T, &s->Xb}) dfree(*p);
     for (float **p : {&s->X32, &s->R32, &s->R032, &s->P32, &s->V32, &s->S32, &s->T32}) dfree(*p);
-    for (double **p : {&s->partA, &s->partB, &s->partC, &s->partD, &s->partE, &s->partF, &s->chunk_sums})
+    for (double **p : {&s->partA, &s->partB, &s->partC, &s->partD, &s->partE, &s->partF, &s->chunk_sums, &s->partX})
         dfree(*p);
+    for (hbs::SysRange *&r : s->sysx) dfree(r);
     dfree(s->st);
     if (s->hst) (void)hipHostFree(s->hst);
     dfree(s->vblocks);
@@ -116,6 +117,19 @@ extern "C" int hbs_solver_create(hbs_csr a, hbs_solver *out)
         return bail(rc);
     for (hipEvent_t *e : {&s->ev[0], &s->ev[1], &s->e0, &s->e1})
         if (hipEventCreate(e) != hipSuccess) return bail(fail(HBS_ERR_HIP, "event creation failed"));
+    // x-segmented iterations with the P and S updates in the product passes
+    if (s->nsys == 1 && g_xseg_fuse) {
+        int nx = 0;
+        for (int gi = 0; gi < 2; ++gi) {
+            const hbs_csr M = gi == 0 ? a : (a->od ? a->od : a);
+            if (M->xs_nseg <= 0 || (gi == 1 && !a->val32)) continue;
+            const int nxw = M->xs_nseg * M->xs_split;
+            std::vector<SysRange> r{SysRange{M->sys_host[0].g0, M->sys_host[0].g1, 0, nxw}};
+            if ((rc = dalloc(&s->sysx[gi], 1)) || (rc = upload(s->sysx[gi], r))) return bail(rc);
+            nx = std::max(nx, nxw);
+        }
+        if (nx > 0 && (rc = dalloc(&s->partX, (size_t)nx))) return bail(rc);
+    }
     *out = s;
     return HBS_OK;
 }
@@ -169,10 +183,88 @@ SpmvArgs<T> solver_spmv(hbs_solver s, const T *val)
     return sa;
 }
 
+// Whether T's iterations update P and S in the product passes of an
+// x-segmented SpMV (k_xseg with XFUSE_P / XFUSE_S) instead of k_vec_P and
+// k_vec_S: one x-segmented system, hbs_set_xseg_fuse on (at creation).
+template <typename T>
+bool xseg_fused(hbs_solver s)
+{
+    return s->sysx[sizeof(T) == 4 ? 1 : 0] != nullptr && s->partX != nullptr;
+}
+
+// One BiCGStab iteration of an x-segmented system with the P and S updates
+// in the product passes: the same steps, scalars and state slots as below,
+// with the ||s|| partials from the product pass of T = A S.
+template <typename T>
+hipError_t launch_iteration_fused(hbs_solver s, const Work<T> &w)
+{
+    hipStream_t q = s->A->ctx->stream;
+    SolverState *A = s->slot(kSlotA), *B = s->slot(kSlotB);
+    const hbs_csr M = spmv_mat<T>(s->A);
+    const int ng = M->plan.ngroups, nvb = nvb_of<T>(s), nxw = M->xs_nseg * M->xs_split;
+    hipError_t e;
+    // FIN_XR of the previous iteration, P = ((P - V w) b) + R in the product
+    // pass ; V = A P ; (r0, v)
+    XsegArgs<T> f{};
+    f.fuse = XFUSE_P;
+    f.xv_out = w.P;
+    f.R = w.R;
+    f.V = w.V;
+    f.st_in = A;
+    f.st_out = B;
+    f.part_in = s->partD;
+    f.nparts_in = nvb;
+    f.pin0 = 0;
+    f.pin1 = nvb;
+    SpmvArgs<T> sa = solver_spmv<T>(s, w.val);
+    sa.x = w.P;
+    sa.y = w.V;
+    sa.w = w.R0;
+    sa.st_in = B;
+    sa.check_done = 1;
+    sa.part_out = s->partA;
+    if ((e = run_spmv(sa, s->A, EPI_DOT1, B, q, &f)) != hipSuccess) return e;
+    // alpha ; S = R - V a in the product pass, ||S||^2 ; T = A S ; ||s||
+    // test ; (t, s), (t, t)
+    XsegArgs<T> g{};
+    g.fuse = XFUSE_S;
+    g.xv_out = w.S;
+    g.R = w.R;
+    g.V = w.V;
+    g.st_in = B;
+    g.st_out = A;
+    g.part_in = s->partA;
+    g.nparts_in = ng;
+    g.pin0 = M->sys_host[0].g0;
+    g.pin1 = M->sys_host[0].g1;
+    g.part_out = s->partX;
+    sa.x = w.S;
+    sa.y = w.T_;
+    sa.w = w.S;
+    sa.st_in = A;
+    sa.st_out = B;
+    sa.fin_in = FIN_S;
+    sa.part_in = s->partX;
+    sa.nparts_in = nxw;
+    sa.sys = s->sysx[sizeof(T) == 4 ? 1 : 0];
+    sa.part_out = s->partC;
+    if ((e = run_spmv(sa, s->A, EPI_DOT2, A, q, &g)) != hipSuccess) return e;
+    // omega ; X, R ; (r, r0), ||r||^2
+    VecArgs<T> va = vec_args<T>(s, w);
+    va.st_in = B;
+    va.st_out = A;
+    va.part_in = s->partC;
+    va.nparts_in = ng;
+    va.in_spmv = 1;
+    va.part_out = s->partD;
+    return launch_vec_XR(va, q);
+}
+
 // One BiCGStab iteration, state slots alternating A -> B -> A.
 template <typename T>
 hipError_t launch_iteration(hbs_solver s, const Work<T> &w)
 {
+    if (xseg_fused<T>(s)) return launch_iteration_fused<T>(s, w);
     hipStream_t q = s->A->ctx->stream;
     SolverState *A = s->slot(kSlotA), *B = s->slot(kSlotB);
     const int ng = spmv_mat<T>(s->A)->plan.ngroups;

@@ -160,6 +160,7 @@ _PROTOS = {
     "hbs_csr_spmm_runs": (_i, [_vp, _ip, _ip]),
     "hbs_csr_xseg": (_i, [_vp, _ip]),
     "hbs_set_spmv_diag_split": (_i, [_i]),
+    "hbs_set_xseg_fuse": (_i, [_i]),
     "hbs_csr_diag_split": (_i, [_vp, _ip]),
     "hbs_shard_pack_host": (_i, [_vp, _vp, _dp]),
     "hbs_vector_part": (_i, [_i, _dp, _ip, _i, _i, _dp, _ip]),
@@ -885,6 +886,12 @@ def plan_grid_runs(rowptr, col):
     check(lib().hbs_plan_grid_runs(m, rp.ctypes.data_as(_ip), c.ctypes.data_as(_ip), out.ctypes.data_as(_ip),
                                    int(out.size), C.byref(nr), C.byref(ln)))
     return out[:nr.value].copy(), ln.value
+
+
+def set_xseg_fuse(mode=1):
+    """Process-wide: solvers created afterwards update P and S inside the
+    x-segmented product passes (1) or in their own vector passes (0)."""
+    check(lib().hbs_set_xseg_fuse(mode))
 
 
 def set_spmv_diag_split(mode=1):

@@ -433,6 +433,14 @@ int hbs_csr_xseg(hbs_csr a, int *nseg);
  * split, 1 split with a unit diagonal, 2 split with diagonal values (also
  * after hbs_csr_set_values on a matrix split with a unit diagonal). */
 int hbs_set_spmv_diag_split(int mode);
+/* BiCGStab on an x-segmented system (solvers created afterwards): 1
+ * (default) updates P and S inside the product passes of the SpMVs that
+ * read them -- each x segment's workgroup derives the iteration's scalars
+ * from the partials, updates its segment, writes it and stages it -- in
+ * place of the separate P and S vector passes; 0 keeps those passes.  The
+ * iterates are the same either way; only ||s||^2 (the ||s|| stop test) is
+ * summed over other partitions. */
+int hbs_set_xseg_fuse(int mode);
 int hbs_csr_diag_split(hbs_csr a, int *mode);
 
 /* 3x3-block matrices (every block row's three rows share one list of

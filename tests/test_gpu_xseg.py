@@ -272,6 +272,40 @@ def test_xseg_fp32_diagonal_apart_off_matches_on(ctx):
 
 
 @pytest.mark.parametrize("graph", [0, 1])
+def test_xseg_fused_updates_same_iterates(ctx, graph):
+    """With P and S updated inside the x-segmented product passes the FP64
+    and the mixed solves take the same iterates as with the separate vector
+    passes (the same arithmetic; only ||s||^2, for the ||s|| test, is summed
+    over other partitions): bit-identical x, same iteration counts."""
+    rp, c, v = hbs.gen_circuit(150000, nheavy=10, heavy_lo=1000, heavy_hi=20000, seed=5)
+    n = rp.size - 1
+    xt = hbs.gen_uniform(n, -1.0, 1.0, 17)
+    b = hbs.host_spmv(rp, c, v, xt)
+    A = _forced(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    out = {}
+    for fuse in (1, 0):
+        hbs.set_xseg_fuse(fuse)
+        try:
+            S = hbs.Solver(A)
+        finally:
+            hbs.set_xseg_fuse()
+        db, dx = ctx.to_device(b), ctx.empty(n)
+        dx.zero()
+        i64 = S.bicgstab(db, dx, tol=1e-8, maxit=2000, use_graph=graph)
+        x64 = dx.download()
+        dx.zero()
+        im = S.bicgstab_mixed(db, dx, tol=1e-8, maxit=4000, use_graph=graph)
+        out[fuse] = (x64, i64.iters, i64.relres, dx.download(), im.iters, im.sweeps)
+        print(f"fuse {fuse}: fp64 {i64.iters} its relres {i64.relres:.3e}; mixed {im.iters} its "
+              f"{im.sweeps} sweeps relres {im.relres:.3e}", flush=True)
+        assert i64.status == 0 and im.status == 0
+        S.close()
+    f, u = out[1], out[0]
+    assert f[1:3] == u[1:3] and np.array_equal(f[0], u[0])
+    assert f[4:] == u[4:] and np.array_equal(f[3], u[3])
+
+
+@pytest.mark.parametrize("graph", [0, 1])
 def test_xseg_bicgstab_fp64_and_mixed(ctx, graph):
     rp, c, v = hbs.gen_circuit(150000, nheavy=10, heavy_lo=1000, heavy_hi=20000, seed=3)
     n = rp.size - 1
