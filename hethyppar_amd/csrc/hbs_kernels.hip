@@ -825,7 +825,7 @@ __device__ __forceinline__ bool xseg_state(const XsegArgs<T> &a, SolverState &sh
 template <typename T>
 constexpr int stage_unroll()
 {
-    return 4;
+    return sizeof(T) == 4 ? 8 : 4;
 }
 template <typename T, int FUSE>
 struct StageLoad {
