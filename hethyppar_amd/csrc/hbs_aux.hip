@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm64(int m, const int *__restric
         sweep_span(sc, wv, nw, x0, G, p0, ps);
         const int cbase = (int)blockIdx.y * 64;
         wave_sweep64v<T>(rowptr, col, val, X + cbase, ld, min(64, k - cbase), rr, p0, x1, ps, sc.mine,
-                         [&](int, int row, const Cols16<T> &acc, int cc) {
+                         [](int, int) {}, [&](int, int row, const Cols16<T> &acc, int cc) {
                              // Y stored non-temporally, out of the L2 the
                              // gathered X rows share (FP64 128^3 x 64: 0.81
                              // -> 0.77 ms)

@@ -31,6 +31,7 @@ struct BlockWork {
     int ngroups = 0;              // a multiple of 8: rows are dealt per XCD
     int sgroups = 0;              // the SpMMs' workgroups (a multiple of 8, at most ngroups; block_tune)
     int tile = 0, rb = 0;         // columns per group, SpMM rows per lane step (block_tune)
+    int v16 = 0;                  // 64-column groups gathered in 16-byte pieces (block_tune)
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;   // (first column, columns, chunk)
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
@@ -71,6 +72,17 @@ int block_groups(int n, int tile)
     const long long pass = (long long)kWaves * (64 / kp);
     const long long want = std::min<long long>(kMaxBlockGroups, (n + pass - 1) / pass);
     return (int)std::max<long long>(8, (want + 7) / 8 * 8);
+}
+
+// Whether the SpMMs of groups of `tile` columns (k in all) gather X in
+// 16-byte pieces, two columns per lane (k_bspmm2): 64-column groups, k
+// even (so every group's first column and width, and the row length, keep
+// the column pairs 16-byte aligned), wide loads on (hbs_set_spmm_tuning)
+// and no forced rows per lane step (those are the one-column-per-lane
+// form's).
+bool block_v16(int k, int tile)
+{
+    return g_spmm_v16 && g_block_rb == 0 && pow2_ceil(std::min(tile, kBlockCols)) == 64 && k % 2 == 0;
 }
 
 int block_work(hbs_solver s, int k, BlockWork **out)
@@ -131,13 +143,15 @@ BlockArgs group_args(hbs_solver s, BlockWork *w, int c0, int kc)
     a.part = w->part;
     a.rr = s->A->row_runs();
     a.rb = w->rb > 0 ? w->rb : 2;
+    a.v16 = w->v16 && a.kp == 64 && kc % 2 == 0 && c0 % 2 == 0 && a.ld % 2 == 0;
     return a;
 }
 
 // The block solve's autotuner: for k right-hand sides, the column-group
 // width (the SpMM's RHS tile: 64, 32, 16 ... columns, at most k rounded up
 // to a power of two) and the SpMM rows per lane step (1, 2, 4; with the
-// tile's 64 / tile rows per wave, the rows a wavefront works on at once).
+// tile's 64 / tile rows per wave, the rows a wavefront works on at once;
+// 64-column tiles in the 16-byte form always take two rows per step).
 // Each candidate times the (r0, v) SpMM of one group, best of 3 after a
 // warm-up, on the workspace's own blocks; the score is that time times the
 // groups the k columns need.  Forced choices (hbs_set_block_tuning) skip the
@@ -156,6 +170,7 @@ int block_tune(hbs_solver s, BlockWork *w, hipStream_t q)
     if (tiles.size() == 1 && rbs.size() == 1) {
         w->tile = tiles[0];
         w->rb = rr.run && rr.len % rbs[0] != 0 ? 2 : rbs[0];
+        w->v16 = block_v16(w->k, w->tile);
         w->ngroups = block_groups(w->n, w->tile);
         w->sgroups = g_block_sgroups > 0 ? std::min(g_block_sgroups, w->ngroups) : w->ngroups;
         return HBS_OK;
@@ -181,12 +196,14 @@ int block_tune(hbs_solver s, BlockWork *w, hipStream_t q)
         else
             for (int d = 1; d <= 4; d *= 2)
                 if (g0 / d >= 8 && (d == 1 || g0 / d >= 256) && (g0 / d) % 8 == 0) sgs.push_back(g0 / d);
+        const bool v16 = block_v16(w->k, t);
         for (int sg : sgs)
-        for (int rb : rbs) {
+        for (int rb : v16 ? std::vector<int>{2} : rbs) {
             if (rr.run && rr.len % rb != 0) continue;
             BlockArgs a = group_args(s, w, 0, std::min(t, w->k));
             a.ngroups = sg;
             a.rb = rb;
+            a.v16 = v16 && a.kp == 64;
             a.skip_done = 0;
             a.x = w->P;
             a.y = w->V;
@@ -220,6 +237,7 @@ int block_tune(hbs_solver s, BlockWork *w, hipStream_t q)
     if (rc != HBS_OK) return rc;
     w->tile = bt;
     w->rb = brb;
+    w->v16 = block_v16(w->k, bt);
     w->ngroups = block_groups(w->n, bt);
     w->sgroups = bsg;
     return HBS_OK;
@@ -337,8 +355,8 @@ extern "C" int hbs_block_tuning(hbs_solver s, int k, int *tile, int *rows_per_st
     const BlockWork *w = s->blk;
     if (!w || w->k != k || w->tile == 0) return fail(HBS_ERR_ARG, "no block solve with k = %d tuned yet", k);
     *tile = w->tile;
-    *rows_per_step = w->rb;
-    *rows_per_wave = (64 / pow2_ceil(w->tile)) * w->rb;
+    *rows_per_step = w->v16 ? 2 : w->rb;
+    *rows_per_wave = w->v16 ? 2 : (64 / pow2_ceil(w->tile)) * w->rb;
     return HBS_OK;
 }
 

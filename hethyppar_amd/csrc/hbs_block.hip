@@ -53,6 +53,23 @@ __device__ __forceinline__ RowSweep row_sweep(const RowRuns &rr, int n, int ngro
     return RowSweep{p0 + wi * tile, p1, per * tile};
 }
 
+// Column sums of the waves' column partials red[d][w][c] -> part[d][c][p],
+// waves 0..3 in order.
+template <int ND>
+__device__ __forceinline__ void waves_to_partials(const BlockArgs &a, const double *red, int p)
+{
+    __syncthreads();
+    if ((int)threadIdx.x < a.kc) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double t = red[(d * kWaves) * 64 + threadIdx.x];
+#pragma unroll
+            for (int k = 1; k < kWaves; ++k) t += red[(d * kWaves + k) * 64 + threadIdx.x];
+            a.part[((size_t)d * kBlockCols + threadIdx.x) * a.ngroups + p] = t;
+        }
+    }
+}
+
 // Column sums of ND per-lane values -> part[d][c][p], fixed order: xor
 // butterfly over the lanes of one column inside a wave, then waves 0..3.
 template <int ND>
@@ -67,25 +84,43 @@ __device__ __forceinline__ void column_partials(const BlockArgs &a, double (&v)[
 #pragma unroll
         for (int d = 0; d < ND; ++d) red[(d * kWaves + w) * 64 + l] = v[d];
     }
-    __syncthreads();
-    if ((int)threadIdx.x < a.kc) {
+    waves_to_partials<ND>(a, red, p);
+}
+
+// column_partials for lanes holding two columns each (2 (l % 32) and
+// 2 (l % 32) + 1, the 16-byte form): lanes l and l + 32 first, then waves
+// 0..3.
+template <int ND>
+__device__ __forceinline__ void column_partials2(const BlockArgs &a, double (&v)[ND][2], int p)
+{
+    __shared__ double red[ND * kWaves * 64];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            double t = red[(d * kWaves) * 64 + threadIdx.x];
+    for (int d = 0; d < ND; ++d)
 #pragma unroll
-            for (int k = 1; k < kWaves; ++k) t += red[(d * kWaves + k) * 64 + threadIdx.x];
-            a.part[((size_t)d * kBlockCols + threadIdx.x) * a.ngroups + p] = t;
-        }
+        for (int e = 0; e < 2; ++e) v[d][e] += __shfl_xor(v[d][e], 32, 64);
+    if (l < 32) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) red[(d * kWaves + w) * 64 + 2 * l + e] = v[d][e];
     }
+    waves_to_partials<ND>(a, red, p);
+}
+
+// Whether column c takes part in the pass.
+__device__ __forceinline__ bool col_active(const BlockArgs &a, int c, bool need_not_sconv)
+{
+    if (c >= a.kc) return false;
+    if (!a.skip_done) return true;
+    const SolverState &s = a.st[c];
+    return !s.done && !(need_not_sconv && s.s_conv);
 }
 
 // Whether this lane's column takes part in the pass.
 __device__ __forceinline__ bool column_active(const BlockArgs &a, const LaneMap &m, bool need_not_sconv)
 {
-    if (!m.col_ok) return false;
-    if (!a.skip_done) return true;
-    const SolverState &s = a.st[m.c];
-    return !s.done && !(need_not_sconv && s.s_conv);
+    return m.col_ok && col_active(a, m.c, need_not_sconv);
 }
 
 // WIDE (kp == 64): one row per wave step; otherwise 64 / kp rows per wave.
@@ -176,6 +211,84 @@ __global__ __launch_bounds__(kThreads) void k_bspmm(BlockArgs a, const int *__re
     column_partials<ND>(a, v, p);
 }
 
+// 64-column groups with X gathered in 16-byte pieces (wave_sweep64v): a
+// wave step is two consecutive rows, lane l on row l / 32 and columns
+// cc = 2 (l % 32), cc + 1, so one gather instruction covers an entry of
+// both rows (2 x 512 bytes) and a step issues half the gathers of the
+// one-column-per-lane form.  Each column's sum runs over its row in CSR
+// order, as there; the epilogue operand's two columns load with the step's
+// gathers.  Needs kc, ld and the group's first column even (16-byte
+// aligned rows of x, y and w).
+template <int EPI>
+__global__ __launch_bounds__(kThreads) void k_bspmm2(BlockArgs a, const int *__restrict__ rowptr,
+                                                    const int *__restrict__ col, const double *__restrict__ val)
+{
+    constexpr int ND = EPI == BEPI_DOT1 ? 1 : 2;
+    using Pair = Cols16<double>;
+    const int lane = threadIdx.x & 63;
+    const int cc = (lane & 31) * 2;
+    const bool act0 = col_active(a, cc, EPI == BEPI_DOT2), act1 = col_active(a, cc + 1, EPI == BEPI_DOT2);
+    double v[ND][2];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) v[d][0] = v[d][1] = 0.0;
+    const bool any = __syncthreads_or(act0 || act1);
+    const int xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, per = a.ngroups / kXcds;
+    const int nw = per * kWaves;
+    const int wv = __builtin_amdgcn_readfirstlane(wi * kWaves + (int)(threadIdx.x >> 6));
+    const SweepCtr sc =
+        sweep_ctr(a.rr.ctr ? a.rr.ctr + xcd * kCtrSet * kCtrStride : nullptr, wv, nw, a.rr.ctr_per_xcd);
+    if (any) {
+        int x0, x1;
+        xcd_positions(a.rr, a.n, xcd, x0, x1);
+        int p0, ps;
+        sweep_span(sc, wv, nw, x0, 2, p0, ps);
+        // lanes past the group's columns read its first two (in bounds)
+        const int wc = cc < a.kc ? cc : 0;
+        Pair wr;
+        wave_sweep64v<double>(
+            rowptr, col, val, a.x, a.ld, a.kc, a.rr, p0, x1, ps, sc.mine,
+            [&](int ru, int nr) {
+                const int r = ru + min(lane >> 5, max(nr - 1, 0));
+                wr = *reinterpret_cast<const Pair *>(a.w + (size_t)r * a.ld + wc);
+            },
+            [&](int, int row, const Pair &acc, int) {
+                Pair o;
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const bool on = e == 0 ? act0 : act1;
+                    if constexpr (EPI == BEPI_DOT1) {
+                        o.v[e] = acc.v[e];
+                        v[0][e] += on ? wr.v[e] * o.v[e] : 0.0;
+                    } else if constexpr (EPI == BEPI_DOT2) {
+                        o.v[e] = acc.v[e];
+                        v[0][e] += on ? o.v[e] * wr.v[e] : 0.0;
+                        v[1][e] += on ? o.v[e] * o.v[e] : 0.0;
+                    } else {
+                        o.v[e] = wr.v[e] - acc.v[e];
+                        v[0][e] += on ? o.v[e] * o.v[e] : 0.0;
+                        v[1][e] += on ? o.v[e] * o.v[e] : 0.0;
+                    }
+                }
+                // y stored non-temporally (read back only by the next kernel)
+                double *yp = a.y + (size_t)row * a.ld + cc;
+                if (act0 && act1) {
+                    using V2 = double __attribute__((ext_vector_type(2)));
+                    V2 t;
+                    t[0] = o.v[0];
+                    t[1] = o.v[1];
+                    __builtin_nontemporal_store(t, reinterpret_cast<V2 *>(yp));
+                } else {
+                    if (act0) __builtin_nontemporal_store(o.v[0], yp);
+                    if (act1) __builtin_nontemporal_store(o.v[1], yp + 1);
+                }
+            });
+    }
+    // every wave arrives at the XCD's step counter once, also when no
+    // column is active
+    sweep_done(sc, nw);
+    column_partials2<ND>(a, v, blockIdx.x);
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_bvec(BlockArgs a)
 {
@@ -260,6 +373,10 @@ void bspmm_launch(const BlockArgs &a, hipStream_t s)
 template <int EPI>
 hipError_t bspmm_rb(const BlockArgs &a, hipStream_t s)
 {
+    if (a.v16) {
+        k_bspmm2<EPI><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
+        return hipGetLastError();
+    }
     switch (a.rb) {
     case 1: bspmm_launch<EPI, 1>(a, s); break;
     case 2: bspmm_launch<EPI, 2>(a, s); break;
@@ -275,6 +392,9 @@ hipError_t launch_block_spmm(const BlockArgs &a, int epi, hipStream_t s)
 {
     if (a.ngroups <= 0) return hipSuccess;
     if (a.rr.run && ((long long)a.rr.nruns * a.rr.len != a.n || a.rr.len % 2 != 0 || a.rr.len % a.rb != 0))
+        return hipErrorInvalidValue;
+    if (a.v16 && (a.kp != 64 || a.kc % 2 != 0 || a.ld % 2 != 0 || (uintptr_t)a.x % 16 != 0 ||
+                  (uintptr_t)a.y % 16 != 0 || (uintptr_t)a.w % 16 != 0))
         return hipErrorInvalidValue;
     switch (epi) {
     case BEPI_DOT1: return bspmm_rb<BEPI_DOT1>(a, s);

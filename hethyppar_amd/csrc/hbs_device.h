@@ -390,14 +390,15 @@ struct alignas(16) Cols16 {
 // the vector-memory pipeline, not HBM, bounds the scalar-lane form.  Needs
 // ld * sizeof(T), ncols * sizeof(T) (the tile's columns, ncols <= 64) and
 // xb, the output, 16-byte aligned.  Each column's sums are the ordered sums
-// of row_dot.  out(q, row, acc, cc) receives the lane's N column sums
-// (columns cc .. cc + N - 1 of the tile) of row row = ru + q (q = its row in
-// the step).
-template <typename T, typename Out>
+// of row_dot.  before(ru, nr) runs ahead of a step's gathers (its first row
+// and row count), for loads the step's epilogue uses.  out(q, row, acc, cc)
+// receives the lane's N column sums (columns cc .. cc + N - 1 of the tile)
+// of row row = ru + q (q = its row in the step).
+template <typename T, typename Before, typename Out>
 __device__ __forceinline__ void wave_sweep64v(const int *__restrict__ rowptr, const int *__restrict__ col,
                                               const T *__restrict__ val, const T *__restrict__ xb, int ld,
                                               int ncols, const RowRuns &rr, int p0, int p1, int pstride,
-                                              int *ctr, Out &&out)
+                                              int *ctr, Before &&before, Out &&out)
 {
     constexpr int C = kRowPad;
     constexpr int N = Cols16<T>::N;
@@ -473,6 +474,7 @@ __device__ __forceinline__ void wave_sweep64v(const int *__restrict__ rowptr, co
         const int vrp2 = load_rp(ru2, nr2);
         const int pn3 = pos_of(tk3);
         const int bn3 = run_of(pn3);
+        before(ru, nr);
         Cols16<T> xv[C];
         T ev[C];
 #pragma unroll

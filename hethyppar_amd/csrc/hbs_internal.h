@@ -265,6 +265,7 @@ struct BlockArgs {
     double *part;                 // [kMaxDots][kBlockCols][ngroups]
     RowRuns rr;                   // SpMM row traversal
     int rb;                       // SpMM rows per lane step (1, 2, 4)
+    int v16;                      // 64-column SpMM with 16-byte X gathers (k_bspmm2; rb unused)
 };
 enum BlockEpi : int {
     BEPI_DOT1 = 0,    // y = A x ; d0 = (w, y)

@@ -191,7 +191,10 @@ int hbs_bicgstab_block_f64(hbs_solver s, int k, const double *B_dev, double *X_d
  * of each candidate on the workspace.  hbs_set_block_tuning forces either
  * (0: tuned) for workspaces created afterwards; hbs_block_tuning reports the
  * choice for k: tile, rows per lane step, and the rows a wavefront works on
- * at once ((64 / tile) * rows per step).  No reference counterpart. */
+ * at once ((64 / tile) * rows per step).  64-column tiles with k even, wide
+ * loads on (hbs_set_spmm_tuning) and the rows per step not forced gather X
+ * in 16-byte pieces, two columns per lane and two rows per step (reported
+ * as 2, 2).  No reference counterpart. */
 int hbs_set_block_tuning(int tile, int rows_per_step);
 int hbs_block_tuning(hbs_solver s, int k, int *tile, int *rows_per_step, int *rows_per_wave);
 /* The tuner also times the SpMMs with the streaming kernels' workgroup
@@ -476,7 +479,8 @@ int hbs_plan_grid_runs(int m, const int *rowptr, const int *col, int *runs, int 
  * waves take the next step from one of `dynamic` counters (1..16, default
  * 16), so they work on neighbouring rows; 0: a fixed stride per wave) and whether hbs_csr_spmm_* gathers X in
  * 16-byte pieces where k * element size and X, Y are 16-byte aligned (1,
- * default) or one element per lane (0).  The multi-column SpMMs of one
+ * default) or one element per lane (0); the block solve's 64-column SpMMs
+ * follow the same switch for workspaces created afterwards.  The multi-column SpMMs of one
  * matrix (hbs_csr_spmm_*, the block solve) must not overlap in time. */
 int hbs_set_spmm_tuning(int slab_lines, int groups, int dynamic, int wide_loads);
 /* Runs of a device CSR's SpMM traversal (*nruns == 0: rows in order). */

@@ -169,6 +169,54 @@ def test_block_forced_tiles_match_oracle(ctx, tile, rb):
     assert len(loose) <= MAX_ORDER_SENSITIVE * k
 
 
+@pytest.mark.parametrize("k", [64, 40, 100, 80])
+@pytest.mark.parametrize("wide", [1, 0])
+def test_block_wide_gathers_match_oracle(ctx, k, wide):
+    """64-column groups with X gathered in 16-byte pieces (two columns per
+    lane, two rows per step; k = 40: lanes past the group's columns; 100:
+    a second group of 36 columns; 80: a 16-column remainder group in the
+    one-column-per-lane form) and, with wide loads off, the one-column
+    form: each column gets the oracle's solve.  The columns stop at
+    different iterations, and a zero column is done at once, so a lane's
+    two columns are not always both active."""
+    rp, c, v = hbs.gen_convdiff2d(40, 36, 0.6)
+    n = rp.size - 1
+    Xt = hbs.gen_uniform(n * k, -1.0, 1.0, 11 + k).reshape(n, k)
+    B = np.stack([hbs.host_spmv(rp, c, v, np.ascontiguousarray(Xt[:, j])) for j in range(k)], axis=1)
+    B[:, 5] = 0.0
+    A = hbs.DeviceCSR(ctx, rp, c, v)
+    hbs.set_block_tuning(64, 0)
+    hbs.set_spmm_tuning(0, 0, 16, wide)
+    try:
+        S = hbs.Solver(A)
+        dB = ctx.to_device(np.ascontiguousarray(B).ravel())
+        dX = ctx.to_device(np.zeros(n * k))
+        infos = S.bicgstab_block(dB, dX, k, tol=1e-8, maxit=1000)
+        tile, rb, rpw = S.block_tuning(k)
+        assert tile == 64
+        if wide:
+            assert (rb, rpw) == (2, 2)
+        else:
+            assert rb in (1, 2, 4) and rpw == rb
+    finally:
+        hbs.set_block_tuning(0, 0)
+        hbs.set_spmm_tuning()
+    X = dX.download().reshape(n, k)
+    Ao = orc.CSR(rp, c, v)
+    assert infos[5].iters == 0 and infos[5].stop == 2 and np.all(X[:, 5] == 0.0)
+    loose = []
+    for j in range(k):
+        if j == 5:
+            continue
+        bj = np.ascontiguousarray(B[:, j])
+        xo, ito, _, _ = orc.bicgstab(Ao, bj, tol=1e-8, maxit=1000)
+        if _check(X[:, j], infos[j], xo, ito, f"k {k} wide {wide} column {j}", Ao, bj,
+                  np.ascontiguousarray(Xt[:, j])) != "plain":
+            loose.append(j)
+    print(f"order-sensitive bar used by columns {loose} of {k}", flush=True)
+    assert len(loose) <= MAX_ORDER_SENSITIVE * k
+
+
 @pytest.mark.parametrize("sgroups", [512, 1024])
 def test_block_spmm_groups_forced_match_oracle(ctx, sgroups):
     """The block solve's SpMMs on fewer workgroups than its streaming
