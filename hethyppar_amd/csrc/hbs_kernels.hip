@@ -751,14 +751,33 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_chunks(ChunkArgs<T> a)
 {
     __shared__ T red[kWaves];
+    static_assert(kChunkLen <= 4 * kThreads, "a chunk is at most four strides");
     const Chunk c = a.chunks[blockIdx.x];
     if (a.st && a.st[c.sys].done) return;
-    T acc = T(0);
+    // thread t adds the terms t, t + 256, ... of the chunk in turn; the
+    // four strides' terms load at once (indices clamped into the chunk), so
+    // the sum waits for one round trip, not four
+    const int j = c.j0 + (int)threadIdx.x;
+    T t[4];
     if (a.prod) {
-        for (int j = c.j0 + (int)threadIdx.x; j < c.j1; j += kThreads) acc += a.prod[j];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] = a.prod[min(j + k * kThreads, c.j1 - 1)];
     } else {
-        for (int j = c.j0 + (int)threadIdx.x; j < c.j1; j += kThreads) acc += a.val[j] * a.x[a.col[j]];
+        int cc[4];
+        T vv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int jk = min(j + k * kThreads, c.j1 - 1);
+            cc[k] = a.col[jk];
+            vv[k] = a.val[jk];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] = vv[k] * a.x[cc[k]];
     }
+    T acc = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (j + k * kThreads < c.j1) acc += t[k];
     acc = block_sum1<T>(acc, red);
     if (threadIdx.x == 0) a.chunk_sums[blockIdx.x] = (double)acc;
 }

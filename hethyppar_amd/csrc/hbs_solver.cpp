@@ -547,6 +547,8 @@ int iterate(hbs_solver s, const Work<T> &w, int maxit, int check_every, bool use
                 const double need = std::log(h->thr * h->thr / h->rr) / per;   // iterations after h
                 const double left = need - (nfly ? fly_len[0] : 0);
                 if (left <= 0.5) ahead = false;
+                else if (us_per_iter >= kLongIterUs && left <= k)
+                    next = (int)std::max(1.0, std::ceil(0.5 * left));   // the end in halves
                 else next = (int)std::min<double>(k, std::max(1.0, std::ceil(left)));
             }
             rr_prev = h->rr;
@@ -596,17 +598,6 @@ hipError_t residual_b(hbs_solver s, const double *x, const double *b, double *r,
     hipError_t e = run_spmv(sa, s->A, EPI_RESIDB, nullptr, q);
     if (e != hipSuccess) return e;
     return launch_finalize(s->slot(k), s->sys, s->nsys, s->partE, s->A->plan.ngroups, 1, FIN_RESIDB, q);
-}
-
-// (b, b) into out of slot k (every system).
-hipError_t bnorm2(hbs_solver s, const double *b, int k)
-{
-    hipStream_t q = s->A->ctx->stream;
-    VecArgs<double> va = vec_args<double>(s, work64(s));
-    va.part_out = s->partF;
-    hipError_t e = launch_dot(va, b, b, q);
-    if (e != hipSuccess) return e;
-    return launch_finalize(s->slot(k), s->sys, s->nsys, s->partF, s->nvb, 0, FIN_STORE, q);
 }
 
 void fill_info(const SolverState &hs, bool fixed, float ms, hbs_solve_info *info)
@@ -801,7 +792,7 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
     SolverState *outer = s->slot(kSlotOuter);
     // The host reads states from the host-mapped mirror: read buffer 3 the
     // sweep's inner state, 4 the outer one, each stamped after a fence.
-    SolverState *const m_inner = s->hst_dev + 3, *const m_outer = s->hst_dev + 4;
+    SolverState *const m_inner = s->hst_dev + 3;
     const volatile SolverState *const h_inner = s->hst + 3, *const h_outer = s->hst + 4;
     // the iterate alternates between X and Xb: a sweep reads one and writes
     // the other, so its update can be queued again after a misprediction
@@ -810,24 +801,21 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
     HIPCHK(hipEventRecord(s->e0, q));
     HIPCHK(hipMemcpyAsync(xc, x, sizeof(double) * n, hipMemcpyDeviceToDevice, q));
     HIPCHK(launch_state_init(outer, 1, o.tol, 0, 0, q));
-    // FP64 outer residual r = b - A x in R, ||b||^2
-    HIPCHK(residual(s, xc, b, s->R, kSlotOuter));
-    HIPCHK(bnorm2(s, b, kSlotOuter));
-    int stamp = next_stamp(s);
-    HIPCHK(launch_mirror(outer, nullptr, 1, m_outer, stamp, q));
-    CHK(wait_mirror(s, 4, 1, stamp));
+    // FP64 outer residual r = b - A x in R with ||r||^2 and ||b||^2 (one
+    // pass).  The host does not wait for them: the first sweep's inner
+    // tolerance does not depend on ||r0||, and the first sweep's state
+    // carries ||r0|| (its scale) to the host with its result.
+    HIPCHK(residual_b(s, xc, b, s->R, kSlotOuter));
+    int stamp = 0;
     SolverState hs;
-    std::memcpy(&hs, (const void *)h_outer, sizeof hs);
-    const double r0 = std::sqrt(hs.resid2);
-    const double bnorm2v = hs.out;
-    const double thr = o.tol * r0;
-    double rnorm = r0;
+    double r0 = 0.0, bnorm2v = 0.0, thr = 0.0, rnorm = 0.0;
     int total_iters = 0, sweeps = 0, stop = 2;
-    bool converged = !(r0 > 0.0);
+    bool converged = false;
     bool breakdown = false;
     while (!converged && sweeps < o.max_sweeps && total_iters < o.maxit) {
         // inner tolerance: the requested one, or just what is still missing
-        double itol = std::max(o.inner_tol, 0.5 * thr / rnorm);
+        // (0.5 thr / ||r||, which is 0.5 tol for the first sweep)
+        double itol = std::max(o.inner_tol, sweeps == 0 ? 0.5 * o.tol : 0.5 * thr / rnorm);
         if (itol > 0.5) itol = 0.5;
         const int imax = std::max(1, std::min(o.inner_maxit, o.maxit - total_iters));
         HIPCHK(launch_state_init(s->st, 2, itol, imax, 0, q));
@@ -853,6 +841,20 @@ extern "C" int hbs_bicgstab_mixed(hbs_solver s, const double *b, double *x, cons
         SolverState hi;
         std::memcpy(&hi, (const void *)h_inner, sizeof hi);
         std::memcpy(&hs, (const void *)h_outer, sizeof hs);
+        if (sweeps == 0) {
+            // ||r0|| as the first sweep's start scaled by it, ||b||^2 as
+            // the outer residual left it
+            r0 = hi.scale;
+            bnorm2v = hs.out;
+            thr = o.tol * r0;
+            if (!(r0 > 0.0)) {
+                // x solves the system already: the sweep (on r = 0) is
+                // discarded and x stays as it came
+                rnorm = r0;
+                converged = true;
+                break;
+            }
+        }
         std::swap(xc, xn);
         total_iters += std::max(hi.iter, 0);
         ++sweeps;

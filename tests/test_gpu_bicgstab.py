@@ -114,6 +114,27 @@ def test_zero_rhs(ctx):
     assert info.iters == 0 and np.all(x == 0)
 
 
+@pytest.mark.parametrize("x0", ["zero", "nonzero"])
+def test_mixed_zero_residual_start(ctx, x0):
+    """Mixed precision with r0 = b - A x0 = 0 (b = 0 and x0 = 0): no sweep
+    counts, x comes back as it went in, status OK.  The first sweep is
+    queued before ||r0|| reaches the host and is then discarded.  With a
+    nonzero x0 and b = 0 the solve runs and drives x to 0."""
+    rp, c, v, xt, b = convdiff64()
+    z = np.zeros_like(b)
+    if x0 == "zero":
+        x, info = _solve(ctx, rp, c, v, z, mixed=True, tol=1e-8, maxit=1000)
+        assert info.status == 0 and info.iters == 0 and info.sweeps == 0 and info.stop == 2
+        assert np.all(x == 0)
+    else:
+        x, info = _solve(ctx, rp, c, v, z, x0=xt, mixed=True, tol=1e-8, maxit=4000)
+        assert info.status == 0 and info.sweeps >= 1 and info.iters > 0
+        # relres is relative to ||b|| = 0: it reports ||r|| itself
+        assert info.rel_r0 <= 1.01e-8
+        r0 = np.linalg.norm(hbs.host_spmv(rp, c, v, xt))
+        assert np.linalg.norm(hbs.host_spmv(rp, c, v, x)) <= 1.02e-8 * r0
+
+
 def test_direct_iterate_in_caller_x_matches_graphs(ctx):
     """From 4M entries the default launch mode issues kernels one by one and
     iterates in the caller's x when it is 16-byte aligned and not b.  A
