@@ -238,6 +238,34 @@ def test_spmm_grid_runs_bit_exact(ctx, k, tile):
     assert np.array_equal(dY32.download(), dZ32.download())
 
 
+@pytest.mark.parametrize("k", [64, 40])
+def test_spmm_long_rows_bit_exact(ctx, k):
+    """Power-law rows (up to 3000 entries, rows in order, n not a multiple
+    of a step's rows): the 64-column SpMM's lanes hold two rows each, a
+    step's rows of very different lengths, every column in CSR order."""
+    rp, c, v = hbs.gen_circuit(20003, nheavy=8, heavy_lo=300, heavy_hi=3000, seed=19)
+    n = rp.size - 1
+    X = np.random.default_rng(k + 5).uniform(-1, 1, (n, k))
+    A = hbs.DeviceCSR(ctx, rp, c, v, flags=hbs.HBS_CSR_F32)
+    Ao = orc.CSR(rp, c, v)
+    dX, dY = ctx.to_device(X.ravel()), ctx.empty(n * k)
+    A.spmm(dX, dY, k, 64)
+    Y = dY.download().reshape(n, k)
+    for j in range(k):
+        assert np.array_equal(Y[:, j], orc.spmv(Ao, X[:, j])), f"column {j}"
+    X32 = X.astype(np.float32)
+    dX32, dY32 = ctx.to_device(X32.ravel()), ctx.empty(n * k, np.float32)
+    A.spmm(dX32, dY32, k, 64)
+    Y32 = dY32.download().reshape(n, k).astype(np.float64)
+    kk = np.diff(rp)
+    u = np.finfo(np.float32).eps / 2
+    bound = (2 * kk * u / (1 - kk * u)) * orc.spmv(orc.CSR(rp, c, np.abs(v)), np.ones(n))
+    v32 = v.astype(np.float32).astype(np.float64)
+    for j in (0, k // 2, k - 1):
+        ref = orc.spmv(orc.CSR(rp, c, v32), X32[:, j].astype(np.float64))
+        assert np.all(np.abs(Y32[:, j] - ref) <= 1.01 * bound + 1e-30), f"FP32 column {j}"
+
+
 def test_spmm_f32(ctx):
     rp, c, v, xt, b = convdiff64()
     n, k = rp.size - 1, 32
