@@ -108,10 +108,8 @@ __global__ __launch_bounds__(kThreads) void k_spmm(int m, const int *__restrict_
 // is read once per wave and every gather is a contiguous 64-column stretch
 // of an X row.  Each XCD sweeps its share of the positions in steps of RB
 // consecutive rows (a run's length is even, so a step never leaves its
-// run), taken from the tile's step counter where it has one.  With 16-byte
-// gathers each lane takes RQ rows of a step (RQ = 2: twice the gathers in
-// flight per wave; FP64 128^3 x 64: 0.75 -> 0.73 ms, FP32 0.40 -> 0.38 ms).
-template <typename T, bool V16, int RQ = 1>
+// run), taken from the tile's step counter where it has one.
+template <typename T, bool V16>
 __global__ __launch_bounds__(kThreads) void k_spmm64(int m, const int *__restrict__ rowptr,
                                                     const int *__restrict__ col, const T *__restrict__ val,
                                                     int k, int ld, const T *__restrict__ X, T *__restrict__ Y,
@@ -132,10 +130,10 @@ __global__ __launch_bounds__(kThreads) void k_spmm64(int m, const int *__restric
     int p0, ps;
     if constexpr (V16) {
         // 16-byte gathers: a step of G rows, columns in 16-byte pieces
-        constexpr int G = RQ * 64 / (64 / Cols16<T>::N);
+        constexpr int G = 64 / (64 / Cols16<T>::N);
         sweep_span(sc, wv, nw, x0, G, p0, ps);
         const int cbase = (int)blockIdx.y * 64;
-        wave_sweep64v<T, RQ>(rowptr, col, val, X + cbase, ld, min(64, k - cbase), rr, p0, x1, ps, sc.mine,
+        wave_sweep64v<T>(rowptr, col, val, X + cbase, ld, min(64, k - cbase), rr, p0, x1, ps, sc.mine,
                          [](int, int) {}, [&](int, int row, const Cols16<T> &acc, int cc) {
                              // Y stored non-temporally, out of the L2 the
                              // gathered X rows share (FP64 128^3 x 64: 0.81
@@ -289,10 +287,7 @@ hipError_t spmm_dispatch(int m, const int *rowptr, const int *col, const T *val,
     gx = std::max(8LL, (gx + 7) / 8 * 8);
     dim3 grid((unsigned)gx, (unsigned)ytiles);
     if (rr.run && ((long long)rr.nruns * rr.len != m || rr.len % 2 != 0)) return hipErrorInvalidValue;
-    // two rows per lane where the runs hold whole steps of them
-    const bool rq2 = !rr.run || rr.len % (2 * Cols16<T>::N) == 0;
-    if (tile == 64 && v16 && rq2) k_spmm64<T, true, 2><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, rr);
-    else if (tile == 64 && v16) k_spmm64<T, true><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, rr);
+    if (tile == 64 && v16) k_spmm64<T, true><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, rr);
     else if (tile == 64) k_spmm64<T, false><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, rr);
     else k_spmm<T><<<grid, kThreads, 0, s>>>(m, rowptr, col, val, k, k, X, Y, tile, rr);
     return hipGetLastError();

@@ -394,7 +394,7 @@ struct alignas(16) Cols16 {
 // and row count), for loads the step's epilogue uses.  out(q, row, acc, cc)
 // receives the lane's N column sums (columns cc .. cc + N - 1 of the tile)
 // of row row = ru + q (q = its row in the step).
-template <typename T, int RQ = 1, typename Before, typename Out>
+template <typename T, typename Before, typename Out>
 __device__ __forceinline__ void wave_sweep64v(const int *__restrict__ rowptr, const int *__restrict__ col,
                                               const T *__restrict__ val, const T *__restrict__ xb, int ld,
                                               int ncols, const RowRuns &rr, int p0, int p1, int pstride,
@@ -403,11 +403,10 @@ __device__ __forceinline__ void wave_sweep64v(const int *__restrict__ rowptr, co
     constexpr int C = kRowPad;
     constexpr int N = Cols16<T>::N;
     constexpr int LPR = 64 / N;
-    constexpr int RL = 64 / LPR;               // rows side by side in the wave
-    constexpr int RB = RL * RQ;                // rows per step: lane rows lq, lq + RL, ...
+    constexpr int RB = 64 / LPR;
     static_assert(RB * C <= 64, "one entry per lane");
     const int lane = (int)threadIdx.x & 63;
-    const int lq = lane / LPR;                 // this lane's first row of the step
+    const int lq = lane / LPR;                 // this lane's row of the step
     const int cc = (lane % LPR) * N;           // and its first column
     // lanes past the tile's ncols columns gather its first columns (in bounds)
     const T *__restrict__ xl = xb + (cc < ncols ? cc : 0);
@@ -436,17 +435,6 @@ __device__ __forceinline__ void wave_sweep64v(const int *__restrict__ rowptr, co
         vc = col[j];
         vv = val[j];
     };
-    // entry k of the step's entries (lane k of vc, vv), made uniform
-    auto entry = [&](int vc, T vv, int k, int &cj, T &e) {
-        cj = __builtin_amdgcn_readlane(vc, k);
-        if constexpr (sizeof(T) == 8) {
-            const long long bits = __builtin_bit_cast(long long, vv);
-            e = __builtin_bit_cast(T, ((long long)__builtin_amdgcn_readlane((int)(bits >> 32), k) << 32) |
-                                          (unsigned)__builtin_amdgcn_readlane((int)bits, k));
-        } else {
-            e = __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vv), k));
-        }
-    };
     int p = pos_of(take()), pn = pos_of(take()), pn2 = pos_of(take());
     int tk3 = take();
     int b = run_of(p), bn = run_of(pn), bn2 = run_of(pn2);
@@ -460,21 +448,15 @@ __device__ __forceinline__ void wave_sweep64v(const int *__restrict__ rowptr, co
     int vrp1 = load_rp(ru1, nr1);
     int tk4 = take();
     while (p < p1) {
-        // this lane's rows: their entries' start and count
-        int rpq[RQ], len[RQ];
+        // this lane's row: its entries' start and count
+        int rpq = __builtin_amdgcn_readlane(vrp, 0), rpe = __builtin_amdgcn_readlane(vrp, 1);
 #pragma unroll
-        for (int u = 0; u < RQ; ++u) {
-            int a0 = __builtin_amdgcn_readlane(vrp, u * RL), a1 = __builtin_amdgcn_readlane(vrp, u * RL + 1);
-#pragma unroll
-            for (int q = 1; q < RL; ++q) {
-                const int b0 = __builtin_amdgcn_readlane(vrp, u * RL + q);
-                const int b1 = __builtin_amdgcn_readlane(vrp, u * RL + q + 1);
-                a0 = lq == q ? b0 : a0;
-                a1 = lq == q ? b1 : a1;
-            }
-            rpq[u] = a0;
-            len[u] = u * RL + lq < nr ? a1 - a0 : 0;
+        for (int q = 1; q < RB; ++q) {
+            const int a0 = __builtin_amdgcn_readlane(vrp, q), a1 = __builtin_amdgcn_readlane(vrp, q + 1);
+            rpq = lq == q ? a0 : rpq;
+            rpe = lq == q ? a1 : rpe;
         }
+        const int len = lq < nr ? rpe - rpq : 0;
         // the step's longest row (wave-uniform): gathers past it are not
         // issued (a 7-point row leaves the 8th of a chunk to padding), since
         // the vector-memory pipeline, not HBM, bounds this sweep
@@ -493,63 +475,68 @@ __device__ __forceinline__ void wave_sweep64v(const int *__restrict__ rowptr, co
         const int pn3 = pos_of(tk3);
         const int bn3 = run_of(pn3);
         before(ru, nr);
-        // every gather of the step issued before the first sum
-        Cols16<T> xv[RQ][C];
-        T ev[RQ][C];
+        Cols16<T> xv[C];
+        T ev[C];
 #pragma unroll
         for (int i = 0; i < C; ++i) {
-#pragma unroll
-            for (int u = 0; u < RQ; ++u) {
-                int cj;
-                T e;
-                entry(vc, vv, u * RL * C + i, cj, e);
-#pragma unroll
-                for (int q = 1; q < RL; ++q) {
-                    int cq;
-                    T eq;
-                    entry(vc, vv, (u * RL + q) * C + i, cq, eq);
-                    cj = lq == q ? cq : cj;
-                    e = lq == q ? eq : e;
-                }
-                if (i < lmax) xv[u][i] = *reinterpret_cast<const Cols16<T> *>(xl + (size_t)cj * ld);
-                else
-#pragma unroll
-                    for (int w = 0; w < N; ++w) xv[u][i].v[w] = T(0);
-                ev[u][i] = e;
+            int cj = __builtin_amdgcn_readlane(vc, i);
+            T e = T(0);
+            if constexpr (sizeof(T) == 8) {
+                const long long bits = __builtin_bit_cast(long long, vv);
+                e = __builtin_bit_cast(T, ((long long)__builtin_amdgcn_readlane((int)(bits >> 32), i) << 32) |
+                                              (unsigned)__builtin_amdgcn_readlane((int)bits, i));
+            } else {
+                e = __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vv), i));
             }
+#pragma unroll
+            for (int q = 1; q < RB; ++q) {
+                const int cq = __builtin_amdgcn_readlane(vc, q * C + i);
+                T eq;
+                if constexpr (sizeof(T) == 8) {
+                    const long long bits = __builtin_bit_cast(long long, vv);
+                    eq = __builtin_bit_cast(
+                        T, ((long long)__builtin_amdgcn_readlane((int)(bits >> 32), q * C + i) << 32) |
+                               (unsigned)__builtin_amdgcn_readlane((int)bits, q * C + i));
+                } else {
+                    eq = __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vv), q * C + i));
+                }
+                cj = lq == q ? cq : cj;
+                e = lq == q ? eq : e;
+            }
+            if (i < lmax) xv[i] = *reinterpret_cast<const Cols16<T> *>(xl + (size_t)cj * ld);
+            else
+#pragma unroll
+                for (int u = 0; u < N; ++u) xv[i].v[u] = T(0);
+            ev[i] = e;
         }
         tk3 = tk4;
         tk4 = take();
+        Cols16<T> acc;
 #pragma unroll
-        for (int u = 0; u < RQ; ++u) {
-            Cols16<T> acc;
+        for (int e = 0; e < N; ++e) acc.v[e] = T(0);
 #pragma unroll
-            for (int e = 0; e < N; ++e) acc.v[e] = T(0);
+        for (int i = 0; i < C; ++i)
+            if (i < len) {
+#pragma unroll
+                for (int e = 0; e < N; ++e) acc.v[e] += ev[i] * xv[i].v[e];
+            }
+        // rows longer than kRowPad: the rest of this lane's row, in chunks
+        for (int j0 = rpq + C; j0 < rpq + len; j0 += C) {
+            T cv[C];
+            Cols16<T> cx[C];
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                cv[i] = val[j0 + i];
+                cx[i] = *reinterpret_cast<const Cols16<T> *>(xl + (size_t)col[j0 + i] * ld);
+            }
 #pragma unroll
             for (int i = 0; i < C; ++i)
-                if (i < len[u]) {
+                if (j0 + i < rpq + len) {
 #pragma unroll
-                    for (int e = 0; e < N; ++e) acc.v[e] += ev[u][i] * xv[u][i].v[e];
+                    for (int e = 0; e < N; ++e) acc.v[e] += cv[i] * cx[i].v[e];
                 }
-            // rows longer than kRowPad: the rest of this lane's row, in chunks
-            for (int j0 = rpq[u] + C; j0 < rpq[u] + len[u]; j0 += C) {
-                T cv[C];
-                Cols16<T> cx[C];
-#pragma unroll
-                for (int i = 0; i < C; ++i) {
-                    cv[i] = val[j0 + i];
-                    cx[i] = *reinterpret_cast<const Cols16<T> *>(xl + (size_t)col[j0 + i] * ld);
-                }
-#pragma unroll
-                for (int i = 0; i < C; ++i)
-                    if (j0 + i < rpq[u] + len[u]) {
-#pragma unroll
-                        for (int e = 0; e < N; ++e) acc.v[e] += cv[i] * cx[i].v[e];
-                    }
-            }
-            const int q = u * RL + lq;
-            if (q < nr) out(q, ru + q, acc, cc);
         }
+        if (lq < nr) out(lq, ru + lq, acc, cc);
         p = pn;
         pn = pn2;
         pn2 = pn3;
