@@ -373,7 +373,11 @@ void bspmm_launch(const BlockArgs &a, hipStream_t s)
 template <int EPI>
 hipError_t bspmm_rb(const BlockArgs &a, hipStream_t s)
 {
-    if (a.v16) {
+    // the 16-byte form where every column pair it loads or stores is
+    // 16-byte aligned (a caller's B need not be), else one column per lane
+    const bool v16 = a.v16 && a.kp == 64 && a.kc % 2 == 0 && a.ld % 2 == 0 && (uintptr_t)a.x % 16 == 0 &&
+                     (uintptr_t)a.y % 16 == 0 && (uintptr_t)a.w % 16 == 0;
+    if (v16) {
         k_bspmm2<EPI><<<a.ngroups, kThreads, 0, s>>>(a, a.rowptr, a.col, a.val);
         return hipGetLastError();
     }
@@ -392,9 +396,6 @@ hipError_t launch_block_spmm(const BlockArgs &a, int epi, hipStream_t s)
 {
     if (a.ngroups <= 0) return hipSuccess;
     if (a.rr.run && ((long long)a.rr.nruns * a.rr.len != a.n || a.rr.len % 2 != 0 || a.rr.len % a.rb != 0))
-        return hipErrorInvalidValue;
-    if (a.v16 && (a.kp != 64 || a.kc % 2 != 0 || a.ld % 2 != 0 || (uintptr_t)a.x % 16 != 0 ||
-                  (uintptr_t)a.y % 16 != 0 || (uintptr_t)a.w % 16 != 0))
         return hipErrorInvalidValue;
     switch (epi) {
     case BEPI_DOT1: return bspmm_rb<BEPI_DOT1>(a, s);
