@@ -470,7 +470,9 @@ bool graphs(const hbs_csr a, int use_graph)
 }
 
 // Iterations at least this long (host-measured, between polls) use the
-// optimistic convergence prediction in iterate().
+// optimistic convergence prediction in iterate() and, once the predicted
+// end is within one chunk, chunks of half the predicted rest (an iteration
+// queued past convergence then costs more than a poll).
 constexpr double kLongIterUs = 150.0;
 
 // Queue iteration chunks until the device states say every system is done.
