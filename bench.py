@@ -134,6 +134,8 @@ def main():
                     help="only the circuit solves and SpMVs (for kernel traces)")
     ap.add_argument("--markov", action="store_true",
                     help="only the 445k-row Markov-like solves, SpMV and shard (for kernel traces)")
+    ap.add_argument("--batched", action="store_true",
+                    help="only the 128^3 x 64 block solve and the 256 uneven systems (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -150,6 +152,11 @@ def main():
     if args.markov:
         ctx = hbs.Context(0)
         print(json.dumps({"markov_445k_fp64_cache_resident": extras_markov(hbs, ctx, args)}), flush=True)
+        ctx.close()
+        return
+    if args.batched:
+        ctx = hbs.Context(0)
+        print(json.dumps(extras_batched(hbs, ctx)), flush=True)
         ctx.close()
         return
 
